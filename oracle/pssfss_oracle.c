@@ -1,0 +1,785 @@
+/*
+ * ORACLE — test infrastructure only.  A plain-C FP64 restatement of PSSFSS.jl's
+ * per-frequency MoM hot path, used by tests/ (as the checker) and by bench.py's
+ * cpu_baseline leg.  Nothing in the product library links or calls this file.
+ *
+ * Each function names the reference code it restates (file:line in PSSFSS.jl
+ * v0.1.4).  Floating-point expressions keep the reference's evaluation order and
+ * the file is compiled with -ffp-contract=off, so discrete decisions (table cell
+ * selection, ring-count breaks, small-argument branches) are taken on the same
+ * IEEE values as the reference would compute them.
+ *
+ * Index conventions: all index arrays are 0-based int64; basis/edge "none" is -1.
+ * Complex arrays are interleaved (re, im) doubles, matrices column-major.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+static const double TWOPI = 6.283185307179586;     /* Float64(2pi) */
+static const double FOURPI = 12.566370614359172;   /* Float64(4pi) */
+static const double PI_ = 3.141592653589793;
+static const double MU0 = 1.25663706212e-6;        /* src/Constants.jl:6 */
+static const double C0 = 299792458.0;
+
+/* Dunavant 7-point rule, src/Zint.jl:55-62 and :219-226. */
+static const double QXI[7] = {0.33333333333333330, 0.10128650732345633, 0.79742698535308730,
+                              0.10128650732345633, 0.47014206410511505, 0.05971587178976989,
+                              0.47014206410511505};
+static const double QETA[7] = {0.33333333333333333, 0.79742698535308730, 0.10128650732345633,
+                               0.10128650732345633, 0.05971587178976989, 0.47014206410511505,
+                               0.47014206410511505};
+static const double QW[7] = {0.1125, 0.06296959027241358, 0.06296959027241358,
+                             0.06296959027241358, 0.06619707639425308, 0.06619707639425308,
+                             0.06619707639425308};
+
+static inline cplx mk(double re, double im) { return CMPLX(re, im); }
+static inline cplx cis(double t) { return CMPLX(cos(t), sin(t)); }
+
+/* mysqrt, src/PGF.jl:116-120: principal sqrt, but a pure negative-imaginary
+ * result is flipped to the positive imaginary axis. */
+static inline cplx mysqrt(cplx z) {
+    cplx a = csqrt(z);
+    if (creal(a) == 0.0 && cimag(a) < 0.0) a = -a;
+    return a;
+}
+
+/* Ring(r) element k (0 <= k < 8r, or k = 0 for r = 0), src/Rings.jl:9-63. */
+static inline void ring_mn(int r, int k, int *m, int *n) {
+    if (r == 0) { *m = 0; *n = 0; return; }
+    int side = 2 * r + 1;
+    if (k < side) { *m = r; *n = -r + k; return; }
+    k -= side;
+    if (k < side) { *m = -r; *n = -r + k; return; }
+    k -= side;
+    if (k < 2 * r - 1) { *m = -r + 1 + k; *n = -r; return; }
+    k -= 2 * r - 1;
+    *m = -r + 1 + k; *n = r;
+}
+
+/* ---------------------------------------------------------------- jksums */
+/* src/PGF.jl:67-106. urho00, us1, us2 are u-scaled 2-vectors (sheet units). */
+void orc_jksums(const double *urho00, double psi1, double psi2, const double *us1,
+                const double *us2, int extract, double convtest, double *jsum_out,
+                double *ksum_out, int *rings_out) {
+    const int jkringmax = 65, rmin = 5;
+    const double small = 1e-5;
+    double arg = sqrt(urho00[0] * urho00[0] + urho00[1] * urho00[1]);
+    double rterm = exp(-arg);
+    cplx ksum = rterm, jsum;
+    if (extract) {
+        if (arg <= small) jsum = (-arg / 6 + 0.5) * arg - 1;
+        else jsum = (rterm - 1) / arg;
+    } else {
+        jsum = rterm / arg;
+    }
+    int rsave = 0;
+    for (int r = 1; r <= jkringmax; ++r) {
+        rsave = r;
+        cplx jring = 0, kring = 0;
+        for (int k = 0; k < 8 * r; ++k) {
+            int m, n;
+            ring_mn(r, k, &m, &n);
+            double dx = urho00[0] - ((double)m * us1[0] + (double)n * us2[0]);
+            double dy = urho00[1] - ((double)m * us1[1] + (double)n * us2[1]);
+            double a = sqrt(dx * dx + dy * dy);
+            double ph = (double)m * psi1 + (double)n * psi2;
+            /* exp(-complex(a, ph)) */
+            double ea = exp(-a);
+            cplx term = mk(ea * cos(-ph), ea * sin(-ph));
+            jring += term / a;
+            kring += term;
+        }
+        jsum += jring;
+        ksum += kring;
+        if (r > rmin) {
+            double c1 = cabs(jring / jsum), c2 = cabs(kring / ksum);
+            double conv = c1 > c2 ? c1 : c2;
+            if (conv < convtest) break;
+        }
+    }
+    jsum_out[0] = creal(jsum); jsum_out[1] = cimag(jsum);
+    ksum_out[0] = creal(ksum); ksum_out[1] = cimag(ksum);
+    if (rings_out) *rings_out = rsave;
+}
+
+/* --------------------------------------------------------------- filljk! */
+/* src/Zint.jl:54-179 with closed=true.  rho: (nnode,2) sheet units; fv (3,nface)
+ * column-major; rep[i]: representative face pair ifm + nface*ifs of class i.
+ * Outputs JK: (6, nufp) complex column-major = (J, J_xi, J_eta, K, K_xi, K_eta). */
+void orc_filljk(int64_t nface, const double *rho, const int64_t *fv, int64_t nufp,
+                const int64_t *rep, double ulocal, double psi1, double psi2,
+                const double *s1, const double *s2, double *JK, double *rho_r,
+                double *rinv) {
+    double us1[2] = {ulocal * s1[0], ulocal * s1[1]};
+    double us2[2] = {ulocal * s2[0], ulocal * s2[1]};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t iu = 0; iu < nufp; ++iu) {
+        int64_t fp = rep[iu];
+        int64_t ifm = fp % nface, ifs = fp / nface;
+        double rs[3][2], rm[3][2];
+        for (int i = 0; i < 3; ++i) {
+            rs[i][0] = rho[2 * fv[3 * ifs + i]];
+            rs[i][1] = rho[2 * fv[3 * ifs + i] + 1];
+            rm[i][0] = rho[2 * fv[3 * ifm + i]];
+            rm[i][1] = rho[2 * fv[3 * ifm + i] + 1];
+        }
+        double rs32[2] = {rs[2][0] - rs[1][0], rs[2][1] - rs[1][1]};
+        double rs12[2] = {rs[0][0] - rs[1][0], rs[0][1] - rs[1][1]};
+        double area2 = (-rs32[1]) * rs12[0] + rs32[0] * rs12[1];
+        double unz = 1.0;
+        if (area2 < 0) { area2 = fabs(area2); unz = -1.0; }
+        double rs31[2] = {rs[2][0] - rs[0][0], rs[2][1] - rs[0][1]};
+        double rs21[2] = {-rs12[0], -rs12[1]};
+        double rmc[2] = {((rm[0][0] + rm[1][0]) + rm[2][0]) / 3,
+                         ((rm[0][1] + rm[1][1]) + rm[2][1]) / 3};
+        cplx J = 0, Jx = 0, Je = 0, K = 0, Kx = 0, Ke = 0;
+        for (int i = 0; i < 7; ++i) {
+            double rt[2] = {(rs[0][0] + rs21[0] * QXI[i]) + rs31[0] * QETA[i],
+                            (rs[0][1] + rs21[1] * QXI[i]) + rs31[1] * QETA[i]};
+            double ur[2] = {ulocal * (rmc[0] - rt[0]), ulocal * (rmc[1] - rt[1])};
+            double js[2], ks[2];
+            orc_jksums(ur, psi1, psi2, us1, us2, 1, 1e-8, js, ks, NULL);
+            cplx jw = mk(js[0], js[1]) * QW[i];
+            Jx += jw * QXI[i]; Je += jw * QETA[i]; J += jw;
+            cplx kw = mk(ks[0], ks[1]) * QW[i];
+            Kx += kw * QXI[i]; Ke += kw * QETA[i]; K += kw;
+        }
+        double rr[2] = {0, 0}, ri = 0;
+        for (int i = 0; i < 3; ++i) {
+            int ip1 = (i + 1) % 3;
+            double lv[2] = {rs[ip1][0] - rs[i][0], rs[ip1][1] - rs[i][1]};
+            double ln = sqrt(lv[0] * lv[0] + lv[1] * lv[1]);
+            lv[0] /= ln; lv[1] /= ln;
+            double uv[2] = {(-unz) * (-lv[1]), (-unz) * lv[0]};
+            double p0 = uv[0] * (rs[i][0] - rmc[0]) + uv[1] * (rs[i][1] - rmc[1]);
+            double p0sq = p0 * p0;
+            double lp = lv[0] * (rs[ip1][0] - rmc[0]) + lv[1] * (rs[ip1][1] - rmc[1]);
+            double lm = lv[0] * (rs[i][0] - rmc[0]) + lv[1] * (rs[i][1] - rmc[1]);
+            double ledge = lp - lm;
+            double pplus = sqrt(p0sq + lp * lp), pminus = sqrt(p0sq + lm * lm);
+            double factor;
+            if (fabs(p0 / ledge) < 1e-4) {
+                factor = lp * pplus - lm * pminus;
+            } else {
+                factor = p0 * log((pplus + lp) / (pminus + lm));
+                ri += factor;
+                factor = (factor * p0 + pplus * lp) - pminus * lm;
+            }
+            rr[0] += uv[0] * factor;
+            rr[1] += uv[1] * factor;
+        }
+        rr[0] *= 0.5; rr[1] *= 0.5;
+        rr[0] /= area2; rr[1] /= area2;
+        ri /= (area2 * ulocal);
+        cplx v[6] = {J, Jx, Je, K, Kx, Ke};
+        for (int q = 0; q < 6; ++q) {
+            JK[2 * (6 * iu + q)] = creal(v[q]);
+            JK[2 * (6 * iu + q) + 1] = cimag(v[q]);
+        }
+        rho_r[2 * iu] = rr[0]; rho_r[2 * iu + 1] = rr[1];
+        rinv[iu] = ri;
+    }
+}
+
+/* ---------------------------------------------------------- c3 / d3 */
+/* src/PGF.jl:138-162 */
+static cplx c3_calc(double k0, double u, cplx mu1, cplx eps1, cplx mu2, cplx eps2) {
+    cplx w1sq = ((k0 * k0) * mu1) * eps1 + u * u;
+    cplx w2sq = ((k0 * k0) * mu2) * eps2 + u * u;
+    return (mu1 * w2sq + mu2 * w1sq) / (2 * (mu1 + mu2));
+}
+static cplx d3_calc(double k0, double u, cplx mu1, cplx eps1, cplx mu2, cplx eps2) {
+    cplx w1sq = ((k0 * k0) * mu1) * eps1 + u * u;
+    cplx w2sq = ((k0 * k0) * mu2) * eps2 + u * u;
+    cplx num = mu1 * (w2sq * (2 * eps1 + eps2) - w1sq * eps1) +
+               mu2 * (w1sq * (2 * eps2 + eps1) - w2sq * eps2);
+    return num / ((2 * (mu1 + mu2)) * (eps1 + eps2));
+}
+
+void orc_c3d3(double k0, double u, const double *mu1, const double *eps1, const double *mu2,
+              const double *eps2, double *c3, double *d3) {
+    cplx a = c3_calc(k0, u, mk(mu1[0], mu1[1]), mk(eps1[0], eps1[1]), mk(mu2[0], mu2[1]),
+                     mk(eps2[0], eps2[1]));
+    cplx b = d3_calc(k0, u, mk(mu1[0], mu1[1]), mk(eps1[0], eps1[1]), mk(mu2[0], mu2[1]),
+                     mk(eps2[0], eps2[1]));
+    c3[0] = creal(a); c3[1] = cimag(a); d3[0] = creal(b); d3[1] = cimag(b);
+}
+
+/* ------------------------------------------------------------------ FFT */
+/* Unnormalised forward DFT (sign -1) of length n = 2^a 3^b 5^c, as FFTW's fft!. */
+static void fft_rec(const cplx *in, cplx *out, int n, int stride, cplx *scratch) {
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : 5;
+    int m = n / p;
+    for (int j = 0; j < p; ++j)
+        fft_rec(in + j * stride, out + j * m, m, stride * p, scratch);
+    /* out[j*m + k] holds sub-transform j at frequency k. */
+    for (int k = 0; k < m; ++k) {
+        cplx t[5];
+        for (int j = 0; j < p; ++j) {
+            double ang = -TWOPI * (double)(j * k) / (double)n;
+            t[j] = out[j * m + k] * cis(ang);
+        }
+        for (int q = 0; q < p; ++q) {
+            cplx s = 0;
+            for (int j = 0; j < p; ++j) {
+                int e = (j * q) % p;
+                double ang = -TWOPI * (double)e / (double)p;
+                s += t[j] * cis(ang);
+            }
+            scratch[q * m + k] = s;
+        }
+    }
+    memcpy(out, scratch, sizeof(cplx) * (size_t)n);
+}
+
+/* In-place 2-D forward DFT of an M x M column-major array. */
+static void fft2(cplx *a, int M) {
+    cplx *in = malloc(sizeof(cplx) * M), *out = malloc(sizeof(cplx) * M),
+         *scr = malloc(sizeof(cplx) * M);
+    for (int q = 0; q < M; ++q) {       /* along first index (columns) */
+        memcpy(in, a + (size_t)q * M, sizeof(cplx) * M);
+        fft_rec(in, out, M, 1, scr);
+        memcpy(a + (size_t)q * M, out, sizeof(cplx) * M);
+    }
+    for (int p = 0; p < M; ++p) {       /* along second index */
+        for (int q = 0; q < M; ++q) in[q] = a[p + (size_t)q * M];
+        fft_rec(in, out, M, 1, scr);
+        for (int q = 0; q < M; ++q) a[p + (size_t)q * M] = out[q];
+    }
+    free(in); free(out); free(scr);
+}
+
+void orc_fft2(double *a, int M) { fft2((cplx *)a, M); }
+
+static int nextprod235(int n) {
+    for (int v = n;; ++v) {
+        int x = v;
+        while (x % 2 == 0) x /= 2;
+        while (x % 3 == 0) x /= 3;
+        while (x % 5 == 0) x /= 5;
+        if (x == 1) return v;
+    }
+}
+
+/* Kummer-subtracted spectral summands of one lattice index, electric
+ * (src/PGF.jl:240-283) or magnetic (src/PGF.jl:529-567). */
+typedef struct {
+    int nl, s, magnetic;
+    const cplx *eps, *mu;
+    const double *width;
+    double k0sq, u, b2min;
+    double beta00[2], beta1[2], beta2[2];
+    cplx mut, epsb, c3, d3, f1, f3, p1, p3;
+} summand_ctx;
+
+static void summand(const summand_ctx *c, int m, int n, cplx *t1, cplx *t2) {
+    double bx = (c->beta00[0] + (double)m * c->beta1[0]) + (double)n * c->beta2[0];
+    double by = (c->beta00[1] + (double)m * c->beta1[1]) + (double)n * c->beta2[1];
+    double b2 = bx * bx + by * by;
+    if (!(b2 >= c->b2min)) b2 = c->b2min;          /* max(β², β²min) */
+    double kap2 = b2 + c->u * c->u;
+    double kap = sqrt(kap2);
+    int nl = c->nl, s = c->s;                      /* s: 1-based interface */
+    const cplx *eps = c->eps, *mu = c->mu;
+    cplx g = mysqrt(b2 - (c->k0sq * eps[0]) * mu[0]);
+    if (!c->magnetic) {
+        cplx zlte = mu[0] / g, zltm = g / eps[0];
+        for (int i = 1; i < s; ++i) {
+            g = mysqrt(b2 - (c->k0sq * eps[i]) * mu[i]);
+            cplx z0te = mu[i] / g, z0tm = g / eps[i];
+            cplx th = ctanh(c->width[i] * g);
+            zlte = (z0te * (zlte + z0te * th)) / (z0te + zlte * th);
+            zltm = (z0tm * (zltm + z0tm * th)) / (z0tm + zltm * th);
+        }
+        g = mysqrt(b2 - (c->k0sq * eps[nl - 1]) * mu[nl - 1]);
+        cplx zrte = mu[nl - 1] / g, zrtm = g / eps[nl - 1];
+        for (int i = nl - 2; i >= s; --i) {
+            g = mysqrt(b2 - (c->k0sq * eps[i]) * mu[i]);
+            cplx z0te = mu[i] / g, z0tm = g / eps[i];
+            cplx th = ctanh(c->width[i] * g);
+            zrte = (z0te * (zrte + z0te * th)) / (z0te + zrte * th);
+            zrtm = (z0tm * (zrtm + z0tm * th)) / (z0tm + zrtm * th);
+        }
+        cplx vte = (zlte * zrte) / (zlte + zrte);
+        cplx vtm = (zltm * zrtm) / (zltm + zrtm);
+        vte = (2 * vte) / c->mut;
+        *t1 = vte - (1 + c->c3 / kap2) / kap;
+        vtm = (2 * c->epsb) * vtm;
+        *t2 = (vtm + ((c->k0sq * c->epsb) * c->mut) * vte) / b2 - (1 + c->d3 / kap2) / kap;
+    } else {
+        cplx ylte = g / mu[0], yltm = eps[0] / g;
+        for (int i = 1; i < s; ++i) {
+            g = mysqrt(b2 - (c->k0sq * eps[i]) * mu[i]);
+            cplx y0te = g / mu[i], y0tm = eps[i] / g;
+            cplx th = ctanh(c->width[i] * g);
+            ylte = (y0te * (ylte + y0te * th)) / (y0te + ylte * th);
+            yltm = (y0tm * (yltm + y0tm * th)) / (y0tm + yltm * th);
+        }
+        g = mysqrt(b2 - (c->k0sq * eps[nl - 1]) * mu[nl - 1]);
+        cplx yrte = g / mu[nl - 1], yrtm = eps[nl - 1] / g;
+        for (int i = nl - 2; i >= s; --i) {
+            g = mysqrt(b2 - (c->k0sq * eps[i]) * mu[i]);
+            cplx y0te = g / mu[i], y0tm = eps[i] / g;
+            cplx th = ctanh(c->width[i] * g);
+            yrte = (y0te * (yrte + y0te * th)) / (y0te + yrte * th);
+            yrtm = (y0tm * (yrtm + y0tm * th)) / (y0tm + yrtm * th);
+        }
+        *t1 = (yltm + yrtm) - (c->f1 + c->f3 / kap2) / kap;
+        *t2 = ((((yltm + yrtm) * c->k0sq) + ylte) + yrte) / b2 - (c->p1 + c->p3 / kap2) / kap;
+    }
+}
+
+/* Spectral tables, src/PGF.jl:196-365 (electric) / :478-629 (magnetic).
+ * eps, mu: nl complex; width: nl (m); s: 1-based interface.  Output tables have
+ * (M+2)^2 complex entries, index (p+1) + (M+2)(q+1) for p, q in -1..M; the caller
+ * provides 2050^2 complex per table.  Returns M (mmax); *converged set. */
+int orc_modal_tables(int magnetic, double k0, double u, double psi1, double psi2, int nl,
+                     const double *eps_in, const double *mu_in, const double *width, int s,
+                     const double *beta1, const double *beta2, const double *beta00,
+                     double convtest, double *tab1_out, double *tab2_out, int *converged_out,
+                     int *mmax_cap) {
+    const int mg = 1024, mmax_last = 2048, convrepeat = 40;
+    cplx *eps = malloc(sizeof(cplx) * nl), *mu = malloc(sizeof(cplx) * nl);
+    for (int i = 0; i < nl; ++i) {
+        eps[i] = mk(eps_in[2 * i], eps_in[2 * i + 1]);
+        mu[i] = mk(mu_in[2 * i], mu_in[2 * i + 1]);
+    }
+    summand_ctx c;
+    c.nl = nl; c.s = s; c.magnetic = magnetic; c.eps = eps; c.mu = mu; c.width = width;
+    c.k0sq = k0 * k0; c.u = u; c.b2min = 1e-10 * c.k0sq;
+    c.beta00[0] = beta00[0]; c.beta00[1] = beta00[1];
+    c.beta1[0] = beta1[0]; c.beta1[1] = beta1[1];
+    c.beta2[0] = beta2[0]; c.beta2[1] = beta2[1];
+    cplx ms = mu[s - 1], ms1 = mu[s], es = eps[s - 1], es1 = eps[s];
+    c.mut = ((2 * ms) * ms1) / (ms + ms1);
+    c.epsb = (es + es1) / 2;
+    double area = (4 * (PI_ * PI_)) / fabs(beta1[0] * beta2[1] - beta1[1] * beta2[0]);
+    if (!magnetic) {
+        c.c3 = c3_calc(k0, u, ms, es, ms1, es1);
+        c.d3 = d3_calc(k0, u, ms, es, ms1, es1);
+    } else {
+        cplx c3s = c3_calc(k0, u, es, ms, es, ms), c3sp1 = c3_calc(k0, u, es1, ms1, es1, ms1);
+        cplx d3s = d3_calc(k0, u, es, ms, es, ms), d3sp1 = d3_calc(k0, u, es1, ms1, es1, ms1);
+        c.f1 = 2 * c.epsb;
+        c.f3 = es * c3s + es1 * c3sp1;
+        c.p1 = 2 / c.mut;
+        c.p3 = d3s / ms + d3sp1 / ms1;
+    }
+    size_t W = 2 * mg + 1;
+    cplx *t1 = calloc(W * W, sizeof(cplx)), *t2 = calloc(W * W, sizeof(cplx));
+#define TIX(m, n) ((size_t)((m) + mg) + W * (size_t)((n) + mg))
+    char *convlist = calloc(mg + 1, 1);
+    int cap = mmax_cap ? *mmax_cap : mmax_last;
+    int M = 0, Mold = -2, first = 1, converged = 0;
+    while (M < mmax_last) {
+        if (first) { M = 0; first = 0; }
+        else {
+            M = nextprod235(1 + M);
+            while (M % 2 != 0) M = nextprod235(1 + M);
+        }
+        int Mo2 = M / 2, Moldo2 = Mold / 2;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int r = Moldo2 + 1; r <= Mo2; ++r) {
+            cplx rs1 = 0, rs2 = 0;
+            int len = r == 0 ? 1 : 8 * r;
+            for (int k = 0; k < len; ++k) {
+                int m, n;
+                ring_mn(r, k, &m, &n);
+                cplx a, b;
+                summand(&c, m, n, &a, &b);
+                t1[TIX(m, n)] = a; t2[TIX(m, n)] = b;
+                rs1 += a; rs2 += b;
+            }
+            convlist[r] = 0;  /* filled after the ring-0 value is known */
+            if (r > 0) {
+                double test1 = cabs(rs1 / t1[TIX(0, 0)]);
+                double test2 = cabs(rs2 / t2[TIX(0, 0)]);
+                convlist[r] = (test1 < convtest && test2 < convtest);
+            }
+        }
+        if (Mo2 >= convrepeat) {
+            int all = 1;
+            for (int r = Mo2 - convrepeat + 1; r <= Mo2; ++r) all &= convlist[r];
+            if (all) { converged = 1; break; }
+        }
+        Mold = M;
+        if (M >= cap) break;
+    }
+    int Mo2 = M / 2;
+    cplx *ft1 = malloc(sizeof(cplx) * M * M), *ft2 = malloc(sizeof(cplx) * M * M);
+    for (int q = 0; q < M; ++q)
+        for (int p = 0; p < M; ++p) {
+            ft1[p + (size_t)M * q] = t1[TIX(p - Mo2, q - Mo2)];
+            ft2[p + (size_t)M * q] = t2[TIX(p - Mo2, q - Mo2)];
+        }
+    fft2(ft1, M);
+    fft2(ft2, M);
+    double scale = magnetic ? area : 2 * area;
+    for (int q = 0; q < M; ++q) {
+        double qterm = q * (PI_ - psi2 / M);
+        for (int p = 0; p < M; ++p) {
+            double pterm = p * (PI_ - psi1 / M);
+            cplx cf = cis(pterm + qterm) / scale;
+            ft1[p + (size_t)M * q] *= cf;
+            ft2[p + (size_t)M * q] *= cf;
+        }
+    }
+    size_t E = (size_t)M + 2;
+    cplx *o1 = (cplx *)tab1_out, *o2 = (cplx *)tab2_out;
+    memset(o1, 0, sizeof(cplx) * E * E);
+    memset(o2, 0, sizeof(cplx) * E * E);
+#define OIX(p, q) ((size_t)((p) + 1) + E * (size_t)((q) + 1))
+    for (int q = 0; q < M; ++q)
+        for (int p = 0; p < M; ++p) {
+            o1[OIX(p, q)] = ft1[p + (size_t)M * q];
+            o2[OIX(p, q)] = ft2[p + (size_t)M * q];
+        }
+    cplx cis1 = cis(-psi1), cis2 = cis(-psi2);
+    if (!magnetic) {
+        cplx cis12 = cis1 * cis2;
+        for (int k = 0; k < M; ++k) {
+            o1[OIX(M, k)] = cis1 * o1[OIX(0, k)];
+            o1[OIX(k, M)] = cis2 * o1[OIX(k, 0)];
+            o2[OIX(M, k)] = cis1 * o2[OIX(0, k)];
+            o2[OIX(k, M)] = cis2 * o2[OIX(k, 0)];
+        }
+        o1[OIX(M, M)] = cis12 * o1[OIX(0, 0)];
+        o2[OIX(M, M)] = cis12 * o2[OIX(0, 0)];
+        cis1 = 1 / cis1; cis2 = 1 / cis2;
+    } else {
+        for (int k = 0; k < M; ++k) {
+            o1[OIX(M, k)] = cis1 * o1[OIX(0, k)];
+            o1[OIX(k, M)] = cis2 * o1[OIX(k, 0)];
+            o2[OIX(M, k)] = cis1 * o2[OIX(0, k)];
+            o2[OIX(k, M)] = cis2 * o2[OIX(k, 0)];
+        }
+        cplx c12 = cis(-(psi1 + psi2));
+        o1[OIX(M, M)] = c12 * o1[OIX(0, 0)];
+        o2[OIX(M, M)] = c12 * o2[OIX(0, 0)];
+        cis1 = cis(psi1); cis2 = cis(psi2);
+    }
+    for (int k = 0; k <= M; ++k) {
+        o1[OIX(-1, k)] = cis1 * o1[OIX(M - 1, k)];
+        o2[OIX(-1, k)] = cis1 * o2[OIX(M - 1, k)];
+    }
+    for (int k = -1; k <= M - 1; ++k) {
+        o1[OIX(k, -1)] = cis2 * o1[OIX(k, M - 1)];
+        o2[OIX(k, -1)] = cis2 * o2[OIX(k, M - 1)];
+    }
+#undef OIX
+#undef TIX
+    free(ft1); free(ft2); free(t1); free(t2); free(convlist); free(eps); free(mu);
+    if (converged_out) *converged_out = converged;
+    return M;
+}
+
+/* Sigma_m interpolation, src/PGF.jl:392-443.  beta1/beta2 in 1/m, rho in m. */
+static cplx sigma_interp(const cplx *tab, int M, const double *b1, const double *b2,
+                         double psi1, double psi2, double rx, double ry) {
+    double x1o = (b1[0] * rx + b1[1] * ry) / TWOPI;
+    if (fabs(x1o) < 1e-8) x1o = 0.0;
+    double x2o = (b2[0] * rx + b2[1] * ry) / TWOPI;
+    if (fabs(x2o) < 1e-8) x2o = 0.0;
+    /* Julia mod(x, 1) */
+    double x1 = fmod(x1o, 1.0);
+    if (x1 == 0) x1 = 0.0; else if (x1 < 0) x1 = x1 + 1.0;
+    double x2 = fmod(x2o, 1.0);
+    if (x2 == 0) x2 = 0.0; else if (x2 < 0) x2 = x2 + 1.0;
+    double msh = rint(x1o - x1), nsh = rint(x2o - x2);
+    int m = (int)(M * x1);
+    double p = M * x1 - m;
+    int n = (int)(M * x2);
+    double q = M * x2 - n;
+    if (m > M - 1) m = M - 1;   /* guard: M*x rounding up to M */
+    if (n > M - 1) n = M - 1;
+    double p2 = p * p, q2 = q * q, pq = p * q;
+    size_t E = (size_t)M + 2;
+#define T(a, b) tab[(size_t)((a) + 1) + E * (size_t)((b) + 1)]
+    cplx sm = (0.5 * (q2 - q)) * T(m, n - 1);
+    sm = sm + (0.5 * (p2 - p)) * T(m - 1, n);
+    sm = sm + (((1 + pq) - p2) - q2) * T(m, n);
+    sm = sm + (0.5 * ((p2 - 2 * pq) + p)) * T(m + 1, n);
+    sm = sm + (0.5 * ((q2 - 2 * pq) + q)) * T(m, n + 1);
+    sm = sm + pq * T(m + 1, n + 1);
+#undef T
+    if (msh != 0 || nsh != 0) sm *= cis(-(msh * psi1 + nsh * psi2));
+    return sm;
+}
+
+void orc_sigma(const double *tab, int M, const double *b1, const double *b2, double psi1,
+               double psi2, const double *rho, double *out) {
+    cplx v = sigma_interp((const cplx *)tab, M, b1, b2, psi1, psi2, rho[0], rho[1]);
+    out[0] = creal(v); out[1] = cimag(v);
+}
+
+/* ------------------------------------------------------------- fill Z / Y */
+/* fillz src/FillZY.jl:50-245 (magnetic=0) and filly :273-457 (magnetic=1).
+ * rho: sheet units, upm units per meter.  ufp given as CSR (offsets, members)
+ * or identity (offsets == NULL: class i = face pair i).  JK/rho_r/rinv from
+ * orc_filljk.  tab1/tab2 from orc_modal_tables with the same kind.
+ * layer params: epsr/mur of layers s and s+1 (complex pairs).  Z: nbf x nbf
+ * complex column-major, overwritten. */
+void orc_fill(int magnetic, int64_t nface, const double *rho, double upm, const int64_t *fv,
+              const int64_t *fe, const double *fr, int64_t nbf, const int64_t *bff,
+              const int64_t *ebf, const int64_t *eci, int64_t nufp, const int64_t *offsets,
+              const int64_t *members, const double *JK, const double *rho_r,
+              const double *rinv, double k0, double u, const double *eps1_, const double *mu1_,
+              const double *eps2_, const double *mu2_, const double *tab1, const double *tab2,
+              int M, const double *beta1, const double *beta2, double psi1, double psi2,
+              double *Zout) {
+    cplx *Z = (cplx *)Zout;
+    memset(Z, 0, sizeof(cplx) * (size_t)nbf * nbf);
+    cplx e1 = mk(eps1_[0], eps1_[1]), m1 = mk(mu1_[0], mu1_[1]);
+    cplx e2 = mk(eps2_[0], eps2_[1]), m2 = mk(mu2_[0], mu2_[1]);
+    const double EPS0 = 1.0 / (MU0 * (C0 * C0));
+    cplx mut = ((2 * m1) * m2) / (m1 + m2);
+    cplx epsb = magnetic ? 0.5 * (e1 + e2) : (e1 + e2) / 2;
+    double omega = k0 * C0;
+    cplx c3 = 0, d3 = 0, I1fact = 0, KFfact = 0, I2fact = 0, KPfact = 0;
+    if (!magnetic) {
+        c3 = c3_calc(k0, u, m1, e1, m2, e2);
+        d3 = d3_calc(k0, u, m1, e1, m2, e2);
+    } else {
+        cplx c3s = c3_calc(k0, u, m1, e1, m1, e1), c3sp1 = c3_calc(k0, u, m2, e2, m2, e2);
+        cplx d3s = d3_calc(k0, u, m1, e1, m1, e1), d3sp1 = d3_calc(k0, u, m2, e2, m2, e2);
+        I1fact = PI_ / epsb;
+        KFfact = (c3s * e1 + c3sp1 * e2) / ((2 * epsb) * u);
+        I2fact = PI_ * mut;
+        KPfact = (mut / (2 * u)) * (d3s / m1 + d3sp1 / m2);
+    }
+    cplx ff[5] = {1, 1, cis(-psi1), 1, cis(-psi2)};
+    static const int nxt[3] = {1, 2, 0}, prv[3] = {2, 0, 1};
+    static const int third[3][3] = {{-1, 2, 1}, {2, -1, 0}, {1, 0, -1}};
+    const cplx *t1 = (const cplx *)tab1, *t2 = (const cplx *)tab2;
+    int nthreads = 1;
+#ifdef _OPENMP
+    nthreads = omp_get_max_threads();
+#endif
+    /* Per-thread private Z accumulation would cost nthreads*N^2; instead use
+     * atomic updates on the two doubles of each entry. */
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t iu = 0; iu < nufp; ++iu) {
+        int64_t fp0 = offsets ? members[offsets[iu]] : iu;
+        int64_t ifm = fp0 % nface, ifs = fp0 / nface;
+        double rs[3][2], rm[3][2];
+        for (int i = 0; i < 3; ++i) {
+            rs[i][0] = rho[2 * fv[3 * ifs + i]] / upm;
+            rs[i][1] = rho[2 * fv[3 * ifs + i] + 1] / upm;
+            rm[i][0] = rho[2 * fv[3 * ifm + i]] / upm;
+            rm[i][1] = rho[2 * fv[3 * ifm + i] + 1] / upm;
+        }
+        double rs32[2] = {rs[2][0] - rs[1][0], rs[2][1] - rs[1][1]};
+        double rs12[2] = {rs[0][0] - rs[1][0], rs[0][1] - rs[1][1]};
+        double area = 0.5 * (rs32[0] * rs12[1] - rs32[1] * rs12[0]);
+        double area48 = 48 * fabs(area);
+        int self_tri = (ifm == ifs);
+        double ls[3] = {0, 0, 0};
+        if (self_tri) {
+            double r13[2] = {rs[0][0] - rs[2][0], rs[0][1] - rs[2][1]};
+            ls[0] = sqrt(rs32[0] * rs32[0] + rs32[1] * rs32[1]);
+            ls[1] = sqrt(r13[0] * r13[0] + r13[1] * r13[1]);
+            ls[2] = sqrt(rs12[0] * rs12[0] + rs12[1] * rs12[1]);
+        }
+        double rmc[2] = {((rm[0][0] + rm[1][0]) + rm[2][0]) / 3,
+                         ((rm[0][1] + rm[1][1]) + rm[2][1]) / 3};
+        /* zint, src/Zint.jl:218-245 */
+        cplx I1x = 0, I1e = 0, I1 = 0, I2 = 0;
+        for (int i = 0; i < 7; ++i) {
+            double rt[2] = {(rs[0][0] + (rs[1][0] - rs[0][0]) * QXI[i]) +
+                                (rs[2][0] - rs[0][0]) * QETA[i],
+                            (rs[0][1] + (rs[1][1] - rs[0][1]) * QXI[i]) +
+                                (rs[2][1] - rs[0][1]) * QETA[i]};
+            double rd[2] = {rmc[0] - rt[0], rmc[1] - rt[1]};
+            cplx sg1 = sigma_interp(t1, M, beta1, beta2, psi1, psi2, rd[0], rd[1]);
+            cplx sg2 = sigma_interp(t2, M, beta1, beta2, psi1, psi2, rd[0], rd[1]);
+            cplx sw = sg1 * QW[i];
+            I1x += sw * QXI[i];
+            I1e += sw * QETA[i];
+            I1 += sw;
+            I2 += sg2 * QW[i];
+        }
+        const double *jk = JK + 12 * iu;
+        cplx J = mk(jk[0], jk[1]), Jx = mk(jk[2], jk[3]), Je = mk(jk[4], jk[5]);
+        cplx K = mk(jk[6], jk[7]), Kx = mk(jk[8], jk[9]), Ke = mk(jk[10], jk[11]);
+        double ri = rinv[iu];
+        double rr[2] = {rho_r[2 * iu], rho_r[2 * iu + 1]};
+        cplx I1z = (I1 - I1x) - I1e, Jz = (J - Jx) - Je, Kz = (K - Kx) - Ke;
+        double rc2[3][2];
+        for (int i = 0; i < 3; ++i) {
+            rc2[i][0] = 0.5 * (rmc[0] - rm[i][0]);
+            rc2[i][1] = 0.5 * (rmc[1] - rm[i][1]);
+        }
+        int64_t nmem = offsets ? offsets[iu + 1] - offsets[iu] : 1;
+        for (int64_t im_ = 0; im_ < nmem; ++im_) {
+            int64_t fp = offsets ? members[offsets[iu] + im_] : iu;
+            int64_t fm = fp % nface, fs = fp / nface;
+            for (int isl = 0; isl < 3; ++isl) {
+                int64_t se = fe[3 * fs + isl];
+                int64_t sbf = ebf[se];
+                if (sbf < 0) continue;
+                cplx sflag;
+                if (bff[2 * sbf] == fs) sflag = ff[eci[se]];
+                else sflag = -ff[eci[se]];
+                double ax = ((u * ri) * (rmc[0] - rs[isl][0]));
+                double ay = ((u * ri) * (rmc[1] - rs[isl][1]));
+                double sing[2] = {rr[0] + ax, rr[1] + ay};
+                cplx I1i[2], Ji[2], Ki[2];
+                for (int d = 0; d < 2; ++d) {
+                    I1i[d] = ((rs[0][d] * I1x + rs[1][d] * I1e) + rs[2][d] * I1z) - rs[isl][d] * I1;
+                    Ji[d] = ((rs[0][d] * Jx + rs[1][d] * Je) + rs[2][d] * Jz) - rs[isl][d] * J;
+                    Ki[d] = ((rs[0][d] * Kx + rs[1][d] * Ke) + rs[2][d] * Kz) - rs[isl][d] * K;
+                }
+                cplx Ai[2], Phi;
+                if (!magnetic) {
+                    cplx asf = ((MU0 / FOURPI) * sflag) * mut;
+                    cplx psf = (I / epsb) * (sflag / ((TWOPI * omega) * EPS0));
+                    for (int d = 0; d < 2; ++d)
+                        Ai[d] = asf * ((((FOURPI * I1i[d]) + sing[d]) + u * Ji[d]) + (c3 / u) * Ki[d]);
+                    Phi = psf * (((FOURPI * I2) + u * (ri + J)) + (d3 / u) * K);
+                } else {
+                    cplx fsf = ((-EPS0 / PI_) * epsb) * sflag;
+                    cplx ysf = (2 * I / (((PI_ * omega) * MU0) * mut)) * sflag;
+                    for (int d = 0; d < 2; ++d)
+                        Ai[d] = fsf * ((((I1fact * I1i[d]) + sing[d]) + u * Ji[d]) + KFfact * Ki[d]);
+                    Phi = ysf * (((I2fact * I2) + u * (ri + J)) + KPfact * K);
+                }
+                for (int iml = 0; iml < 3; ++iml) {
+                    int64_t me = fe[3 * fm + iml];
+                    int64_t mbf = ebf[me];
+                    if (mbf < 0) continue;
+                    cplx mflag;
+                    if (bff[2 * mbf] == fm) mflag = conj(ff[eci[me]]);
+                    else mflag = -conj(ff[eci[me]]);
+                    cplx dot = rc2[iml][0] * Ai[0] + rc2[iml][1] * Ai[1];
+                    cplx zc;
+                    if (!magnetic) zc = mflag * ((I * omega) * dot - Phi);
+                    else zc = mflag * ((-I * omega) * dot - Phi);
+                    if (!magnetic && self_tri && fr[fm] != 0) {
+                        double zl;
+                        if (fe[3 * fm + iml] == se) {
+                            zl = fr[fs] / area48 *
+                                 (3 * (ls[nxt[isl]] * ls[nxt[isl]] + ls[prv[isl]] * ls[prv[isl]]) -
+                                  ls[isl] * ls[isl]);
+                            zc += zl;
+                        } else {
+                            cplx zlc = ((fr[fs] / area48) * sflag) * mflag *
+                                       ((ls[isl] * ls[isl] + ls[iml] * ls[iml]) -
+                                        3 * ls[third[isl][iml]] * ls[third[isl][iml]]);
+                            zc += zlc;
+                        }
+                    }
+                    double *zp = (double *)&Z[(size_t)mbf + (size_t)nbf * sbf];
+#pragma omp atomic
+                    zp[0] += creal(zc);
+#pragma omp atomic
+                    zp[1] += cimag(zc);
+                }
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- rwgbfft */
+/* src/RWG.jl:428-529.  ft: (2, nbf) complex, overwritten.  k in 1/m. */
+static double sj0(double x) {
+    if (fabs(x) < 1e-3) { double x2 = x * x; return 1 - x2 / 6 * (1 - x2 / 20); }
+    return sin(x) / x;
+}
+static double sj1(double x) {
+    if (fabs(x) < 1e-3) return x * (1.0 / 3 - x * x / 30);
+    return (sin(x) / x - cos(x)) / x;
+}
+
+void orc_rwgbfft(int64_t nface, const double *rho, double upm, const int64_t *fv,
+                 const int64_t *fe, int64_t nbf, const int64_t *bff, const int64_t *ebf,
+                 const int64_t *eci, const double *k, double psi1, double psi2, double *ftout) {
+    cplx *ft = (cplx *)ftout;
+    memset(ft, 0, sizeof(cplx) * 2 * (size_t)nbf);
+    cplx ff[5] = {1, 1, cis(-psi1), 1, cis(-psi2)};
+    double kmagsq = k[0] * k[0] + k[1] * k[1];
+    double kmag = sqrt(kmagsq);
+    for (int64_t f = 0; f < nface; ++f) {
+        double r[3][2], lv[3][2], rc[3][2];
+        for (int i = 0; i < 3; ++i) {
+            r[i][0] = rho[2 * fv[3 * f + i]] / upm;
+            r[i][1] = rho[2 * fv[3 * f + i] + 1] / upm;
+        }
+        double lmax = 0;
+        for (int i = 0; i < 3; ++i) {
+            int nx = (i + 1) % 3;
+            lv[i][0] = r[nx][0] - r[i][0];
+            lv[i][1] = r[nx][1] - r[i][1];
+            rc[i][0] = r[i][0] + 0.5 * lv[i][0];
+            rc[i][1] = r[i][1] + 0.5 * lv[i][1];
+            double ln = sqrt(lv[i][0] * lv[i][0] + lv[i][1] * lv[i][1]);
+            if (ln > lmax) lmax = ln;
+        }
+        if (kmag * lmax < 1e-4) {
+            double cen[2] = {((r[0][0] + r[1][0]) + r[2][0]) / 3, ((r[0][1] + r[1][1]) + r[2][1]) / 3};
+            cplx cph = cis(k[0] * cen[0] + k[1] * cen[1]);
+            for (int i = 0; i < 3; ++i) {
+                int64_t ie = fe[3 * f + i], ib = ebf[ie];
+                if (ib < 0) continue;
+                double f0[2] = {0.5 * (cen[0] - r[i][0]), 0.5 * (cen[1] - r[i][1])};
+                cplx fac = cph * ff[eci[ie]];
+                double sg = (bff[2 * ib] == f) ? 1.0 : -1.0;
+                ft[2 * ib] += sg * (f0[0] * fac);
+                ft[2 * ib + 1] += sg * (f0[1] * fac);
+            }
+        } else {
+            double kf[2] = {2 * k[0] / kmagsq, 2 * k[1] / kmagsq};
+            double darea = lv[0][0] * lv[1][1] - lv[0][1] * lv[1][0];
+            double denom = darea * kmagsq;
+            double zlk[3], j0v[3], rt2[3][2];
+            cplx ct1[3][2], cph[3];
+            for (int i = 0; i < 3; ++i) {
+                zlk[i] = lv[i][0] * k[1] - lv[i][1] * k[0];
+                ct1[i][0] = mk(0.0 * rc[i][0], rc[i][0]) - kf[0];
+                ct1[i][1] = mk(0.0 * rc[i][1], rc[i][1]) - kf[1];
+            }
+            for (int i = 0; i < 3; ++i) {
+                double dk = 0.5 * (k[0] * lv[i][0] + k[1] * lv[i][1]);
+                j0v[i] = sj0(dk);
+                double j1 = sj1(dk);
+                rt2[i][0] = ((0.5 * zlk[i]) * j1) * lv[i][0];
+                rt2[i][1] = ((0.5 * zlk[i]) * j1) * lv[i][1];
+                cph[i] = cis(k[0] * rc[i][0] + k[1] * rc[i][1]);
+            }
+            for (int i = 0; i < 3; ++i) {
+                int64_t ie = fe[3 * f + i], ib = ebf[ie];
+                if (ib < 0) continue;
+                cplx cjr[2] = {mk(0.0 * r[i][0], r[i][0]), mk(0.0 * r[i][1], r[i][1])};
+                cplx cs[2] = {0, 0};
+                for (int n = 0; n < 3; ++n) {
+                    double zh[2] = {-lv[n][1], lv[n][0]};
+                    for (int d = 0; d < 2; ++d) {
+                        cplx ct3 = (zh[d] + zlk[n] * (ct1[n][d] - cjr[d])) * j0v[n];
+                        cs[d] += cph[n] * (ct3 - rt2[n][d]);
+                    }
+                }
+                cplx fac = ff[eci[ie]] / denom;
+                double sg = (bff[2 * ib] == f) ? 1.0 : -1.0;
+                ft[2 * ib] += sg * (cs[0] * fac);
+                ft[2 * ib + 1] += sg * (cs[1] * fac);
+            }
+        }
+    }
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
