@@ -1,0 +1,1094 @@
+// C ABI of libpssfss_hip: contexts, sheets and the per-point MoM pipeline
+//   J/K spatial integrals (cached by (psi1, psi2, u))  -> spectral tables ->
+//   fill -> LU (FP64, or FP32 + FP64 refinement) -> RHS -> solve -> projection
+// for one or many points of a sheet.  Host code here only orchestrates and does
+// the O(n_modes^2) GSM bookkeeping of src/PSSFSS.jl:431-456 / 557-584.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "../../include/pssfss_hip.h"
+#include "internal.hpp"
+
+namespace {
+
+constexpr double TWOPI = 6.283185307179586;
+constexpr double PI_ = 3.141592653589793;
+constexpr double TOL_U = 1e-4;       // src/Constants.jl:19 (J/K cache key on u)
+constexpr int CONVREPEAT = 40;       // src/PGF.jl:217
+constexpr int MMAX_LAST = 2048;      // src/PGF.jl:18
+constexpr double CONVTEST_FILL = 1e-7;   // src/FillZY.jl:69,293
+
+// c3 / d3 wide-band coefficients, eqs. (4.30)/(4.33), src/PGF.jl:138-162.
+cd c3_calc(double k0, double u, cd mu1, cd eps1, cd mu2, cd eps2) {
+    cd w1 = ((k0 * k0) * mu1) * eps1 + u * u;
+    cd w2 = ((k0 * k0) * mu2) * eps2 + u * u;
+    return (mu1 * w2 + mu2 * w1) / (2.0 * (mu1 + mu2));
+}
+cd d3_calc(double k0, double u, cd mu1, cd eps1, cd mu2, cd eps2) {
+    cd w1 = ((k0 * k0) * mu1) * eps1 + u * u;
+    cd w2 = ((k0 * k0) * mu2) * eps2 + u * u;
+    cd num = mu1 * (w2 * (2.0 * eps1 + eps2) - w1 * eps1) +
+             mu2 * (w1 * (2.0 * eps2 + eps1) - w2 * eps2);
+    return num / ((2.0 * (mu1 + mu2)) * (eps1 + eps2));
+}
+
+cd hcis(double t) { return cd{cos(t), sin(t)}; }
+
+// The mmax sequence of the table convergence loop (src/PGF.jl:223-233):
+// 0, then successive even 5-smooth numbers, ending at 2048.
+const std::vector<int> &mmax_sequence() {
+    static std::vector<int> seq;
+    if (seq.empty()) {
+        auto next235 = [](int n) {
+            for (int v = n;; ++v) {
+                int x = v;
+                while (x % 2 == 0) x /= 2;
+                while (x % 3 == 0) x /= 3;
+                while (x % 5 == 0) x /= 5;
+                if (x == 1) return v;
+            }
+        };
+        int M = 0;
+        seq.push_back(0);
+        while (M < MMAX_LAST) {
+            M = next235(1 + M);
+            while (M % 2 != 0) M = next235(1 + M);
+            seq.push_back(M);
+        }
+    }
+    return seq;
+}
+
+bool approx2(const double *a, const double *b) {
+    double d = hypot(a[0] - b[0], a[1] - b[1]);
+    double na = hypot(a[0], a[1]), nb = hypot(b[0], b[1]);
+    return d <= 1.4901161193847656e-08 * std::max(na, nb);
+}
+
+}  // namespace
+
+struct pssfss_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    std::map<int, DBuf<cd>> dft;     // DFT matrices by size
+    // reusable workspaces
+    DBuf<cd> wZ, wB, wX, wR, wft, wout, wjk;
+    DBuf<cf> wZf, wRf;
+    DBuf<int32_t> wpiv;
+    DBuf<double> wnorm;
+    const cd *dft_matrix(int M) {
+        auto it = dft.find(M);
+        if (it != dft.end()) return it->second.p;
+        DBuf<cd> &w = dft[M];
+        w.alloc((size_t)M * M);
+        launch_dft_matrix(M, w.p, st);
+        return w.p;
+    }
+};
+
+struct pssfss_sheet {
+    int64_t nnode = 0, nface = 0, nedge = 0, nbf = 0, nufp = 0;
+    bool identity = true;
+    char cls = 'J';
+    double upm = 1.0;
+    double s1[2] = {0, 0}, s2[2] = {0, 0}, b1[2] = {0, 0}, b2[2] = {0, 0};
+    DBuf<double> fxs, fxm, fr;
+    DBuf<int32_t> fbf, fsg, feci, fedge, bf_face, bf_loc;
+    DBuf<int64_t> uoff, umem;
+    // J/K cache with the key semantics of src/FillZY.jl:97-108
+    bool jk_valid = false;
+    double jk_psi1 = 0, jk_psi2 = 0, jk_u = 0;
+    DBuf<cd> jk;
+    // singular integrals (depend on u only)
+    bool sing_valid = false;
+    double sing_u = 0;
+    DBuf<double> rho_r, rinv;
+
+    SheetDev dev() const {
+        SheetDev d;
+        d.nface = nface; d.nbf = nbf; d.nufp = nufp;
+        d.identity = identity ? 1 : 0;
+        d.fxs = fxs.p; d.fxm = fxm.p;
+        d.fbf = fbf.p; d.fsg = fsg.p; d.feci = feci.p; d.fedge = fedge.p;
+        d.fr = fr.p;
+        d.uoff = uoff.p; d.umem = umem.p;
+        return d;
+    }
+};
+
+namespace {
+
+template <class F>
+int guarded(pssfss_ctx *ctx, F &&f) {
+    if (!ctx) return PSSFSS_EINVAL;
+    try {
+        f();
+        ctx->err.clear();
+        return PSSFSS_OK;
+    } catch (const ArgError &e) {
+        ctx->err = e.what();
+        return PSSFSS_EINVAL;
+    } catch (const HipError &e) {
+        ctx->err = e.what();
+        return strstr(e.what(), "hipMalloc") ? PSSFSS_ENOMEM : PSSFSS_EHIP;
+    } catch (const std::bad_alloc &) {
+        ctx->err = "host allocation failed";
+        return PSSFSS_ENOMEM;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return PSSFSS_EINVAL;
+    }
+}
+
+void sync(pssfss_ctx *ctx) { HIPCHECK(hipStreamSynchronize(ctx->st)); }
+
+template <class T>
+void d2h(pssfss_ctx *ctx, T *h, const T *d, size_t n) {
+    if (n == 0) return;
+    HIPCHECK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->st));
+}
+
+cd lay(const double *a, int i) { return cd{a[2 * i], a[2 * i + 1]}; }
+
+void check_point(const pssfss_sheet *sh, const pssfss_point *pt, bool need_modes) {
+    if (!pt) throw ArgError("null point");
+    if (pt->nl < 2 || pt->nl > MAXL) throw ArgError("point: nl must be in 2..16");
+    if (pt->s < 1 || pt->s >= pt->nl) throw ArgError("point: s must be in 1..nl-1");
+    if (!pt->epsr || !pt->mur || !pt->width) throw ArgError("point: null layer arrays");
+    if (!(pt->k0 > 0) || !(pt->u > 0)) throw ArgError("point: k0 and u must be positive");
+    if (need_modes) {
+        for (int r = 0; r < 2; ++r) {
+            if (pt->nmodes[r] < 0 || pt->nmodes[r] > pt->nmax)
+                throw ArgError("point: nmodes out of range");
+            if (pt->nmodes[r] > 0 && (!pt->beta[r] || !pt->tvec[r] || !pt->c[r]))
+                throw ArgError("point: null mode arrays");
+            if (sh->cls == 'M' && pt->nmodes[r] > 0 && !pt->Y[r])
+                throw ArgError("point: class 'M' needs modal admittances");
+        }
+        if (!pt->vinc || !pt->tlgf) throw ArgError("point: null vinc/tlgf");
+    }
+}
+
+// ------------------------------------------------------------ spectral tables
+struct TabInput {
+    TabParams tp;
+    double psi1, psi2, area;
+};
+
+TabInput make_tab_input(const pssfss_point *pt, bool magnetic, const double *b1,
+                        const double *b2, const double *b00) {
+    TabInput in;
+    TabParams &c = in.tp;
+    memset(&c, 0, sizeof(c));
+    c.k0sq = pt->k0 * pt->k0;
+    c.u = pt->u;
+    c.b2min = 1e-10 * c.k0sq;
+    c.b00[0] = b00[0]; c.b00[1] = b00[1];
+    c.b1[0] = b1[0]; c.b1[1] = b1[1]; c.b2[0] = b2[0]; c.b2[1] = b2[1];
+    c.nl = pt->nl;
+    c.s = pt->s;
+    c.magnetic = magnetic ? 1 : 0;
+    for (int i = 0; i < pt->nl; ++i) {
+        c.eps[i] = lay(pt->epsr, i);
+        c.mu[i] = lay(pt->mur, i);
+        c.width[i] = pt->width[i];
+    }
+    const int s = pt->s;
+    cd ms = c.mu[s - 1], ms1 = c.mu[s], es = c.eps[s - 1], es1 = c.eps[s];
+    c.mut = ((2.0 * ms) * ms1) / (ms + ms1);
+    c.epsb = (es + es1) / 2.0;
+    const double k0 = pt->k0, u = pt->u;
+    if (!magnetic) {
+        c.c3 = c3_calc(k0, u, ms, es, ms1, es1);
+        c.d3 = d3_calc(k0, u, ms, es, ms1, es1);
+    } else {
+        cd c3s = c3_calc(k0, u, es, ms, es, ms), c3sp1 = c3_calc(k0, u, es1, ms1, es1, ms1);
+        cd d3s = d3_calc(k0, u, es, ms, es, ms), d3sp1 = d3_calc(k0, u, es1, ms1, es1, ms1);
+        c.f1 = 2.0 * c.epsb;
+        c.f3 = es * c3s + es1 * c3sp1;
+        c.p1 = 2.0 / c.mut;
+        c.p3 = d3s / ms + d3sp1 / ms1;
+    }
+    in.psi1 = pt->psi1;
+    in.psi2 = pt->psi2;
+    in.area = (4.0 * (PI_ * PI_)) / fabs(b1[0] * b2[1] - b1[1] * b2[0]);
+    return in;
+}
+
+TabInput sheet_tab_input(const pssfss_sheet *sh, const pssfss_point *pt) {
+    const double upm = sh->upm;
+    double b1[2] = {sh->b1[0] * upm, sh->b1[1] * upm}, b2[2] = {sh->b2[0] * upm, sh->b2[1] * upm};
+    double b00[2] = {(pt->psi1 * b1[0] + pt->psi2 * b2[0]) / TWOPI,
+                     (pt->psi1 * b1[1] + pt->psi2 * b2[1]) / TWOPI};
+    return make_tab_input(pt, sh->cls == 'M', b1, b2, b00);
+}
+
+struct TableSet {
+    int npts = 0, Mmax = 0;
+    std::vector<int> M, converged;
+    DBuf<cd> tabs;          // per point: two tables of (M+2)^2
+    int64_t tab_stride = 0; // complex entries per point
+    const cd *tab1(int p) const { return tabs.p + p * tab_stride; }
+    const cd *tab2(int p) const {
+        int64_t E = (int64_t)M[p] + 2;
+        return tabs.p + p * tab_stride + E * E;
+    }
+};
+
+// Tables of several points: summands ring by ring until the convergence test of
+// src/PGF.jl:291-298 passes at an mmax of the loop's sequence (rings below 40
+// can never pass it, so the first test is at mmax = 80), then the 2-D DFT as two
+// GEMMs, the phase of eq. (5.32) and the ghost rows.
+void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convtest,
+                  TableSet &out) {
+    hipStream_t st = ctx->st;
+    const int np = (int)in.size();
+    const auto &seq = mmax_sequence();
+    std::vector<TabParams> tp(np);
+    for (int p = 0; p < np; ++p) tp[p] = in[p].tp;
+    DBuf<TabParams> d_tp(np);
+    d_tp.upload(tp.data(), np, st);
+    int rcap = 64;
+    int64_t cap = (int64_t)(2 * rcap + 1) * (2 * rcap + 1);
+    DBuf<cd> t1((size_t)cap * np), t2((size_t)cap * np);
+    const int rstride = MMAX_LAST / 2 + 1;
+    DBuf<int> dflags((size_t)rstride * np);
+    std::vector<int> hflags((size_t)rstride * np, 0);
+    std::vector<size_t> cand(np);
+    std::vector<int> decided(np, 0);
+    out.M.assign(np, 0);
+    out.converged.assign(np, 0);
+    size_t first = 0;
+    while (seq[first] < 2 * CONVREPEAT) ++first;
+    for (int p = 0; p < np; ++p) cand[p] = first;
+    int computed = -1;
+    for (;;) {
+        int need = computed;
+        for (int p = 0; p < np; ++p)
+            if (!decided[p]) need = std::max(need, seq[cand[p]] / 2);
+        if (need > computed) {
+            if (need > rcap) {
+                int nr = rcap;
+                while (nr < need) nr *= 2;
+                nr = std::min(nr, MMAX_LAST / 2);
+                int64_t ncap = (int64_t)(2 * nr + 1) * (2 * nr + 1);
+                DBuf<cd> n1((size_t)ncap * np), n2((size_t)ncap * np);
+                if (computed >= 0) {
+                    int64_t used = (int64_t)(2 * computed + 1) * (2 * computed + 1);
+                    for (int p = 0; p < np; ++p) {
+                        HIPCHECK(hipMemcpyAsync(n1.p + p * ncap, t1.p + p * cap,
+                                                used * sizeof(cd), hipMemcpyDeviceToDevice, st));
+                        HIPCHECK(hipMemcpyAsync(n2.p + p * ncap, t2.p + p * cap,
+                                                used * sizeof(cd), hipMemcpyDeviceToDevice, st));
+                    }
+                }
+                sync(ctx);
+                t1 = std::move(n1);
+                t2 = std::move(n2);
+                rcap = nr;
+                cap = ncap;
+            }
+            launch_summands(d_tp.p, np, cap, computed + 1, need, t1.p, t2.p, st);
+            launch_ringsums(d_tp.p, np, cap, computed + 1, need, rstride, t1.p, t2.p, convtest,
+                            dflags.p, st);
+            d2h(ctx, hflags.data(), dflags.p, hflags.size());
+            sync(ctx);
+            computed = need;
+        }
+        bool all = true;
+        for (int p = 0; p < np; ++p) {
+            while (!decided[p] && seq[cand[p]] / 2 <= computed) {
+                int M = seq[cand[p]], mo2 = M / 2;
+                bool ok = true;
+                for (int r = mo2 - CONVREPEAT + 1; r <= mo2 && ok; ++r)
+                    ok = hflags[(size_t)p * rstride + r] != 0;
+                if (ok || M >= MMAX_LAST) {
+                    decided[p] = 1;
+                    out.M[p] = M;
+                    out.converged[p] = ok ? 1 : 0;
+                } else {
+                    ++cand[p];
+                }
+            }
+            if (!decided[p]) all = false;
+        }
+        if (all) break;
+    }
+    int Mmax = 0;
+    for (int p = 0; p < np; ++p) Mmax = std::max(Mmax, out.M[p]);
+    out.npts = np;
+    out.Mmax = Mmax;
+    const int64_t sq = (int64_t)Mmax * 2 * Mmax;
+    DBuf<cd> S((size_t)sq * np), A((size_t)sq * np), F((size_t)sq * np);
+    DBuf<int> dM(np), dmag(np);
+    std::vector<int> hmag(np);
+    std::vector<double> hpsi(2 * np), harea(np);
+    for (int p = 0; p < np; ++p) {
+        hmag[p] = in[p].tp.magnetic;
+        hpsi[2 * p] = in[p].psi1;
+        hpsi[2 * p + 1] = in[p].psi2;
+        harea[p] = in[p].area;
+    }
+    DBuf<double> dpsi(2 * np), darea(np);
+    dM.upload(out.M.data(), np, st);
+    dmag.upload(hmag.data(), np, st);
+    dpsi.upload(hpsi.data(), 2 * np, st);
+    darea.upload(harea.data(), np, st);
+    launch_gather_square(np, dM.p, Mmax, cap, t1.p, t2.p, S.p, sq, st);
+    const cd one = {1.0, 0.0}, zero = {0.0, 0.0};
+    for (int p = 0; p < np; ++p) {
+        const int M = out.M[p];
+        const cd *W = ctx->dft_matrix(M);
+        zgemm_batched(M, 2 * M, M, one, W, M, 0, S.p + p * sq, M, 0, zero, A.p + p * sq, M, 0, 1,
+                      st);
+        zgemm_batched(M, M, M, one, A.p + p * sq, M, (int64_t)M * M, W, M, 0, zero, F.p + p * sq,
+                      M, (int64_t)M * M, 2, st);
+    }
+    out.tab_stride = 2 * (int64_t)(Mmax + 2) * (Mmax + 2);
+    out.tabs.alloc((size_t)out.tab_stride * np);
+    launch_phase_ghost(np, dM.p, dpsi.p, dmag.p, darea.p, F.p, sq, out.tabs.p, out.tab_stride,
+                       Mmax, st);
+    sync(ctx);   // temporaries die at return
+}
+
+// --------------------------------------------------------- J/K spatial sums
+void ensure_singular(pssfss_ctx *ctx, pssfss_sheet *sh, double ul) {
+    if (sh->sing_valid && sh->sing_u == ul) return;
+    sh->rho_r.ensure(2 * sh->nufp);
+    sh->rinv.ensure(sh->nufp);
+    launch_singular(sh->dev(), sh->nufp, ul, sh->rho_r.p, sh->rinv.p, ctx->st);
+    sh->sing_valid = true;
+    sh->sing_u = ul;
+}
+
+// J/K integrals of nk phasings (all with the same u_local) into dst + i*6*nufp.
+void compute_jk(pssfss_ctx *ctx, pssfss_sheet *sh, double ul, const std::vector<double> &psis,
+                cd *dst, int mixed) {
+    const int nk = (int)psis.size() / 2;
+    double us1[2] = {ul * sh->s1[0], ul * sh->s1[1]};
+    double us2[2] = {ul * sh->s2[0], ul * sh->s2[1]};
+    DBuf<double> dpsi(psis.size());
+    dpsi.upload(psis.data(), psis.size(), ctx->st);
+    const int64_t stride = 6 * sh->nufp;
+    for (int g = 0; g < nk; g += 8) {
+        int nf = std::min(8, nk - g);
+        launch_jk(sh->dev(), sh->nufp, ul, us1, us2, dpsi.p + 2 * g, nf, dst + g * stride, stride,
+                  mixed, ctx->st);
+    }
+    sync(ctx);
+}
+
+bool key_matches(const pssfss_sheet *sh, bool valid, double k1, double k2, double ku,
+                 double psi1, double psi2, double ul) {
+    (void)sh;
+    if (!valid) return false;
+    if (psi1 != k1 || psi2 != k2 || ku == 0) return false;
+    return !(fabs((ul - ku) / ku) > TOL_U);
+}
+
+// Resolve the J/K data every point of a chunk uses.  Keys follow the reference's
+// sequential cache: a point reuses the most recent integrals when its (psi, u)
+// pass the test of src/FillZY.jl:97-98, otherwise new integrals are computed.
+// Returns per-point device pointers; the sheet's cache ends holding the last key.
+std::vector<const cd *> resolve_jk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts,
+                                   int np, int mixed, DBuf<cd> &slots, double &key_u) {
+    std::vector<int> slot(np);
+    std::vector<double> keys;   // psi1, psi2, u per new slot
+    bool valid = sh->jk_valid;
+    double k1 = sh->jk_psi1, k2 = sh->jk_psi2, ku = sh->jk_u;
+    int cur = -1;               // -1: the sheet's cached integrals
+    bool uses_old = false;
+    for (int p = 0; p < np; ++p) {
+        double ul = pts[p].u / sh->upm;
+        if (key_matches(sh, valid, k1, k2, ku, pts[p].psi1, pts[p].psi2, ul)) {
+            slot[p] = cur;
+            if (cur < 0) uses_old = true;
+        } else {
+            keys.push_back(pts[p].psi1);
+            keys.push_back(pts[p].psi2);
+            keys.push_back(ul);
+            cur = (int)keys.size() / 3 - 1;
+            valid = true;
+            k1 = pts[p].psi1; k2 = pts[p].psi2; ku = ul;
+            slot[p] = cur;
+        }
+    }
+    key_u = slot[0] < 0 ? sh->jk_u : keys[3 * slot[0] + 2];
+    const int nnew = (int)keys.size() / 3;
+    const int64_t stride = 6 * sh->nufp;
+    std::vector<const cd *> out(np);
+    if (nnew == 0) {
+        for (int p = 0; p < np; ++p) out[p] = sh->jk.p;
+        return out;
+    }
+    const int total = nnew + (uses_old ? 1 : 0);
+    slots.ensure((size_t)stride * total);
+    if (uses_old)   // the cache is overwritten below; keep the old integrals for these points
+        HIPCHECK(hipMemcpyAsync(slots.p + nnew * stride, sh->jk.p, stride * sizeof(cd),
+                                hipMemcpyDeviceToDevice, ctx->st));
+    int g0 = 0;     // consecutive keys with equal u share a launch
+    while (g0 < nnew) {
+        int g1 = g0;
+        std::vector<double> ps;
+        while (g1 < nnew && keys[3 * g1 + 2] == keys[3 * g0 + 2]) {
+            ps.push_back(keys[3 * g1]);
+            ps.push_back(keys[3 * g1 + 1]);
+            ++g1;
+        }
+        compute_jk(ctx, sh, keys[3 * g0 + 2], ps, slots.p + g0 * stride, mixed);
+        g0 = g1;
+    }
+    for (int p = 0; p < np; ++p)
+        out[p] = slot[p] < 0 ? slots.p + nnew * stride : slots.p + slot[p] * stride;
+    sh->jk.ensure(stride);
+    HIPCHECK(hipMemcpyAsync(sh->jk.p, slots.p + (nnew - 1) * stride, stride * sizeof(cd),
+                            hipMemcpyDeviceToDevice, ctx->st));
+    sh->jk_valid = true;
+    sh->jk_psi1 = keys[3 * (nnew - 1)];
+    sh->jk_psi2 = keys[3 * (nnew - 1) + 1];
+    sh->jk_u = keys[3 * (nnew - 1) + 2];
+    return out;
+}
+
+// ---------------------------------------------------------------- the fill
+FillParams make_fill_params(const pssfss_sheet *sh, const pssfss_point *pt) {
+    FillParams F;
+    memset(&F, 0, sizeof(F));
+    const bool mag = sh->cls == 'M';
+    const int s = pt->s;
+    cd e1 = lay(pt->epsr, s - 1), m1 = lay(pt->mur, s - 1);
+    cd e2 = lay(pt->epsr, s), m2 = lay(pt->mur, s);
+    const double k0 = pt->k0, u = pt->u;
+    const double MU0 = 1.25663706212e-6, C0 = 299792458.0;
+    F.k0 = k0;
+    F.u = u;
+    F.omega = k0 * C0;
+    F.psi1 = pt->psi1;
+    F.psi2 = pt->psi2;
+    F.b1[0] = sh->b1[0] * sh->upm; F.b1[1] = sh->b1[1] * sh->upm;
+    F.b2[0] = sh->b2[0] * sh->upm; F.b2[1] = sh->b2[1] * sh->upm;
+    F.mut = ((2.0 * m1) * m2) / (m1 + m2);
+    if (!mag) {
+        F.epsb = (e1 + e2) / 2.0;
+        F.c3 = c3_calc(k0, u, m1, e1, m2, e2);
+        F.d3 = d3_calc(k0, u, m1, e1, m2, e2);
+        F.psf_base = cd{0.0, 1.0} / F.epsb;
+    } else {
+        F.epsb = 0.5 * (e1 + e2);
+        cd c3s = c3_calc(k0, u, m1, e1, m1, e1), c3sp1 = c3_calc(k0, u, m2, e2, m2, e2);
+        cd d3s = d3_calc(k0, u, m1, e1, m1, e1), d3sp1 = d3_calc(k0, u, m2, e2, m2, e2);
+        F.I1fact = PI_ / F.epsb;
+        F.KFfact = (c3s * e1 + c3sp1 * e2) / ((2.0 * F.epsb) * u);
+        F.I2fact = PI_ * F.mut;
+        F.KPfact = (F.mut / (2.0 * u)) * (d3s / m1 + d3sp1 / m2);
+        F.psf_base = cd{0.0, 2.0} / (((PI_ * F.omega) * MU0) * F.mut);
+    }
+    F.ff[0] = cd{1, 0};
+    F.ff[1] = cd{1, 0};
+    F.ff[2] = hcis(-pt->psi1);
+    F.ff[3] = cd{1, 0};
+    F.ff[4] = hcis(-pt->psi2);
+    return F;
+}
+
+struct ChunkResult {
+    std::vector<int> M, conv;
+};
+
+// Fill np points into ctx->wZ (row-major, one nbf^2 block per point).
+void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int np, int mixed,
+                ChunkResult &res) {
+    hipStream_t st = ctx->st;
+    const double ul = pts[0].u / sh->upm;
+    for (int p = 1; p < np; ++p)
+        if (pts[p].u != pts[0].u) throw ArgError("batched points must share u");
+    DBuf<cd> slots;
+    double key_u = ul;
+    std::vector<const cd *> jkp = resolve_jk(ctx, sh, pts, np, mixed, slots, key_u);
+    ensure_singular(ctx, sh, key_u);
+    std::vector<TabInput> tin(np);
+    for (int p = 0; p < np; ++p) tin[p] = sheet_tab_input(sh, &pts[p]);
+    TableSet tabs;
+    build_tables(ctx, tin, CONVTEST_FILL, tabs);
+    res.M = tabs.M;
+    res.conv = tabs.converged;
+    const int64_t nb2 = sh->nbf * sh->nbf;
+    ctx->wZ.ensure((size_t)nb2 * np);
+    launch_zero(ctx->wZ.p, nb2 * np, st);
+    std::vector<FillParams> fp(np);
+    for (int p = 0; p < np; ++p) {
+        fp[p] = make_fill_params(sh, &pts[p]);
+        fp[p].M = tabs.M[p];
+        fp[p].tab1 = tabs.tab1(p);
+        fp[p].tab2 = tabs.tab2(p);
+        fp[p].jk = jkp[p];
+        fp[p].Z = ctx->wZ.p + p * nb2;
+    }
+    DBuf<FillParams> dfp(np);
+    dfp.upload(fp.data(), np, st);
+    launch_fill(sh->dev(), dfp.p, np, sh->rho_r.p, sh->rinv.p, sh->cls == 'M' ? 1 : 0, st);
+    sync(ctx);
+}
+
+// Solve the np systems in ctx->wZ for the right-hand sides in ctx->wB (nbf x ncol
+// column-major per point), leaving X in ctx->wB.
+void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
+                 pssfss_point_info *infos) {
+    hipStream_t st = ctx->st;
+    const int64_t nb2 = n * n, sB = n * ncol;
+    ctx->wpiv.ensure((size_t)n * np);
+    if (precision == PSSFSS_FP64) {
+        lu_factor_batched(n, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, st);
+        lu_solve_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, ctx->wB.p, sB, st);
+        sync(ctx);
+        if (infos)
+            for (int p = 0; p < np; ++p) { infos[p].refine_iters = 0; infos[p].refine_resid = 0; }
+        return;
+    }
+    // FP32 LU with FP64 iterative refinement.
+    ctx->wZf.ensure((size_t)nb2 * np);
+    ctx->wX.ensure((size_t)sB * np);
+    ctx->wR.ensure((size_t)sB * np);
+    ctx->wRf.ensure((size_t)sB * np);
+    ctx->wnorm.ensure((size_t)4 * np * ncol + 8);
+    convert_z_to_f32(ctx->wZ.p, ctx->wZf.p, nb2 * np, st);
+    lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st);
+    double *rn = ctx->wnorm.p, *xn = ctx->wnorm.p + np * ncol;
+    std::vector<double> hr(np * ncol), hx(np * ncol), hb(np * ncol);
+    auto residual = [&](const cd *X) {
+        for (int c0 = 0; c0 < ncol; c0 += 8) {
+            int nr = std::min(8, ncol - c0);
+            // norms are per (system, rhs) over a strided sub-block
+            residual_batched(n, nr, np, ctx->wZ.p, nb2, X + c0 * n, ctx->wB.p + c0 * n, sB,
+                             ctx->wR.p + c0 * n, rn, xn, st);
+            std::vector<double> tr(np * nr), tx(np * nr);
+            d2h(ctx, tr.data(), rn, np * nr);
+            d2h(ctx, tx.data(), xn, np * nr);
+            sync(ctx);
+            for (int p = 0; p < np; ++p)
+                for (int c = 0; c < nr; ++c) {
+                    hr[p * ncol + c0 + c] = tr[p * nr + c];
+                    hx[p * ncol + c0 + c] = tx[p * nr + c];
+                }
+        }
+    };
+    // |B| via the residual of X = 0
+    launch_zero(ctx->wX.p, sB * np, st);
+    residual(ctx->wX.p);
+    hb = hr;
+    convert_z_to_f32(ctx->wB.p, ctx->wRf.p, sB * np, st);
+    lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, ctx->wRf.p, sB, st);
+    add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
+    std::vector<double> prev(np, 1e300), best(np, 1e300);
+    std::vector<int> iters(np, 0), done(np, 0), diverged(np, 0);
+    const int maxit = 10;
+    for (int it = 0; it < maxit; ++it) {
+        residual(ctx->wX.p);
+        bool all = true;
+        for (int p = 0; p < np; ++p) {
+            double r = 0;
+            for (int c = 0; c < ncol; ++c) {
+                double bb = hb[p * ncol + c] > 0 ? hb[p * ncol + c] : 1.0;
+                r = std::max(r, hr[p * ncol + c] / bb);
+            }
+            if (!done[p]) {
+                iters[p] = it + 1;
+                if (r <= 1e-15 || r >= 0.5 * prev[p]) done[p] = 1;
+                if (r > 2 * best[p] && it > 0) diverged[p] = 1;
+                best[p] = std::min(best[p], r);
+                prev[p] = r;
+            }
+            if (!done[p]) all = false;
+        }
+        if (all) break;
+        residual_to_f32(ctx->wR.p, ctx->wRf.p, sB * np, st);
+        lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, ctx->wRf.p, sB, st);
+        add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
+    }
+    // Points whose refinement did not reach an FP64-level residual: FP64 LU.
+    std::vector<int> fb;
+    for (int p = 0; p < np; ++p)
+        if (diverged[p] || best[p] > 1e-10) fb.push_back(p);
+    for (int p : fb) {
+        lu_factor_batched(n, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n, st);
+        HIPCHECK(hipMemcpyAsync(ctx->wX.p + p * sB, ctx->wB.p + p * sB, sB * sizeof(cd),
+                                hipMemcpyDeviceToDevice, st));
+        lu_solve_batched(n, ncol, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n,
+                         ctx->wX.p + p * sB, sB, st);
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->wB.p, ctx->wX.p, (size_t)sB * np * sizeof(cd),
+                            hipMemcpyDeviceToDevice, st));
+    sync(ctx);
+    if (infos)
+        for (int p = 0; p < np; ++p) {
+            infos[p].refine_iters = iters[p];
+            infos[p].refine_resid = best[p];
+            infos[p].fallback_fp64 = 0;
+        }
+    if (infos)
+        for (int p : fb) infos[p].fallback_fp64 = 1;
+}
+
+// One chunk of the sheet GSM pipeline.
+void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfss_gsm *gsms,
+               int np, int precision, pssfss_point_info *infos) {
+    hipStream_t st = ctx->st;
+    const bool mag = sh->cls == 'M';
+    const int n1 = pts[0].nmodes[0], n2 = pts[0].nmodes[1];
+    for (int p = 1; p < np; ++p)
+        if (pts[p].nmodes[0] != n1 || pts[p].nmodes[1] != n2)
+            throw ArgError("batched points must have equal mode counts");
+    const int ncol = n1 + n2;
+    const int64_t n = sh->nbf;
+    ChunkResult cr;
+    fill_chunk(ctx, sh, pts, np, precision == PSSFSS_MIXED, cr);
+    // Fourier-transform jobs with the reuse rule of src/PSSFSS.jl:407-423.
+    std::vector<BfftJob> jobs;
+    std::vector<std::vector<int>> jobof(np, std::vector<int>(ncol));
+    for (int p = 0; p < np; ++p) {
+        const pssfss_point &pt = pts[p];
+        const int nm[2] = {n1, n2};
+        for (int sr = 0; sr < 2; ++sr)
+            for (int qp = 0; qp < nm[sr]; ++qp) {
+                const double *k = pt.beta[sr] + 2 * qp;
+                int col = sr == 0 ? qp : n1 + qp;
+                if (qp > 0 && approx2(k, pt.beta[sr] + 2 * (qp - 1))) {
+                    jobof[p][col] = jobof[p][col - 1];
+                    continue;
+                }
+                if (sr == 1 && n1 > qp && approx2(k, pt.beta[0] + 2 * qp)) {
+                    jobof[p][col] = jobof[p][qp];
+                    continue;
+                }
+                BfftJob J;
+                J.k[0] = k[0];
+                J.k[1] = k[1];
+                J.ff[0] = cd{1, 0}; J.ff[1] = cd{1, 0}; J.ff[2] = hcis(-pt.psi1);
+                J.ff[3] = cd{1, 0}; J.ff[4] = hcis(-pt.psi2);
+                jobof[p][col] = (int)jobs.size();
+                jobs.push_back(J);
+            }
+    }
+    const int njobs = (int)jobs.size();
+    DBuf<BfftJob> djobs(njobs);
+    djobs.upload(jobs.data(), njobs, st);
+    ctx->wft.ensure((size_t)njobs * n * 2);
+    launch_bfft(sh->dev(), sh->bf_face.p, sh->bf_loc.p, djobs.p, njobs, ctx->wft.p, st);
+    // Right-hand sides, eq. (7.39) / (7.64).
+    std::vector<int32_t> coljob(np * ncol);
+    std::vector<cd> colsrc(2 * np * ncol);
+    for (int p = 0; p < np; ++p) {
+        const pssfss_point &pt = pts[p];
+        for (int col = 0; col < ncol; ++col) {
+            int sr = col < n1 ? 0 : 1, qp = col < n1 ? col : col - n1;
+            cd vinc = lay(pt.vinc, qp + pt.nmax * sr);
+            cd c = lay(pt.c[sr], qp);
+            double tx = pt.tvec[sr][2 * qp], ty = pt.tvec[sr][2 * qp + 1];
+            cd s0, s1;
+            if (!mag) {
+                cd f = (vinc * c) * pt.acf[sr];
+                s0 = f * tx;
+                s1 = f * ty;
+            } else {
+                cd f = (vinc * c) * lay(pt.Y[sr], qp);
+                s0 = f * (-ty);
+                s1 = f * tx;
+            }
+            coljob[p * ncol + col] = jobof[p][col];
+            colsrc[2 * (p * ncol + col)] = s0;
+            colsrc[2 * (p * ncol + col) + 1] = s1;
+        }
+    }
+    DBuf<int32_t> dcj(coljob.size());
+    DBuf<cd> dcs(colsrc.size());
+    dcj.upload(coljob.data(), coljob.size(), st);
+    dcs.upload(colsrc.data(), colsrc.size(), st);
+    ctx->wB.ensure((size_t)n * ncol * np);
+    launch_rhs(n, np * ncol, ctx->wft.p, dcj.p, dcs.p, ctx->wB.p, n, st);
+    solve_chunk(ctx, n, ncol, np, precision, infos);
+    // Projection of the solutions onto the modes' transforms.
+    ctx->wout.ensure((size_t)ncol * njobs * 2 * np);
+    std::vector<cd> hout((size_t)np * ncol * njobs * 2);
+    // one launch per point: columns of point p against all jobs
+    for (int p = 0; p < np; ++p)
+        launch_project(n, ncol, njobs, ctx->wB.p + (int64_t)p * n * ncol, n, ctx->wft.p,
+                       ctx->wout.p + (int64_t)p * ncol * njobs * 2, st);
+    d2h(ctx, hout.data(), ctx->wout.p, hout.size());
+    sync(ctx);
+    // Scattered-field GSM entries, eqs. (6.18) / (6.37).
+    const double area = sh->s1[0] * sh->s2[1] - sh->s1[1] * sh->s2[0];
+    const double area_m2 = fabs(area) / (sh->upm * sh->upm);
+    for (int p = 0; p < np; ++p) {
+        const pssfss_point &pt = pts[p];
+        pssfss_gsm &g = gsms[p];
+        const int nm[2] = {n1, n2};
+        double *blk[2][2] = {{g.s11, g.s12}, {g.s21, g.s22}};
+        for (int col = 0; col < ncol; ++col) {
+            int sr = col < n1 ? 0 : 1, qp = col < n1 ? col : col - n1;
+            double sigma = sr == 0 ? 1.0 : -1.0;
+            for (int orr = 0; orr < 2; ++orr) {
+                double *sm = blk[orr][sr];
+                if (!sm) continue;
+                for (int q = 0; q < nm[orr]; ++q) {
+                    int ocol = orr == 0 ? q : n1 + q;
+                    int job = jobof[p][ocol];
+                    const cd *FT = &hout[(((size_t)p * ncol + col) * njobs + job) * 2];
+                    double tx = pt.tvec[orr][2 * q], ty = pt.tvec[orr][2 * q + 1];
+                    cd tl = lay(pt.tlgf, q + pt.nmax * orr);
+                    cd cq = lay(pt.c[orr], q);
+                    cd *e = (cd *)(sm + 2 * (q + (int64_t)nm[orr] * qp));
+                    if (!mag) {
+                        cd dot = tx * FT[0] + ty * FT[1];
+                        *e = *e - dot * (tl / ((cq * pt.acf[orr]) * area_m2));
+                    } else {
+                        cd dot = (-ty) * FT[0] + tx * FT[1];
+                        *e = *e + dot * ((sigma * tl) * cq);
+                    }
+                }
+            }
+        }
+        if (infos) {
+            infos[p].mmax = cr.M[p];
+            infos[p].table_converged = cr.conv[p];
+            infos[p].jk_rings_max = 0;
+        }
+    }
+}
+
+int chunk_size(pssfss_ctx *ctx, const pssfss_sheet *sh, int npts, int precision) {
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+        (void)hipGetLastError();
+        freeb = (size_t)8 << 30;
+    }
+    double per = 16.0 * sh->nbf * sh->nbf * (precision == PSSFSS_MIXED ? 1.5 : 1.0) +
+                 96.0 * sh->nufp + 4.0e6;
+    double budget = 0.45 * (double)freeb;
+    int c = (int)std::max(1.0, std::floor(budget / per));
+    (void)ctx;
+    return std::min(std::min(c, 32), npts);
+}
+
+void upload_sheet(pssfss_ctx *ctx, const pssfss_sheet_desc *d, pssfss_sheet *sh) {
+    if (!d) throw ArgError("null sheet descriptor");
+    if (d->nface <= 0 || d->nnode <= 0 || d->nedge <= 0 || d->nbf <= 0)
+        throw ArgError("sheet: sizes must be positive");
+    if (!d->rho || !d->fv || !d->fe || !d->bff || !d->ebf || !d->eci)
+        throw ArgError("sheet: null arrays");
+    if (d->sheet_class != 'J' && d->sheet_class != 'M')
+        throw ArgError("sheet: class must be 'J' or 'M'");
+    if (!(d->units_per_meter > 0)) throw ArgError("sheet: units_per_meter must be positive");
+    const int64_t nf = d->nface, ne = d->nedge, nb = d->nbf, nn = d->nnode;
+    if (nb > 200000 || nf > 2000000) throw ArgError("sheet: too large");
+    sh->nnode = nn; sh->nface = nf; sh->nedge = ne; sh->nbf = nb;
+    sh->cls = d->sheet_class;
+    sh->upm = d->units_per_meter;
+    for (int i = 0; i < 2; ++i) {
+        sh->s1[i] = d->s1[i]; sh->s2[i] = d->s2[i];
+        sh->b1[i] = d->beta1[i]; sh->b2[i] = d->beta2[i];
+    }
+    std::vector<double> fxs(6 * nf), fxm(6 * nf), fr(nf, 0.0);
+    std::vector<int32_t> fbf(3 * nf), fsg(3 * nf), feci(3 * nf), fedge(3 * nf);
+    for (int64_t f = 0; f < nf; ++f) {
+        for (int i = 0; i < 3; ++i) {
+            int64_t v = d->fv[3 * f + i] - 1;
+            if (v < 0 || v >= nn) throw ArgError("sheet: fv index out of range");
+            fxs[6 * f + 2 * i] = d->rho[2 * v];
+            fxs[6 * f + 2 * i + 1] = d->rho[2 * v + 1];
+            fxm[6 * f + 2 * i] = d->rho[2 * v] / sh->upm;
+            fxm[6 * f + 2 * i + 1] = d->rho[2 * v + 1] / sh->upm;
+            int64_t e = d->fe[3 * f + i] - 1;
+            if (e < 0 || e >= ne) throw ArgError("sheet: fe index out of range");
+            int64_t b = d->ebf[e] - 1;
+            if (b < -1 || b >= nb) throw ArgError("sheet: ebf index out of range");
+            int64_t ec = d->eci[e];
+            if (ec < 0 || ec > 4) throw ArgError("sheet: eci must be 0..4");
+            fbf[3 * f + i] = (int32_t)b;
+            feci[3 * f + i] = (int32_t)ec;
+            fedge[3 * f + i] = (int32_t)e;
+            if (b >= 0) {
+                int64_t pf = d->bff[2 * b] - 1, mf = d->bff[2 * b + 1] - 1;
+                if (pf == f) fsg[3 * f + i] = 1;
+                else if (mf == f) fsg[3 * f + i] = -1;
+                else throw ArgError("sheet: basis function does not touch its face");
+            } else {
+                fsg[3 * f + i] = 0;
+            }
+        }
+        if (d->fr) fr[f] = d->fr[f];
+    }
+    // the two half-basis functions of every basis function
+    std::vector<int32_t> bface(2 * nb, -1), bloc(2 * nb, -1);
+    std::vector<int> cnt(nb, 0);
+    for (int64_t f = 0; f < nf; ++f)
+        for (int i = 0; i < 3; ++i) {
+            int b = fbf[3 * f + i];
+            if (b < 0) continue;
+            if (cnt[b] >= 2) throw ArgError("sheet: basis function with more than two halves");
+            bface[2 * b + cnt[b]] = (int32_t)f;
+            bloc[2 * b + cnt[b]] = i;
+            ++cnt[b];
+        }
+    for (int64_t b = 0; b < nb; ++b)
+        if (cnt[b] != 2) throw ArgError("sheet: basis function without two halves");
+    if (d->ufp_offsets) {
+        if (!d->ufp_members) throw ArgError("sheet: ufp_members missing");
+        sh->identity = false;
+        sh->nufp = d->nufp;
+        std::vector<int64_t> off(d->ufp_offsets, d->ufp_offsets + d->nufp + 1);
+        int64_t nm = off[d->nufp];
+        if (off[0] != 0 || nm != nf * nf) throw ArgError("sheet: ufp classes must cover nface^2");
+        std::vector<int64_t> mem(nm);
+        for (int64_t i = 0; i < nm; ++i) {
+            int64_t v = d->ufp_members[i] - 1;
+            if (v < 0 || v >= nf * nf) throw ArgError("sheet: ufp member out of range");
+            mem[i] = v;
+        }
+        for (int64_t i = 0; i < d->nufp; ++i)
+            if (off[i + 1] <= off[i]) throw ArgError("sheet: empty ufp class");
+        sh->uoff.upload(off.data(), off.size(), ctx->st);
+        sh->umem.upload(mem.data(), mem.size(), ctx->st);
+    } else {
+        if (d->nufp != nf * nf) throw ArgError("sheet: nufp must be nface^2 without classes");
+        sh->identity = true;
+        sh->nufp = nf * nf;
+    }
+    hipStream_t st = ctx->st;
+    sh->fxs.upload(fxs.data(), fxs.size(), st);
+    sh->fxm.upload(fxm.data(), fxm.size(), st);
+    sh->fr.upload(fr.data(), fr.size(), st);
+    sh->fbf.upload(fbf.data(), fbf.size(), st);
+    sh->fsg.upload(fsg.data(), fsg.size(), st);
+    sh->feci.upload(feci.data(), feci.size(), st);
+    sh->fedge.upload(fedge.data(), fedge.size(), st);
+    sh->bf_face.upload(bface.data(), bface.size(), st);
+    sh->bf_loc.upload(bloc.data(), bloc.size(), st);
+    sync(ctx);
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int pssfss_ctx_create(int device, pssfss_ctx **out) {
+    if (!out) return PSSFSS_EINVAL;
+    *out = nullptr;
+    pssfss_ctx *c = new (std::nothrow) pssfss_ctx();
+    if (!c) return PSSFSS_ENOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return PSSFSS_EHIP;
+    }
+    *out = c;
+    return PSSFSS_OK;
+}
+
+int pssfss_ctx_destroy(pssfss_ctx *ctx) {
+    if (!ctx) return PSSFSS_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->st);
+    ctx->dft.clear();
+    (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+    return PSSFSS_OK;
+}
+
+const char *pssfss_last_error(const pssfss_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+int pssfss_device_info(pssfss_ctx *ctx, char *name, int name_len, int *num_cu, double *hbm_gib) {
+    return guarded(ctx, [&] {
+        hipDeviceProp_t p;
+        HIPCHECK(hipGetDeviceProperties(&p, ctx->device));
+        if (name && name_len > 0) {
+            snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+        }
+        if (num_cu) *num_cu = p.multiProcessorCount;
+        if (hbm_gib) *hbm_gib = (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0);
+    });
+}
+
+int pssfss_sheet_upload(pssfss_ctx *ctx, const pssfss_sheet_desc *desc, pssfss_sheet **out) {
+    return guarded(ctx, [&] {
+        if (!out) throw ArgError("null output");
+        *out = nullptr;
+        HIPCHECK(hipSetDevice(ctx->device));
+        pssfss_sheet *sh = new pssfss_sheet();
+        try {
+            upload_sheet(ctx, desc, sh);
+        } catch (...) {
+            delete sh;
+            throw;
+        }
+        *out = sh;
+    });
+}
+
+int pssfss_sheet_free(pssfss_ctx *ctx, pssfss_sheet *sheet) {
+    return guarded(ctx, [&] {
+        sync(ctx);
+        delete sheet;
+    });
+}
+
+int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
+                           const pssfss_point *pts, pssfss_gsm *gsms, int precision,
+                           pssfss_point_info *infos) {
+    return guarded(ctx, [&] {
+        if (!sheet || !pts || !gsms || npts < 0) throw ArgError("null argument");
+        if (precision != PSSFSS_FP64 && precision != PSSFSS_MIXED)
+            throw ArgError("precision must be PSSFSS_FP64 or PSSFSS_MIXED");
+        HIPCHECK(hipSetDevice(ctx->device));
+        for (int p = 0; p < npts; ++p) check_point(sheet, &pts[p], true);
+        if (infos) memset(infos, 0, sizeof(pssfss_point_info) * npts);
+        int c = npts > 0 ? chunk_size(ctx, sheet, npts, precision) : 1;
+        for (int p0 = 0; p0 < npts; p0 += c) {
+            int np = std::min(c, npts - p0);
+            gsm_chunk(ctx, sheet, pts + p0, gsms + p0, np, precision, infos ? infos + p0 : nullptr);
+        }
+    });
+}
+
+int pssfss_sheet_gsm(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt,
+                     pssfss_gsm *gsm, int precision, pssfss_point_info *info) {
+    return pssfss_sheet_gsm_batch(ctx, sheet, 1, pt, gsm, precision, info);
+}
+
+int pssfss_fill(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt, double *zmat,
+                pssfss_point_info *info) {
+    return guarded(ctx, [&] {
+        if (!sheet || !zmat) throw ArgError("null argument");
+        HIPCHECK(hipSetDevice(ctx->device));
+        check_point(sheet, pt, false);
+        ChunkResult cr;
+        fill_chunk(ctx, sheet, pt, 1, 0, cr);
+        const int64_t n = sheet->nbf;
+        std::vector<cd> h((size_t)n * n);
+        d2h(ctx, h.data(), ctx->wZ.p, h.size());
+        sync(ctx);
+        cd *o = (cd *)zmat;
+        for (int64_t m = 0; m < n; ++m)
+            for (int64_t s = 0; s < n; ++s) o[m + n * s] = h[m * n + s];
+        if (info) {
+            memset(info, 0, sizeof(*info));
+            info->mmax = cr.M[0];
+            info->table_converged = cr.conv[0];
+        }
+    });
+}
+
+int pssfss_filljk(pssfss_ctx *ctx, pssfss_sheet *sheet, double u_local, double psi1,
+                  double psi2, double *jk, double *rho_r, double *rinv) {
+    return guarded(ctx, [&] {
+        if (!sheet || !jk || !rho_r || !rinv) throw ArgError("null argument");
+        if (!(u_local > 0)) throw ArgError("u_local must be positive");
+        HIPCHECK(hipSetDevice(ctx->device));
+        const int64_t nu = sheet->nufp;
+        DBuf<cd> buf((size_t)6 * nu);
+        compute_jk(ctx, sheet, u_local, {psi1, psi2}, buf.p, 0);
+        ensure_singular(ctx, sheet, u_local);
+        std::vector<cd> h((size_t)6 * nu);
+        d2h(ctx, h.data(), buf.p, h.size());
+        d2h(ctx, rho_r, sheet->rho_r.p, 2 * nu);
+        d2h(ctx, rinv, sheet->rinv.p, nu);
+        sync(ctx);
+        cd *o = (cd *)jk;
+        for (int64_t i = 0; i < nu; ++i)
+            for (int q = 0; q < 6; ++q) o[6 * i + q] = h[q * nu + i];
+    });
+}
+
+int pssfss_modal_tables(pssfss_ctx *ctx, int magnetic, const pssfss_point *pt,
+                        const double *beta1, const double *beta2, const double *beta00,
+                        double convtest, int *mmax, int *converged, double *tab1,
+                        double *tab2) {
+    return guarded(ctx, [&] {
+        if (!pt || !beta1 || !beta2 || !beta00 || !tab1 || !tab2) throw ArgError("null argument");
+        HIPCHECK(hipSetDevice(ctx->device));
+        check_point(nullptr, pt, false);
+        std::vector<TabInput> in = {make_tab_input(pt, magnetic != 0, beta1, beta2, beta00)};
+        TableSet ts;
+        build_tables(ctx, in, convtest, ts);
+        const int64_t E = (int64_t)ts.M[0] + 2;
+        d2h(ctx, (cd *)tab1, ts.tab1(0), (size_t)(E * E));
+        d2h(ctx, (cd *)tab2, ts.tab2(0), (size_t)(E * E));
+        sync(ctx);
+        if (mmax) *mmax = ts.M[0];
+        if (converged) *converged = ts.converged[0];
+    });
+}
+
+int pssfss_rwgbfft(pssfss_ctx *ctx, pssfss_sheet *sheet, const double *k, double psi1,
+                   double psi2, double *ft) {
+    return guarded(ctx, [&] {
+        if (!sheet || !k || !ft) throw ArgError("null argument");
+        HIPCHECK(hipSetDevice(ctx->device));
+        BfftJob J;
+        J.k[0] = k[0];
+        J.k[1] = k[1];
+        J.ff[0] = cd{1, 0}; J.ff[1] = cd{1, 0}; J.ff[2] = hcis(-psi1);
+        J.ff[3] = cd{1, 0}; J.ff[4] = hcis(-psi2);
+        DBuf<BfftJob> dj(1);
+        dj.upload(&J, 1, ctx->st);
+        DBuf<cd> out((size_t)2 * sheet->nbf);
+        launch_bfft(sheet->dev(), sheet->bf_face.p, sheet->bf_loc.p, dj.p, 1, out.p, ctx->st);
+        d2h(ctx, (cd *)ft, out.p, (size_t)2 * sheet->nbf);
+        sync(ctx);
+    });
+}
+
+int pssfss_lu_solve_batch(pssfss_ctx *ctx, int64_t n, int nrhs, int nbatch, const double *A,
+                          double *B, int precision, int *iters, double *resid) {
+    return guarded(ctx, [&] {
+        if (n <= 0 || nrhs <= 0 || nbatch <= 0 || !A || !B) throw ArgError("bad arguments");
+        if (precision != PSSFSS_FP64 && precision != PSSFSS_MIXED)
+            throw ArgError("precision must be PSSFSS_FP64 or PSSFSS_MIXED");
+        HIPCHECK(hipSetDevice(ctx->device));
+        const int64_t nb2 = n * n;
+        std::vector<cd> rm((size_t)nb2 * nbatch);
+        const cd *a = (const cd *)A;
+        for (int b = 0; b < nbatch; ++b)
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t j = 0; j < n; ++j) rm[b * nb2 + i * n + j] = a[b * nb2 + i + n * j];
+        ctx->wZ.upload(rm.data(), rm.size(), ctx->st);
+        ctx->wB.upload((const cd *)B, (size_t)n * nrhs * nbatch, ctx->st);
+        std::vector<pssfss_point_info> inf(nbatch);
+        solve_chunk(ctx, n, nrhs, nbatch, precision, inf.data());
+        d2h(ctx, (cd *)B, ctx->wB.p, (size_t)n * nrhs * nbatch);
+        sync(ctx);
+        for (int b = 0; b < nbatch; ++b) {
+            if (iters) iters[b] = inf[b].refine_iters;
+            if (resid) resid[b] = inf[b].refine_resid;
+        }
+    });
+}
+
+int pssfss_cascade_batch(pssfss_ctx *ctx, int nbatch, int n1, int n2, int n3, const double *a,
+                         const double *b, double *c) {
+    return guarded(ctx, [&] {
+        if (nbatch <= 0 || n1 <= 0 || n2 <= 0 || n3 <= 0 || !a || !b || !c)
+            throw ArgError("bad arguments");
+        HIPCHECK(hipSetDevice(ctx->device));
+        const int64_t la = (int64_t)n1 * n1 + 2 * (int64_t)n1 * n2 + (int64_t)n2 * n2;
+        const int64_t lb = (int64_t)n2 * n2 + 2 * (int64_t)n2 * n3 + (int64_t)n3 * n3;
+        const int64_t lc = (int64_t)n1 * n1 + 2 * (int64_t)n1 * n3 + (int64_t)n3 * n3;
+        DBuf<cd> da((size_t)la * nbatch), db((size_t)lb * nbatch), dc((size_t)lc * nbatch);
+        da.upload((const cd *)a, (size_t)la * nbatch, ctx->st);
+        db.upload((const cd *)b, (size_t)lb * nbatch, ctx->st);
+        cascade_batched_dev(nbatch, n1, n2, n3, da.p, db.p, dc.p, ctx->st);
+        d2h(ctx, (cd *)c, dc.p, (size_t)lc * nbatch);
+        sync(ctx);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cplx.hpp"
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                    \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+// Device buffer with RAII ownership.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw HipError("hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void ensure(size_t count) { if (n < count) alloc(count); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t st) {
+        ensure(count);
+        HIPCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+};
+
+constexpr int MAXL = 16;        // layers per Gblock region
+constexpr int JK_RT = 10;       // rings whose phases are tabulated in LDS
+constexpr int JK_RINGMAX = 65;  // src/PGF.jl:15
+
+// Geometry of a sheet on the device (structure of arrays, 0-based indices).
+struct SheetDev {
+    int64_t nface, nbf, nufp;
+    int identity;               // 1: class i is face pair i
+    const double *fxs;          // nface*6: vertex (x,y) x 3, sheet units
+    const double *fxm;          // nface*6: same in meters
+    const int32_t *fbf;         // nface*3: basis function of local edge, -1 none
+    const int32_t *fsg;         // nface*3: +1 plus face, -1 minus face
+    const int32_t *feci;        // nface*3: edge cell code 0..4
+    const int32_t *fedge;       // nface*3: global edge index
+    const double *fr;           // nface
+    const int64_t *uoff;        // CSR offsets (nufp+1) or null
+    const int64_t *umem;        // CSR members (0-based face pairs)
+};
+
+// Per-point spectral-table parameters.
+struct TabParams {
+    double k0sq, u, b2min;
+    double b00[2], b1[2], b2[2];
+    int nl, s, magnetic, pad;
+    cd eps[MAXL], mu[MAXL];
+    double width[MAXL];
+    cd mut, epsb, c3, d3, f1, f3, p1, p3;
+};
+
+// Per-point fill parameters.
+struct FillParams {
+    double k0, u, omega, psi1, psi2;
+    double b1[2], b2[2];        // 1/m
+    cd mut, epsb, c3, d3;
+    cd I1fact, KFfact, I2fact, KPfact;
+    cd psf_base;                // electric: im/epsb;  magnetic: 2im/(pi*omega*mu0*mut)
+    cd ff[5];
+    int M, pad;
+    const cd *tab1, *tab2;      // (M+2)^2 each
+    const cd *jk;               // 6*nufp: J, Jxi, Jeta, K, Kxi, Keta (SoA)
+    cd *Z;                      // nbf*nbf
+};
+
+// ---- pgf.hip
+void launch_summands(const TabParams *d_tp, int npts, int64_t cap_entries, int r0, int r1,
+                     cd *t1, cd *t2, hipStream_t st);
+void launch_ringsums(const TabParams *d_tp, int npts, int64_t cap_entries, int r0, int r1,
+                     int rcap, const cd *t1, const cd *t2, double convtest, int *flags,
+                     hipStream_t st);
+void launch_gather_square(int npts, const int *d_M, int Mmax, int64_t cap_entries,
+                          const cd *t1, const cd *t2, cd *S, int64_t s_stride,
+                          hipStream_t st);
+void launch_dft_matrix(int M, cd *W, hipStream_t st);
+void launch_phase_ghost(int npts, const int *d_M, const double *d_psi, const int *d_mag,
+                        const double *d_area, const cd *F, int64_t f_stride, cd *tab,
+                        int64_t tab_stride, int Mmax, hipStream_t st);
+void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us1,
+               const double *us2, const double *d_psi, int nf, cd *jk, int64_t jk_stride,
+               int mixed, hipStream_t st);
+void launch_singular(const SheetDev &sh, int64_t nufp, double ulocal, double *rho_r,
+                     double *rinv, hipStream_t st);
+
+// ---- fill.hip
+void launch_zero(cd *p, int64_t n, hipStream_t st);
+void launch_fill(const SheetDev &sh, const FillParams *d_fp, int npts, const double *rho_r,
+                 const double *rinv, int magnetic, hipStream_t st);
+
+// ---- linalg.hip
+void zgemm_batched(int m, int n, int k, cd alpha, const cd *A, int64_t lda, int64_t sA,
+                   const cd *B, int64_t ldb, int64_t sB, cd beta, cd *C, int64_t ldc,
+                   int64_t sC, int batch, hipStream_t st);
+void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                       hipStream_t st);
+void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
+                      const int32_t *ipiv, int64_t sP, cd *B, int64_t sB, hipStream_t st);
+void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv,
+                           int64_t sP, hipStream_t st);
+void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
+                          const int32_t *ipiv, int64_t sP, cf *B, int64_t sB, hipStream_t st);
+void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st);
+// r = b - A x (FP64), batched; then per-(system, rhs) norms of r and of x.
+void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
+                      const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
+                      hipStream_t st);
+void residual_to_f32(const cd *R, cf *Rf, int64_t count, hipStream_t st);
+void add_correction(cd *X, const cf *D, int64_t count, hipStream_t st);
+
+// ---- gsmk.hip
+struct BfftJob {                // one Fourier transform: wavevector and phases
+    double k[2];
+    cd ff[5];
+};
+void launch_bfft(const SheetDev &sh, const int32_t *bf_face, const int32_t *bf_loc,
+                 const BfftJob *d_jobs, int njobs, cd *ft, hipStream_t st);
+void launch_rhs(int64_t nbf, int ncol, const cd *ft, const int32_t *d_col_job,
+                const cd *d_col_src, cd *rhs, int64_t ld, hipStream_t st);
+void launch_project(int64_t nbf, int ncol, int njobs, const cd *X, int64_t ldx, const cd *ft,
+                    cd *out, hipStream_t st);
+void cascade_batched_dev(int nb, int n1, int n2, int n3, const cd *a, const cd *b, cd *c,
+                         hipStream_t st);

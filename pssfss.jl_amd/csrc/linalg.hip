@@ -1,0 +1,475 @@
+// Dense complex linear algebra for gfx950: batched GEMM, blocked LU with partial
+// pivoting, triangular solves and the FP64 refinement helpers.
+//
+// Matrices of the MoM path are stored ROW-major on the device (the fill writes
+// Z[m][s] at m*n + s): row interchanges, panel rows and the U12 block rows are
+// then contiguous.  The factorisation is LAPACK's right-looking zgetrf with the
+// pivot of largest |re|+|im| (izamax), applied to whole rows, so P*A = L*U with
+// the usual packed L\U storage -- the lu! / ldiv! pair of src/PSSFSS.jl:401,439.
+#include "internal.hpp"
+
+namespace {
+
+constexpr int NB = 64;  // panel width
+
+// ------------------------------------------------------------------ GEMM (NN)
+// C = alpha*A*B + beta*C, column-major, batched by blockIdx.z.
+template <class C>
+__global__ __launch_bounds__(256) void k_gemm_nn(int m, int n, int k, C alpha,
+                                                 const C *__restrict__ A, int64_t lda,
+                                                 int64_t sA, const C *__restrict__ B,
+                                                 int64_t ldb, int64_t sB, C beta, C *Cm,
+                                                 int64_t ldc, int64_t sC) {
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ C As[BK][BM];
+    __shared__ C Bs[BK][BN];
+    const int64_t bz = blockIdx.z;
+    A += bz * sA;
+    B += bz * sB;
+    Cm += bz * sC;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+    C acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = CTraits<C>::zero();
+    for (int k0 = 0; k0 < k; k0 += BK) {
+#pragma unroll
+        for (int t = tid; t < BM * BK; t += 256) {
+            int i = t % BM, kk = t / BM;
+            As[kk][i] = (m0 + i < m && k0 + kk < k) ? A[(m0 + i) + lda * (k0 + kk)]
+                                                    : CTraits<C>::zero();
+        }
+#pragma unroll
+        for (int t = tid; t < BK * BN; t += 256) {
+            int kk = t % BK, j = t / BK;
+            Bs[kk][j] = (k0 + kk < k && n0 + j < n) ? B[(k0 + kk) + ldb * (n0 + j)]
+                                                    : CTraits<C>::zero();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; ++kk) {
+            C a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cfma(acc[i][j], a[i], b[j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int64_t r = m0 + tx + 16 * i;
+        if (r >= m) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int64_t c = n0 + ty + 16 * j;
+            if (c >= n) continue;
+            C *p = Cm + r + ldc * c;
+            C v = alpha * acc[i][j];
+            if (!(beta.x == 0 && beta.y == 0)) v += beta * (*p);
+            *p = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------- LU panel
+// Factor columns [k, k+kb) of rows [k, n) of a row-major matrix with partial
+// pivoting; swaps are applied inside the panel only (k_rowswap does the rest).
+template <class C>
+__global__ __launch_bounds__(512) void k_panel(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
+                                               int32_t *ipiv0, int64_t sP) {
+    typedef typename CTraits<C>::R R;
+    C *A = A0 + (int64_t)blockIdx.x * sA;
+    int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
+    const int64_t lda = n;
+    __shared__ C prow[NB];
+    __shared__ R rv[8];
+    __shared__ int ri[8];
+    __shared__ int pivs;
+    const int tid = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+    for (int j = 0; j < kb; ++j) {
+        const int64_t col = k + j;
+        R best = (R)-1;
+        int bi = 0x7fffffff;
+        for (int64_t i = col + tid; i < n; i += nt) {
+            R v = cabs1(A[i * lda + col]);
+            if (v > best) { best = v; bi = (int)i; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            R ov = __shfl_xor(best, off, 64);
+            int oi = __shfl_xor(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            R b = rv[0];
+            int bx = ri[0];
+            for (int w = 1; w < nw; ++w)
+                if (rv[w] > b || (rv[w] == b && ri[w] < bx)) { b = rv[w]; bx = ri[w]; }
+            pivs = bx;
+            ipiv[col] = bx;
+        }
+        __syncthreads();
+        const int64_t p = pivs;
+        if (p != col) {
+            for (int c = tid; c < kb; c += nt) {
+                C t = A[col * lda + k + c];
+                A[col * lda + k + c] = A[p * lda + k + c];
+                A[p * lda + k + c] = t;
+            }
+        }
+        __syncthreads();
+        for (int c = tid; c < kb; c += nt) prow[c] = A[col * lda + k + c];
+        __syncthreads();
+        const C piv = prow[j];
+        const bool nz = !(piv.x == 0 && piv.y == 0);
+        const C rp = nz ? CTraits<C>::one() / piv : CTraits<C>::zero();
+        for (int64_t i = col + 1 + tid; i < n; i += nt) {
+            C *row = A + i * lda + k;
+            C l = nz ? row[j] * rp : row[j];
+            row[j] = l;
+            C ml = -l;
+            for (int c = j + 1; c < kb; ++c) cfma(row[c], ml, prow[c]);
+        }
+        __syncthreads();
+    }
+}
+
+// Apply the panel's interchanges to the rest of each row (columns outside the
+// panel), in pivot order.
+template <class C>
+__global__ void k_rowswap(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
+                          const int32_t *ipiv0, int64_t sP) {
+    C *A = A0 + (int64_t)blockIdx.y * sA;
+    const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.y * sP;
+    const int64_t c0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t other = n - kb;
+    if (c0 >= other) return;
+    const int64_t c = c0 < k ? c0 : c0 + kb;
+    for (int j = 0; j < kb; ++j) {
+        const int64_t r = k + j, p = ipiv[r];
+        if (p != r) {
+            C t = A[r * n + c];
+            A[r * n + c] = A[p * n + c];
+            A[p * n + c] = t;
+        }
+    }
+}
+
+// U12 = L11^{-1} A12 (unit lower L11), one thread per trailing column.
+template <class C>
+__global__ __launch_bounds__(64) void k_trsm_u12(int64_t n, int64_t k, int kb, C *A0,
+                                                 int64_t sA) {
+    C *A = A0 + (int64_t)blockIdx.y * sA;
+    __shared__ C L[NB][NB];
+    for (int t = threadIdx.x; t < kb * kb; t += blockDim.x) {
+        int r = t / kb, l = t % kb;
+        L[r][l] = A[(k + r) * n + k + l];
+    }
+    __syncthreads();
+    const int64_t c = k + kb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    C x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        if (r < kb) {
+            C v = A[(k + r) * n + c];
+#pragma unroll
+            for (int l = 0; l < r; ++l) cfma(v, -L[r][l], x[l]);
+            x[r] = v;
+            A[(k + r) * n + c] = v;
+        }
+    }
+}
+
+// Forward (unit lower) and backward (upper) substitution for nrhs columns of a
+// column-major B, one workgroup per system, after the row interchanges.
+template <class C, int NR>
+__global__ __launch_bounds__(256) void k_trsv(int64_t n, int nrhs, const C *LU0, int64_t sA,
+                                              const int32_t *ipiv0, int64_t sP, C *B0,
+                                              int64_t sB) {
+    const C *LU = LU0 + (int64_t)blockIdx.x * sA;
+    const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
+    C *B = B0 + (int64_t)blockIdx.x * sB;
+    __shared__ C D[NB][NB + 1];
+    __shared__ C Y[NR][NB];
+    const int tid = threadIdx.x;
+    // B <- P B
+    if (tid < nrhs) {
+        for (int64_t j = 0; j < n; ++j) {
+            int64_t p = ipiv[j];
+            if (p != j) {
+                C t = B[j + n * tid];
+                B[j + n * tid] = B[p + n * tid];
+                B[p + n * tid] = t;
+            }
+        }
+    }
+    __syncthreads();
+    // L y = b
+    for (int64_t b0 = 0; b0 < n; b0 += NB) {
+        const int bs = (int)min((int64_t)NB, n - b0);
+        for (int t = tid; t < bs * bs; t += blockDim.x) {
+            int r = t / bs, l = t % bs;
+            D[r][l] = LU[(b0 + r) * n + b0 + l];
+        }
+        for (int t = tid; t < NR * bs; t += blockDim.x) {
+            int c = t / bs, r = t % bs;
+            Y[c][r] = c < nrhs ? B[b0 + r + n * c] : CTraits<C>::zero();
+        }
+        __syncthreads();
+        if (tid < nrhs) {
+            for (int r = 0; r < bs; ++r) {
+                C v = Y[tid][r];
+                for (int l = 0; l < r; ++l) cfma(v, -D[r][l], Y[tid][l]);
+                Y[tid][r] = v;
+                B[b0 + r + n * tid] = v;
+            }
+        }
+        __syncthreads();
+        for (int64_t i = b0 + bs + tid; i < n; i += blockDim.x) {
+            C acc[NR];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[c] = CTraits<C>::zero();
+            const C *row = LU + i * n + b0;
+            for (int l = 0; l < bs; ++l) {
+                C a = row[l];
+#pragma unroll
+                for (int c = 0; c < NR; ++c) cfma(acc[c], a, Y[c][l]);
+            }
+#pragma unroll
+            for (int c = 0; c < NR; ++c)
+                if (c < nrhs) B[i + n * c] -= acc[c];
+        }
+        __syncthreads();
+    }
+    // U x = y
+    const int64_t nblk = (n + NB - 1) / NB;
+    for (int64_t bb = nblk - 1; bb >= 0; --bb) {
+        const int64_t b0 = bb * NB;
+        const int bs = (int)min((int64_t)NB, n - b0);
+        for (int t = tid; t < bs * bs; t += blockDim.x) {
+            int r = t / bs, l = t % bs;
+            D[r][l] = LU[(b0 + r) * n + b0 + l];
+        }
+        for (int t = tid; t < NR * bs; t += blockDim.x) {
+            int c = t / bs, r = t % bs;
+            Y[c][r] = c < nrhs ? B[b0 + r + n * c] : CTraits<C>::zero();
+        }
+        __syncthreads();
+        if (tid < nrhs) {
+            for (int r = bs - 1; r >= 0; --r) {
+                C v = Y[tid][r];
+                for (int l = r + 1; l < bs; ++l) cfma(v, -D[r][l], Y[tid][l]);
+                v = v / D[r][r];
+                Y[tid][r] = v;
+                B[b0 + r + n * tid] = v;
+            }
+        }
+        __syncthreads();
+        for (int64_t i = tid; i < b0; i += blockDim.x) {
+            C acc[NR];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[c] = CTraits<C>::zero();
+            const C *row = LU + i * n + b0;
+            for (int l = 0; l < bs; ++l) {
+                C a = row[l];
+#pragma unroll
+                for (int c = 0; c < NR; ++c) cfma(acc[c], a, Y[c][l]);
+            }
+#pragma unroll
+            for (int c = 0; c < NR; ++c)
+                if (c < nrhs) B[i + n * c] -= acc[c];
+        }
+        __syncthreads();
+    }
+}
+
+// R = B - A X for row-major A, column-major X/B/R (n x nrhs), one wavefront per row;
+// also max |R| and max |X| per (system, rhs), as monotone bit patterns.
+template <int NR>
+__global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd *A0, int64_t sA,
+                                                  const cd *X0, const cd *B0, int64_t sB,
+                                                  cd *R0, unsigned long long *rmax,
+                                                  unsigned long long *xmax) {
+    const int64_t b = blockIdx.y;
+    const cd *A = A0 + b * sA;
+    const cd *X = X0 + b * sB;
+    const cd *Bv = B0 + b * sB;
+    cd *R = R0 + b * sB;
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    cd acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = cd{0.0, 0.0};
+    const cd *row = A + i * n;
+    for (int64_t j = lane; j < n; j += 64) {
+        cd a = row[j];
+#pragma unroll
+        for (int c = 0; c < NR; ++c)
+            if (c < nrhs) cfma(acc[c], a, X[j + n * c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+        for (int off = 32; off > 0; off >>= 1) {
+            acc[c].x += __shfl_xor(acc[c].x, off, 64);
+            acc[c].y += __shfl_xor(acc[c].y, off, 64);
+        }
+    }
+    if (lane < nrhs && lane < NR) {
+        cd a = cd{0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < NR; ++c)
+            if (c == lane) a = acc[c];
+        cd r = Bv[i + n * lane] - a;
+        R[i + n * lane] = r;
+        double rm = fmax(fabs(r.x), fabs(r.y));
+        cd xv = X[i + n * lane];
+        double xm = fmax(fabs(xv.x), fabs(xv.y));
+        atomicMax(rmax + b * nrhs + lane, (unsigned long long)__double_as_longlong(rm));
+        atomicMax(xmax + b * nrhs + lane, (unsigned long long)__double_as_longlong(xm));
+    }
+}
+
+__global__ void k_cd_to_cf(const cd *a, cf *b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) b[i] = cf{(float)a[i].x, (float)a[i].y};
+}
+
+__global__ void k_add_corr(cd *x, const cf *d, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) x[i] = x[i] + cd{(double)d[i].x, (double)d[i].y};
+}
+
+unsigned grid1(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+template <class C>
+void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, const C *B,
+            int64_t ldb, int64_t sB, C beta, C *Cm, int64_t ldc, int64_t sC, int batch,
+            hipStream_t st) {
+    if (m <= 0 || n <= 0 || batch <= 0) return;
+    dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
+    hipLaunchKernelGGL(k_gemm_nn<C>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA, B, ldb,
+                       sB, beta, Cm, ldc, sC);
+    HIPCHECK(hipGetLastError());
+}
+
+template <class C>
+void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                 hipStream_t st) {
+    C mone, one;
+    mone.x = -1; mone.y = 0;
+    one.x = 1; one.y = 0;
+    for (int64_t k = 0; k < n; k += NB) {
+        const int kb = (int)std::min<int64_t>(NB, n - k);
+        hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(512), 0, st, n, k, kb, A, sA,
+                           ipiv, sP);
+        HIPCHECK(hipGetLastError());
+        const int64_t other = n - kb;
+        if (other > 0) {
+            dim3 g((unsigned)((other + 255) / 256), (unsigned)batch);
+            hipLaunchKernelGGL(k_rowswap<C>, g, dim3(256), 0, st, n, k, kb, A, sA, ipiv, sP);
+            HIPCHECK(hipGetLastError());
+        }
+        const int64_t rest = n - k - kb;
+        if (rest > 0) {
+            dim3 g((unsigned)((rest + 63) / 64), (unsigned)batch);
+            hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k, kb, A, sA);
+            HIPCHECK(hipGetLastError());
+            // Row-major A22 -= L21 U12, as a column-major GEMM on the transposes:
+            // A22^T -= U12^T L21^T, every view having leading dimension n.
+            C *u12 = A + k * n + (k + kb);
+            C *l21t = A + (k + kb) * n + k;
+            C *a22 = A + (k + kb) * n + (k + kb);
+            gemm_t<C>((int)rest, (int)rest, kb, mone, u12, n, sA, l21t, n, sA, one, a22, n, sA,
+                      batch, st);
+        }
+    }
+}
+
+template <class C>
+void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *ipiv,
+                int64_t sP, C *B, int64_t sB, hipStream_t st) {
+    for (int c0 = 0; c0 < nrhs; c0 += 8) {
+        int nr = std::min(8, nrhs - c0);
+        C *Bc = B + (int64_t)c0 * n;
+        if (nr <= 4)
+            hipLaunchKernelGGL((k_trsv<C, 4>), dim3((unsigned)batch), dim3(256), 0, st, n, nr, LU,
+                               sA, ipiv, sP, Bc, sB);
+        else
+            hipLaunchKernelGGL((k_trsv<C, 8>), dim3((unsigned)batch), dim3(256), 0, st, n, nr, LU,
+                               sA, ipiv, sP, Bc, sB);
+        HIPCHECK(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+void zgemm_batched(int m, int n, int k, cd alpha, const cd *A, int64_t lda, int64_t sA,
+                   const cd *B, int64_t ldb, int64_t sB, cd beta, cd *C, int64_t ldc,
+                   int64_t sC, int batch, hipStream_t st) {
+    gemm_t<cd>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st);
+}
+
+void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                       hipStream_t st) {
+    lu_factor_t<cd>(n, batch, A, sA, ipiv, sP, st);
+}
+
+void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
+                      const int32_t *ipiv, int64_t sP, cd *B, int64_t sB, hipStream_t st) {
+    lu_solve_t<cd>(n, nrhs, batch, LU, sA, ipiv, sP, B, sB, st);
+}
+
+void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                           hipStream_t st) {
+    lu_factor_t<cf>(n, batch, A, sA, ipiv, sP, st);
+}
+
+void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
+                          const int32_t *ipiv, int64_t sP, cf *B, int64_t sB, hipStream_t st) {
+    lu_solve_t<cf>(n, nrhs, batch, LU, sA, ipiv, sP, B, sB, st);
+}
+
+void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {
+    hipLaunchKernelGGL(k_cd_to_cf, dim3(grid1(count)), dim3(256), 0, st, Z, Zf, count);
+    HIPCHECK(hipGetLastError());
+}
+
+void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
+                      const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
+                      hipStream_t st) {
+    if (nrhs > 8) throw ArgError("residual_batched: at most 8 right-hand sides per call");
+    launch_zero((cd *)rnorm, (batch * nrhs + 1) / 2, st);
+    launch_zero((cd *)xnorm, (batch * nrhs + 1) / 2, st);
+    dim3 g((unsigned)((n + 3) / 4), (unsigned)batch);
+    if (nrhs <= 4)
+        hipLaunchKernelGGL(k_residual<4>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
+                           (unsigned long long *)rnorm, (unsigned long long *)xnorm);
+    else
+        hipLaunchKernelGGL(k_residual<8>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
+                           (unsigned long long *)rnorm, (unsigned long long *)xnorm);
+    HIPCHECK(hipGetLastError());
+}
+
+void residual_to_f32(const cd *R, cf *Rf, int64_t count, hipStream_t st) {
+    convert_z_to_f32(R, Rf, count, st);
+}
+
+void add_correction(cd *X, const cf *D, int64_t count, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_corr, dim3(grid1(count)), dim3(256), 0, st, X, D, count);
+    HIPCHECK(hipGetLastError());
+}

@@ -1,0 +1,562 @@
+// Periodic Green's function kernels for gfx950:
+//   * spectral tables: Kummer-subtracted modal summands (ring-major storage),
+//     per-ring convergence sums, square gather, DFT (as GEMMs, linalg.hip),
+//     Floquet phase + ghost rows          -- src/PGF.jl:196-365, 478-629
+//   * spatial lattice sums integrated over the source triangle for several
+//     phasings at once (filljk! / jksums) -- src/Zint.jl:54-125, src/PGF.jl:67-106
+//   * analytic singular integrals          -- src/Zint.jl:126-165
+#include "internal.hpp"
+
+namespace {
+
+constexpr double TWOPI = 6.283185307179586;
+constexpr double PI_ = 3.141592653589793;
+
+__constant__ double QXI[7] = {0.33333333333333330, 0.10128650732345633, 0.79742698535308730,
+                              0.10128650732345633, 0.47014206410511505, 0.05971587178976989,
+                              0.47014206410511505};
+__constant__ double QETA[7] = {0.33333333333333333, 0.79742698535308730, 0.10128650732345633,
+                               0.10128650732345633, 0.05971587178976989, 0.47014206410511505,
+                               0.47014206410511505};
+__constant__ double QW[7] = {0.1125, 0.06296959027241358, 0.06296959027241358,
+                             0.06296959027241358, 0.06619707639425308, 0.06619707639425308,
+                             0.06619707639425308};
+
+// Ring(r) element k: m=r column, m=-r column (n ascending), then n=-r and n=r rows
+// (m ascending, corners excluded) -- the order of src/Rings.jl:9-63.
+__device__ __forceinline__ void ring_mn(int r, int k, int &m, int &n) {
+    if (r == 0) { m = 0; n = 0; return; }
+    int side = 2 * r + 1;
+    if (k < side) { m = r; n = -r + k; return; }
+    k -= side;
+    if (k < side) { m = -r; n = -r + k; return; }
+    k -= side;
+    if (k < 2 * r - 1) { m = -r + 1 + k; n = -r; return; }
+    k -= 2 * r - 1;
+    m = -r + 1 + k;
+    n = r;
+}
+
+__device__ __forceinline__ int64_t ring_base(int r) {
+    return r == 0 ? 0 : (int64_t)(2 * r - 1) * (2 * r - 1);
+}
+
+__device__ __forceinline__ int ring_of(int64_t e) {
+    if (e == 0) return 0;
+    int r = (int)((sqrt((double)e) + 1.0) * 0.5);
+    while (ring_base(r + 1) <= e) ++r;
+    while (r > 0 && ring_base(r) > e) --r;
+    return r;
+}
+
+__device__ __forceinline__ int64_t ring_index(int m, int n) {
+    int r = max(abs(m), abs(n));
+    if (r == 0) return 0;
+    int64_t b = ring_base(r);
+    int side = 2 * r + 1;
+    if (m == r) return b + (n + r);
+    if (m == -r) return b + side + (n + r);
+    if (n == -r) return b + 2 * side + (m + r - 1);
+    return b + 2 * side + (2 * r - 1) + (m + r - 1);
+}
+
+// Summands of eqs. (5.19a,b) (electric) or (5.26a,b) (magnetic) for lattice index
+// (m, n): transmission-line impedances stepped through the layers from both ends.
+__device__ void summand(const TabParams &c, int m, int n, cd &t1, cd &t2) {
+#pragma clang fp contract(off)
+    double bx = (c.b00[0] + (double)m * c.b1[0]) + (double)n * c.b2[0];
+    double by = (c.b00[1] + (double)m * c.b1[1]) + (double)n * c.b2[1];
+    double b2 = bx * bx + by * by;
+    if (!(b2 >= c.b2min)) b2 = c.b2min;
+    double kap2 = b2 + c.u * c.u;
+    double kap = sqrt(kap2);
+    const int nl = c.nl, s = c.s;
+    cd g = mysqrt(b2 - (c.k0sq * c.eps[0]) * c.mu[0]);
+    if (!c.magnetic) {
+        cd zlte = c.mu[0] / g, zltm = g / c.eps[0];
+        for (int i = 1; i < s; ++i) {
+            g = mysqrt(b2 - (c.k0sq * c.eps[i]) * c.mu[i]);
+            cd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
+            cd th = ctanh_(c.width[i] * g);
+            zlte = (z0te * (zlte + z0te * th)) / (z0te + zlte * th);
+            zltm = (z0tm * (zltm + z0tm * th)) / (z0tm + zltm * th);
+        }
+        g = mysqrt(b2 - (c.k0sq * c.eps[nl - 1]) * c.mu[nl - 1]);
+        cd zrte = c.mu[nl - 1] / g, zrtm = g / c.eps[nl - 1];
+        for (int i = nl - 2; i >= s; --i) {
+            g = mysqrt(b2 - (c.k0sq * c.eps[i]) * c.mu[i]);
+            cd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
+            cd th = ctanh_(c.width[i] * g);
+            zrte = (z0te * (zrte + z0te * th)) / (z0te + zrte * th);
+            zrtm = (z0tm * (zrtm + z0tm * th)) / (z0tm + zrtm * th);
+        }
+        cd vte = (zlte * zrte) / (zlte + zrte);
+        cd vtm = (zltm * zrtm) / (zltm + zrtm);
+        vte = (2.0 * vte) / c.mut;
+        t1 = vte - (1.0 + c.c3 / kap2) / kap;
+        vtm = (2.0 * c.epsb) * vtm;
+        t2 = (vtm + ((c.k0sq * c.epsb) * c.mut) * vte) / b2 - (1.0 + c.d3 / kap2) / kap;
+    } else {
+        cd ylte = g / c.mu[0], yltm = c.eps[0] / g;
+        for (int i = 1; i < s; ++i) {
+            g = mysqrt(b2 - (c.k0sq * c.eps[i]) * c.mu[i]);
+            cd y0te = g / c.mu[i], y0tm = c.eps[i] / g;
+            cd th = ctanh_(c.width[i] * g);
+            ylte = (y0te * (ylte + y0te * th)) / (y0te + ylte * th);
+            yltm = (y0tm * (yltm + y0tm * th)) / (y0tm + yltm * th);
+        }
+        g = mysqrt(b2 - (c.k0sq * c.eps[nl - 1]) * c.mu[nl - 1]);
+        cd yrte = g / c.mu[nl - 1], yrtm = c.eps[nl - 1] / g;
+        for (int i = nl - 2; i >= s; --i) {
+            g = mysqrt(b2 - (c.k0sq * c.eps[i]) * c.mu[i]);
+            cd y0te = g / c.mu[i], y0tm = c.eps[i] / g;
+            cd th = ctanh_(c.width[i] * g);
+            yrte = (y0te * (yrte + y0te * th)) / (y0te + yrte * th);
+            yrtm = (y0tm * (yrtm + y0tm * th)) / (y0tm + yrtm * th);
+        }
+        t1 = (yltm + yrtm) - (c.f1 + c.f3 / kap2) / kap;
+        t2 = ((((yltm + yrtm) * c.k0sq) + ylte) + yrte) / b2 - (c.p1 + c.p3 / kap2) / kap;
+    }
+}
+
+__global__ void k_summands(const TabParams *tp, int64_t cap, int64_t e0, int64_t e1, cd *t1,
+                           cd *t2) {
+    int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const int p = blockIdx.y;
+    int r = ring_of(e);
+    int m, n;
+    ring_mn(r, (int)(e - ring_base(r)), m, n);
+    cd a, b;
+    summand(tp[p], m, n, a, b);
+    t1[p * cap + e] = a;
+    t2[p * cap + e] = b;
+}
+
+// Ring sums in ring order (src/PGF.jl:237-288) and the convergence flag per ring.
+__global__ void k_ringsums(int64_t cap, int r0, int r1, int rcap, const cd *t1, const cd *t2,
+                           double convtest, int *flags) {
+    int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > r1) return;
+    const int p = blockIdx.y;
+    const cd *a = t1 + p * cap, *b = t2 + p * cap;
+    int64_t base = ring_base(r);
+    int len = r == 0 ? 1 : 8 * r;
+    cd s1 = {0.0, 0.0}, s2 = {0.0, 0.0};
+    for (int k = 0; k < len; ++k) {
+        s1 += a[base + k];
+        s2 += b[base + k];
+    }
+    int f = 0;
+    if (r > 0) {
+        double q1 = cabs(s1 / a[0]);
+        double q2 = cabs(s2 / b[0]);
+        f = (q1 < convtest && q2 < convtest) ? 1 : 0;
+    }
+    flags[(int64_t)p * rcap + r] = f;
+}
+
+// S_p = [table1 | table2] as an M x 2M column-major array, entry (i, j) from
+// lattice index (i - M/2, j - M/2) (src/PGF.jl:311-312).
+__global__ void k_gather_square(const int *Ms, int64_t cap, const cd *t1, const cd *t2, cd *S,
+                                int64_t s_stride) {
+    const int p = blockIdx.z;
+    const int M = Ms[p];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= M || j >= M) return;
+    int64_t idx = ring_index(i - M / 2, j - M / 2);
+    cd *Sp = S + p * s_stride;
+    Sp[i + (int64_t)M * j] = t1[p * cap + idx];
+    Sp[i + (int64_t)M * (j + M)] = t2[p * cap + idx];
+}
+
+// Unnormalised forward DFT matrix W[j,k] = exp(-2 pi i jk / M), exact integer phase.
+__global__ void k_dft_matrix(int M, cd *W) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.y * blockDim.y + threadIdx.y;
+    if (j >= M || k >= M) return;
+    int64_t e = ((int64_t)j * k) % M;
+    double ang = -TWOPI * (double)e / (double)M;
+    W[j + (int64_t)M * k] = cd{cos(ang), sin(ang)};
+}
+
+// Phase factor of eq. (5.32), 1/(2A) or 1/A scaling, and the ghost rows/columns
+// at -1 and M (src/PGF.jl:316-357, 593-621).  F holds the two transformed
+// tables side by side (M x 2M); tab receives two (M+2)^2 tables.
+struct PhaseCtx {
+    const cd *F;
+    int M, magnetic;
+    double psi1, psi2, scale;
+    cd c1, c2, c12, g1, g2;
+};
+
+__device__ __forceinline__ cd tab_interior(const PhaseCtx &c, int which, int i, int j) {
+#pragma clang fp contract(off)
+    double pterm = i * (PI_ - c.psi1 / c.M);
+    double qterm = j * (PI_ - c.psi2 / c.M);
+    cd cf = cisd(pterm + qterm) / c.scale;
+    cd v = c.F[i + (int64_t)c.M * (j + which * c.M)];
+    return v * cf;
+}
+
+__device__ cd tab_value(const PhaseCtx &c, int which, int i, int j) {
+    const int M = c.M;
+    // Upper ghosts first (they only reference interior values).
+    auto up = [&](int ii, int jj) -> cd {
+        if (ii < M && jj < M) return tab_interior(c, which, ii, jj);
+        if (ii == M && jj == M) return c.c12 * tab_interior(c, which, 0, 0);
+        if (ii == M) return c.c1 * tab_interior(c, which, 0, jj);
+        return c.c2 * tab_interior(c, which, ii, 0);
+    };
+    if (i >= 0 && j >= 0) return up(i, j);
+    if (i == -1 && j >= 0) return c.g1 * up(M - 1, j);            // row -1, cols 0..M
+    // column -1, rows -1..M-1
+    cd src = (i == -1) ? c.g1 * up(M - 1, M - 1) : up(i, M - 1);
+    return c.g2 * src;
+}
+
+__global__ void k_phase_ghost(const int *Ms, const double *psis, const int *mags,
+                              const double *areas, const cd *F, int64_t f_stride, cd *tab,
+                              int64_t tab_stride) {
+    const int p = blockIdx.z;
+    PhaseCtx c;
+    c.M = Ms[p];
+    c.magnetic = mags[p];
+    c.psi1 = psis[2 * p];
+    c.psi2 = psis[2 * p + 1];
+    c.scale = c.magnetic ? areas[p] : 2.0 * areas[p];
+    c.F = F + p * f_stride;
+    const int E = c.M + 2;
+    int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    if (i > c.M || j > c.M) return;
+    c.c1 = cisd(-c.psi1);
+    c.c2 = cisd(-c.psi2);
+    if (!c.magnetic) {
+        c.c12 = c.c1 * c.c2;
+        c.g1 = 1.0 / c.c1;
+        c.g2 = 1.0 / c.c2;
+    } else {
+        c.c12 = cisd(-(c.psi1 + c.psi2));
+        c.g1 = cisd(c.psi1);
+        c.g2 = cisd(c.psi2);
+    }
+    cd *out = tab + p * tab_stride;
+    if (i == c.M && j == -1) {   // never referenced by the interpolation
+        out[(i + 1) + (int64_t)E * (j + 1)] = cd{0.0, 0.0};
+        out[(int64_t)E * E + (i + 1) + (int64_t)E * (j + 1)] = cd{0.0, 0.0};
+        return;
+    }
+    out[(i + 1) + (int64_t)E * (j + 1)] = tab_value(c, 0, i, j);
+    out[(int64_t)E * E + (i + 1) + (int64_t)E * (j + 1)] = tab_value(c, 1, i, j);
+}
+
+// ---------------------------------------------------------------- spatial sums
+// One lane per (face-pair class, quadrature point); 9 classes x 7 points per
+// wavefront.  Every lane sums the lattice rings for F phasings at once: the
+// frequency-independent distance, exp and 1/distance are shared by the F
+// phasings, whose lattice phases for |m|,|n| <= JK_RT come from LDS.
+template <int F, bool MIXED>
+__global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul, double us1x,
+                                            double us1y, double us2x, double us2y,
+                                            const double *psis, int nf, cd *jk,
+                                            int64_t jk_stride, int *ringmax) {
+#pragma clang fp contract(off)
+    constexpr int RT = JK_RT, W = 2 * JK_RT + 1;
+    __shared__ cd ph[F][W * W];
+    for (int t = threadIdx.x; t < F * W * W; t += blockDim.x) {
+        int f = t / (W * W), e = t % (W * W);
+        int m = e / W - RT, n = e % W - RT;
+        cd v = {0.0, 0.0};
+        if (f < nf) {
+            double a = (double)m * psis[2 * f] + (double)n * psis[2 * f + 1];
+            v = cd{cos(-a), sin(-a)};
+        }
+        ph[f][e] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int qp = lane % 7;
+    const int64_t iu = ((int64_t)blockIdx.x * 4 + wave) * 9 + lane / 7;
+    const bool active = lane < 63 && iu < nufp;
+    double urx = 0.0, ury = 0.0;
+    if (active) {
+        int64_t fp = sh.identity ? iu : sh.umem[sh.uoff[iu]];
+        int64_t ifm = fp % sh.nface, ifs = fp / sh.nface;
+        const double *rs = sh.fxs + 6 * ifs;
+        const double *rm = sh.fxs + 6 * ifm;
+        double rs12x = rs[0] - rs[2], rs12y = rs[1] - rs[3];
+        double rs31x = rs[4] - rs[0], rs31y = rs[5] - rs[1];
+        double rs21x = -rs12x, rs21y = -rs12y;
+        double rmcx = ((rm[0] + rm[2]) + rm[4]) / 3.0;
+        double rmcy = ((rm[1] + rm[3]) + rm[5]) / 3.0;
+        double rtx = (rs[0] + rs21x * QXI[qp]) + rs31x * QETA[qp];
+        double rty = (rs[1] + rs21y * QXI[qp]) + rs31y * QETA[qp];
+        urx = ul * (rmcx - rtx);
+        ury = ul * (rmcy - rty);
+    }
+    cd js[F], ks[F];
+    unsigned done = 0;
+    int rsave = 0;
+    if (active) {
+        double arg = sqrt(urx * urx + ury * ury);
+        double rterm = exp(-arg);
+        double j0;
+        if (arg <= 1e-5) j0 = (-arg / 6 + 0.5) * arg - 1;
+        else j0 = (rterm - 1) / arg;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            js[f] = cd{j0, 0.0};
+            ks[f] = cd{rterm, 0.0};
+        }
+        for (int f = nf; f < F; ++f) done |= 1u << f;
+        const unsigned all = (1u << F) - 1u;
+        for (int r = 1; r <= JK_RINGMAX && done != all; ++r) {
+            rsave = r;
+            cd jr[F], kr[F];
+            cf jrf[F], krf[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                jr[f] = cd{0.0, 0.0};
+                kr[f] = cd{0.0, 0.0};
+                jrf[f] = cf{0.0f, 0.0f};
+                krf[f] = cf{0.0f, 0.0f};
+            }
+            const int len = 8 * r;
+            for (int k = 0; k < len; ++k) {
+                int m, n;
+                ring_mn(r, k, m, n);
+                double dx = urx - ((double)m * us1x + (double)n * us2x);
+                double dy = ury - ((double)m * us1y + (double)n * us2y);
+                if (MIXED) {
+                    float fx = (float)dx, fy = (float)dy;
+                    float a = __fsqrt_rn(fx * fx + fy * fy);
+                    float e = __expf(-a);
+                    float ea = __fdividef(e, a);
+                    if (r <= RT) {
+                        const int off = (m + RT) * W + (n + RT);
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            cd p = ph[f][off];
+                            cf pf = cf{(float)p.x, (float)p.y};
+                            jrf[f] = jrf[f] + ea * pf;
+                            krf[f] = krf[f] + e * pf;
+                        }
+                    } else {
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            double a2 = (double)m * psis[2 * f] + (double)n * psis[2 * f + 1];
+                            float sn, cs;
+                            __sincosf((float)(-a2), &sn, &cs);
+                            cf pf = cf{cs, sn};
+                            jrf[f] = jrf[f] + ea * pf;
+                            krf[f] = krf[f] + e * pf;
+                        }
+                    }
+                } else {
+                    double a = sqrt(dx * dx + dy * dy);
+                    double e = exp(-a);
+                    double ia = 1.0 / a;
+                    if (r <= RT) {
+                        const int off = (m + RT) * W + (n + RT);
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            cd p = ph[f][off];
+                            cd term = cd{e * p.x, e * p.y};
+                            jr[f] += cd{term.x * ia, term.y * ia};
+                            kr[f] += term;
+                        }
+                    } else {
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            double a2 = (double)m * psis[2 * f] + (double)n * psis[2 * f + 1];
+                            cd term = cd{e * cos(-a2), e * sin(-a2)};
+                            jr[f] += cd{term.x * ia, term.y * ia};
+                            kr[f] += term;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                if (done & (1u << f)) continue;
+                cd a = MIXED ? cd{(double)jrf[f].x, (double)jrf[f].y} : jr[f];
+                cd b = MIXED ? cd{(double)krf[f].x, (double)krf[f].y} : kr[f];
+                js[f] += a;
+                ks[f] += b;
+                if (r > 5) {
+                    double c1 = cabs(a / js[f]), c2 = cabs(b / ks[f]);
+                    double conv = c1 > c2 ? c1 : c2;
+                    if (conv < 1e-8) done |= 1u << f;
+                }
+            }
+        }
+    }
+    // Combine the 7 quadrature points of each class in order (src/Zint.jl:117-124).
+    const int base_lane = (lane / 7) * 7;
+    for (int f = 0; f < F; ++f) {
+        if (f >= nf) break;
+        double jx[7], jy[7], kx[7], ky[7];
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            int src = base_lane + q;
+            if (src > 63) src = 63;
+            jx[q] = __shfl(js[f].x, src, 64);
+            jy[q] = __shfl(js[f].y, src, 64);
+            kx[q] = __shfl(ks[f].x, src, 64);
+            ky[q] = __shfl(ks[f].y, src, 64);
+        }
+        if (active && qp == 0) {
+            cd J = {0, 0}, Jx = {0, 0}, Je = {0, 0}, K = {0, 0}, Kx = {0, 0}, Ke = {0, 0};
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                cd jw = cd{jx[q], jy[q]} * QW[q];
+                Jx += jw * QXI[q];
+                Je += jw * QETA[q];
+                J += jw;
+                cd kw = cd{kx[q], ky[q]} * QW[q];
+                Kx += kw * QXI[q];
+                Ke += kw * QETA[q];
+                K += kw;
+            }
+            cd *o = jk + f * jk_stride;
+            o[0 * nufp + iu] = J;
+            o[1 * nufp + iu] = Jx;
+            o[2 * nufp + iu] = Je;
+            o[3 * nufp + iu] = K;
+            o[4 * nufp + iu] = Kx;
+            o[5 * nufp + iu] = Ke;
+        }
+    }
+    if (active && ringmax) atomicMax(ringmax, rsave);
+}
+
+// Closed-form singular integrals of eqs. (B.3) over the source triangle,
+// src/Zint.jl:126-165 (closed = true: every class).
+__global__ void k_singular(SheetDev sh, int64_t nufp, double ul, double *rho_r, double *rinv) {
+#pragma clang fp contract(off)
+    int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iu >= nufp) return;
+    int64_t fp = sh.identity ? iu : sh.umem[sh.uoff[iu]];
+    int64_t ifm = fp % sh.nface, ifs = fp / sh.nface;
+    const double *rsp = sh.fxs + 6 * ifs, *rm = sh.fxs + 6 * ifm;
+    double rs[3][2] = {{rsp[0], rsp[1]}, {rsp[2], rsp[3]}, {rsp[4], rsp[5]}};
+    double rs32x = rs[2][0] - rs[1][0], rs32y = rs[2][1] - rs[1][1];
+    double rs12x = rs[0][0] - rs[1][0], rs12y = rs[0][1] - rs[1][1];
+    double area2 = (-rs32y) * rs12x + rs32x * rs12y;
+    double unz = 1.0;
+    if (area2 < 0) { area2 = fabs(area2); unz = -1.0; }
+    double rmcx = ((rm[0] + rm[2]) + rm[4]) / 3.0;
+    double rmcy = ((rm[1] + rm[3]) + rm[5]) / 3.0;
+    double rrx = 0, rry = 0, ri = 0;
+    for (int i = 0; i < 3; ++i) {
+        int ip1 = (i + 1) % 3;
+        double lx = rs[ip1][0] - rs[i][0], ly = rs[ip1][1] - rs[i][1];
+        double ln = sqrt(lx * lx + ly * ly);
+        lx /= ln;
+        ly /= ln;
+        double ux = (-unz) * (-ly), uy = (-unz) * lx;
+        double p0 = ux * (rs[i][0] - rmcx) + uy * (rs[i][1] - rmcy);
+        double p0sq = p0 * p0;
+        double lp = lx * (rs[ip1][0] - rmcx) + ly * (rs[ip1][1] - rmcy);
+        double lm = lx * (rs[i][0] - rmcx) + ly * (rs[i][1] - rmcy);
+        double ledge = lp - lm;
+        double pplus = sqrt(p0sq + lp * lp), pminus = sqrt(p0sq + lm * lm);
+        double factor;
+        if (fabs(p0 / ledge) < 1e-4) {
+            factor = lp * pplus - lm * pminus;
+        } else {
+            factor = p0 * log((pplus + lp) / (pminus + lm));
+            ri += factor;
+            factor = (factor * p0 + pplus * lp) - pminus * lm;
+        }
+        rrx += ux * factor;
+        rry += uy * factor;
+    }
+    rrx *= 0.5;
+    rry *= 0.5;
+    rrx /= area2;
+    rry /= area2;
+    ri /= (area2 * ul);
+    rho_r[2 * iu] = rrx;
+    rho_r[2 * iu + 1] = rry;
+    rinv[iu] = ri;
+}
+
+}  // namespace
+
+void launch_summands(const TabParams *d_tp, int npts, int64_t cap, int r0, int r1, cd *t1,
+                     cd *t2, hipStream_t st) {
+    int64_t e0 = r0 == 0 ? 0 : (int64_t)(2 * r0 - 1) * (2 * r0 - 1);
+    int64_t e1 = (int64_t)(2 * r1 + 1) * (2 * r1 + 1);
+    if (e1 > cap) throw ArgError("ring capacity exceeded");
+    int64_t cnt = e1 - e0;
+    dim3 grid((unsigned)((cnt + 127) / 128), (unsigned)npts);
+    hipLaunchKernelGGL(k_summands, grid, dim3(128), 0, st, d_tp, cap, e0, e1, t1, t2);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_ringsums(const TabParams *, int npts, int64_t cap, int r0, int r1, int rcap,
+                     const cd *t1, const cd *t2, double convtest, int *flags, hipStream_t st) {
+    int cnt = r1 - r0 + 1;
+    dim3 grid((unsigned)((cnt + 63) / 64), (unsigned)npts);
+    hipLaunchKernelGGL(k_ringsums, grid, dim3(64), 0, st, cap, r0, r1, rcap, t1, t2, convtest,
+                       flags);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_gather_square(int npts, const int *d_M, int Mmax, int64_t cap, const cd *t1,
+                          const cd *t2, cd *S, int64_t s_stride, hipStream_t st) {
+    dim3 blk(16, 16);
+    dim3 grid((unsigned)((Mmax + 15) / 16), (unsigned)((Mmax + 15) / 16), (unsigned)npts);
+    hipLaunchKernelGGL(k_gather_square, grid, blk, 0, st, d_M, cap, t1, t2, S, s_stride);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_dft_matrix(int M, cd *W, hipStream_t st) {
+    dim3 blk(16, 16);
+    dim3 grid((unsigned)((M + 15) / 16), (unsigned)((M + 15) / 16));
+    hipLaunchKernelGGL(k_dft_matrix, grid, blk, 0, st, M, W);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_phase_ghost(int npts, const int *d_M, const double *d_psi, const int *d_mag,
+                        const double *d_area, const cd *F, int64_t f_stride, cd *tab,
+                        int64_t tab_stride, int Mmax, hipStream_t st) {
+    dim3 blk(16, 16);
+    dim3 grid((unsigned)((Mmax + 2 + 15) / 16), (unsigned)((Mmax + 2 + 15) / 16),
+              (unsigned)npts);
+    hipLaunchKernelGGL(k_phase_ghost, grid, blk, 0, st, d_M, d_psi, d_mag, d_area, F, f_stride,
+                       tab, tab_stride);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us1,
+               const double *us2, const double *d_psi, int nf, cd *jk, int64_t jk_stride,
+               int mixed, hipStream_t st) {
+    if (nf < 1 || nf > 8) throw ArgError("launch_jk: 1..8 phasings per launch");
+    int64_t per_block = 36;
+    dim3 grid((unsigned)((nufp + per_block - 1) / per_block));
+    int *rm = nullptr;
+#define JKL(FF, MX)                                                                        \
+    hipLaunchKernelGGL((k_jk<FF, MX>), grid, dim3(256), 0, st, sh, nufp, ulocal, us1[0],  \
+                       us1[1], us2[0], us2[1], d_psi, nf, jk, jk_stride, rm)
+    if (mixed) {
+        if (nf <= 1) JKL(1, true); else if (nf <= 2) JKL(2, true);
+        else if (nf <= 4) JKL(4, true); else JKL(8, true);
+    } else {
+        if (nf <= 1) JKL(1, false); else if (nf <= 2) JKL(2, false);
+        else if (nf <= 4) JKL(4, false); else JKL(8, false);
+    }
+#undef JKL
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_singular(const SheetDev &sh, int64_t nufp, double ulocal, double *rho_r,
+                     double *rinv, hipStream_t st) {
+    dim3 grid((unsigned)((nufp + 255) / 256));
+    hipLaunchKernelGGL(k_singular, grid, dim3(256), 0, st, sh, nufp, ulocal, rho_r, rinv);
+    HIPCHECK(hipGetLastError());
+}

@@ -73,11 +73,13 @@ def allocate_mode_arrays(layer: Layer) -> None:
 
 def setup_modes(layer: Layer, k0: float, kvec) -> None:
     """Per-point modal constants beta, gamma, Y, c, tvec (src/Modes.jl:298-322)."""
+    nm = len(layer.M)
+    if nm == 0:
+        return
     b00 = np.asarray(kvec, float)
     b1, b2 = layer.beta1, layer.beta2
     area = TWOPI ** 2 / abs(b1[0] * b2[1] - b1[1] * b2[0])
     ksq = k0 ** 2 * layer.epsr * layer.mur
-    nm = len(layer.M)
     if layer.beta.shape[0] != nm:
         allocate_mode_arrays(layer)
     for mode in range(nm):

@@ -1,0 +1,284 @@
+"""GPU parity of every stage of the HIP path against the oracle (FP64 CPU
+restatement) on the same inputs, through the C ABI.
+
+Bars: FP64 stages to ~1e-12 relative (tables, spatial sums, transforms, solves),
+Z to a relative Frobenius error <= 1e-10, S to max |dS| <= 1e-9; the mixed
+path (FP32 LU / FP32 smooth sums + FP64 refinement) to max |dS| <= 1e-6.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def _fillzy_case(pk, cls):
+    FGHz = 2.0
+    k0 = 2 * math.pi / (11.80285 / FGHz * 0.0254)
+    L = pk.Layer.make
+    layers = [L(), L(epsr=3.4, tandel=0.003, width=1, units="mil"),
+              L(epsr=3.4, tandel=0.003, width=1, units="mil")]
+    sh = pk.rectstrip(Nx=20, Ny=20, Lx=20, Ly=50, Px=100, Py=100, units="mil", sheet_class=cls)
+    u = 0.5 * max(np.linalg.norm(sh.beta1), np.linalg.norm(sh.beta2)) * sh.upm
+    return layers, sh, u, k0
+
+
+# ------------------------------------------------------------------ linear algebra
+@pytest.mark.parametrize("n,nrhs,nb", [(1, 1, 1), (63, 2, 2), (64, 4, 3), (65, 3, 1),
+                                       (130, 5, 2), (300, 4, 2)])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lu_solve(ctx, n, nrhs, nb, prec):
+    _say(f"lu_solve n={n} nrhs={nrhs} batch={nb} {prec}")
+    rng = np.random.default_rng(n * 7 + nrhs)
+    A = rng.standard_normal((nb, n, n)) + 1j * rng.standard_normal((nb, n, n))
+    A += np.eye(n) * (2 * math.sqrt(n))
+    B = rng.standard_normal((nb, n, nrhs)) + 1j * rng.standard_normal((nb, n, nrhs))
+    X, it, rs = ctx.lu_solve(A, B, prec)
+    for b in range(nb):
+        ref = np.linalg.solve(A[b], B[b])
+        assert _rel(X[b], ref) < 1e-12, (b, _rel(X[b], ref))
+
+
+def test_lu_pivoting(ctx):
+    _say("lu_solve with forced row interchanges")
+    rng = np.random.default_rng(3)
+    n = 150
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A[np.arange(n), np.arange(n)] = 1e-6      # tiny diagonal: pivoting is mandatory
+    B = rng.standard_normal((n, 2)) + 0j
+    X, _, _ = ctx.lu_solve(A, B, "fp64")
+    assert _rel(X[0], np.linalg.solve(A, B)) < 1e-11
+
+
+@pytest.mark.parametrize("n1,n2,n3", [(1, 1, 1), (2, 2, 2), (2, 6, 3), (10, 20, 15), (34, 34, 34)])
+def test_cascade(ctx, pk, n1, n2, n3):
+    _say(f"cascade {n1},{n2},{n3}")
+    rng = np.random.default_rng(n1 + 10 * n2 + 100 * n3)
+
+    def rnd(r, c):
+        return 0.3 * (rng.standard_normal((r, c)) + 1j * rng.standard_normal((r, c))) / math.sqrt(c)
+    a = [pk.GSM(rnd(n1, n1), rnd(n1, n2), rnd(n2, n1), rnd(n2, n2)) for _ in range(3)]
+    b = [pk.GSM(rnd(n2, n2), rnd(n2, n3), rnd(n3, n2), rnd(n3, n3)) for _ in range(3)]
+    out = ctx.cascade_batch(a, b)
+    for x, y, c in zip(a, b, out):
+        ref = pk.cascade(x, y)
+        for m in ("s11", "s12", "s21", "s22"):
+            assert _rel(getattr(c, m), getattr(ref, m)) < 1e-12
+
+
+# ------------------------------------------------------------------ PGF stages
+def _point_for_tables(pk, layers, k0, u, s, psi):
+    from pssfss_amd.hip import PointBuilder
+    sh = pk.rectstrip(Nx=1, Ny=1, Lx=1, Ly=1, Px=2, Py=2, units="mm")
+    pb = PointBuilder(layers, sh, u, s, k0, np.zeros(2), with_modes=False)
+    pb.point.psi1, pb.point.psi2 = psi
+    return pb
+
+
+@pytest.mark.parametrize("magnetic", [0, 1])
+def test_modal_tables_pgf_stack(ctx, pk, orc, magnetic):
+    _say(f"modal tables, 5-layer stack, magnetic={magnetic}")
+    import ctypes as C
+    from pssfss_amd.hip import lib, _d
+    from pssfss_amd.sheets import s1s2_to_beta
+    k0 = 2 * math.pi / (11.80285 / 2.0 * 0.0254)
+    L = pk.Layer.make
+    layers = [L(), L(width=20, units="mil", epsr=3.6, tandel=0.0013, mur=5.0),
+              L(width=10, units="mil", epsr=8.0, tandel=0.0013, mur=2.0),
+              L(width=50, units="mil", epsr=2.6, tandel=0.0003), L()]
+    psi = (0.4, -0.6)
+    inch = 0.0254
+    b1, b2 = s1s2_to_beta([1.14 * inch, 0.0], [0.5707 * inch, 0.9885 * inch])
+    b00 = (psi[0] * b1 + psi[1] * b2) / (2 * math.pi)
+    u = 0.1 * max(np.linalg.norm(b1), np.linalg.norm(b2))
+    pb = _point_for_tables(pk, layers, k0, u, 2, psi)
+    t1 = np.zeros(2 * 2050 * 2050)
+    t2 = np.zeros(2 * 2050 * 2050)
+    M = C.c_int()
+    cv = C.c_int()
+    ctx.check(lib().pssfss_modal_tables(ctx.h, magnetic, C.byref(pb.point), _d(np.array(b1)),
+                                        _d(np.array(b2)), _d(np.array(b00)), 5e-12,
+                                        C.byref(M), C.byref(cv), _d(t1), _d(t2)))
+    Mo, o1, o2, conv = orc.modal_tables(magnetic, k0, u, psi[0], psi[1], layers, 2, b1, b2, b00,
+                                        5e-12)
+    assert M.value == Mo and bool(cv.value) == conv
+    E = (Mo + 2) ** 2
+    g1 = t1[:2 * E].view(np.complex128)
+    g2 = t2[:2 * E].view(np.complex128)
+    r1 = o1.view(np.complex128)
+    r2 = o2.view(np.complex128)
+    assert np.abs(g1 - r1).max() / np.abs(r1).max() < 1e-12
+    assert np.abs(g2 - r2).max() / np.abs(r2).max() < 1e-12
+
+
+@pytest.mark.parametrize("psi", [(0.0, 0.0), (0.3, -0.7)])
+def test_filljk(ctx, pk, orc, psi):
+    _say(f"filljk rectstrip 20x20 mil psi={psi}")
+    from pssfss_amd.hip import DeviceSheet
+    layers, sh, u, k0 = _fillzy_case(pk, "J")
+    rw = pk.setup_rwg(sh)
+    ds = DeviceSheet(ctx, sh, rw)
+    ul = u / sh.upm
+    jk, rr, ri = ds.filljk(ul, *psi)
+    ojk, orr, ori = orc.filljk(sh, rw, ul, *psi)
+    ds.free()
+    assert _rel(jk, ojk) < 1e-12
+    assert _rel(rr, orr) < 1e-13
+    assert _rel(ri, ori) < 1e-13
+
+
+def test_rwgbfft_golden(ctx, pk, orc):
+    _say("rwgbfft golden vectors of the reference")
+    from pssfss_amd.hip import DeviceSheet
+    sh = pk.rectstrip(Lx=1, Ly=1, Px=1, Py=1, Nx=2, Ny=1, units="inch", fufp=False)
+    rw = pk.setup_rwg(sh)
+    ds = DeviceSheet(ctx, sh, rw)
+    ft = ds.rwgbfft(np.array([20.0, -30.3]), 0.8, -0.7)
+    ft0 = ds.rwgbfft(np.array([0.0, 0.0]), 0.0, 0.0)
+    ds.free()
+    ref = np.array([[0.003986416685586114 - 0.0010511300702857503j,
+                     -0.008079847138694855 + 0.0021304773084817956j],
+                    [0.008263557134402535 - 0.0010871639073400372j,
+                     0.008284527320353664 - 0.0010899227712197804j],
+                    [0.004122638196063312 - 1.5707327530113332e-5j,
+                     -0.008355946971820058 + 3.1836312009155855e-5j]])
+    assert np.abs(ft[:3] - ref).max() / np.abs(ref).max() < 1e-12
+    assert _rel(ft0, orc.rwgbfft(sh, rw, np.zeros(2), 0.0, 0.0)) < 1e-13
+
+
+# ------------------------------------------------------------------ fill
+@pytest.mark.parametrize("cls", ["J", "M"])
+def test_fill_golden_and_oracle(ctx, pk, orc, cls):
+    _say(f"fill rectstrip 20x20 mil class {cls}")
+    from pssfss_amd.hip import DeviceSheet, PointBuilder
+    layers, sh, u, k0 = _fillzy_case(pk, cls)
+    rw = pk.setup_rwg(sh)
+    ds = DeviceSheet(ctx, sh, rw)
+    pb = PointBuilder(layers, sh, u, 2, k0, np.zeros(2), with_modes=False)
+    Z, info = ds.fill(pb)
+    ds.free()
+    upm = sh.upm
+    b1m, b2m = sh.beta1 * upm, sh.beta2 * upm
+    JK, rr, ri = orc.filljk(sh, rw, u / upm, 0.0, 0.0)
+    tabs = orc.modal_tables(cls == "M", k0, u, 0.0, 0.0, layers, 2, b1m, b2m, np.zeros(2), 1e-7)
+    Zo = orc.fill(sh, rw, JK, rr, ri, k0, u, layers, 2, tabs[:3], b1m, b2m, 0.0, 0.0)
+    assert info.mmax == tabs[0]
+    err = _rel(Z, Zo)
+    print(f"  rel Frobenius |Z-Zo|/|Zo| = {err:.3e}")
+    assert err <= 1e-10
+    if cls == "J":    # test/FillZY_test.jl:45-46
+        assert abs(Z[19, 29] - (0.8990238712012797 - 354.1682770720877j)) < 0.005
+        assert abs(Z[99, 99] - (110.95156005861463 - 37529.820828933574j)) < 0.01
+    else:             # test/FillZY_test.jl:77-78
+        assert abs(Z[19, 29] - (1.2168396602563192e-6 - 0.02476973884018213j)) < 1e-4
+        assert abs(Z[99, 99] - (2.2127855040521283e-6 - 3.5131557304457406j)) < 1e-4
+
+
+def test_fill_oblique_unstructured_classes(ctx, pk, orc):
+    _say("fill annular ring (every face pair its own class), oblique phasing")
+    from pssfss_amd.hip import DeviceSheet, PointBuilder
+    strata, freqs, steering = pk.workloads.ring_on_slab(nr=2, nphi=24, nfreq=1)
+    layers = [s for s in strata if isinstance(s, pk.Layer)]
+    sh = [s for s in strata if not isinstance(s, pk.Layer)][0]
+    rw = pk.setup_rwg(sh)
+    k0 = 2 * math.pi * 17e9 / 299792458.0
+    kinc = k0 * math.sin(math.radians(30)) * np.array([math.cos(0.7), math.sin(0.7)])
+    u = 0.5 * max(np.linalg.norm(sh.beta1), np.linalg.norm(sh.beta2)) * sh.upm
+    ds = DeviceSheet(ctx, sh, rw)
+    pb = PointBuilder(layers, sh, u, 1, k0, kinc, with_modes=False)
+    Z, info = ds.fill(pb)
+    ds.free()
+    psi1, psi2 = pb.psi1, pb.psi2
+    upm = sh.upm
+    b1m, b2m = sh.beta1 * upm, sh.beta2 * upm
+    b00 = (psi1 * b1m + psi2 * b2m) / (2 * math.pi)
+    JK, rr, ri = orc.filljk(sh, rw, u / upm, psi1, psi2)
+    tabs = orc.modal_tables(False, k0, u, psi1, psi2, layers, 1, b1m, b2m, b00, 1e-7)
+    Zo = orc.fill(sh, rw, JK, rr, ri, k0, u, layers, 1, tabs[:3], b1m, b2m, psi1, psi2)
+    err = _rel(Z, Zo)
+    print(f"  rel Frobenius |Z-Zo|/|Zo| = {err:.3e}  mmax={info.mmax}")
+    assert err <= 1e-10
+
+
+# ------------------------------------------------------------------ end to end
+def _max_dS(res, ref):
+    m = 0.0
+    for a, b in zip(res, ref):
+        for blk in ("s11", "s12", "s21", "s22"):
+            m = max(m, np.abs(getattr(a["gsm"], blk) - getattr(b["gsm"], blk)).max())
+    return m
+
+
+@pytest.mark.parametrize("cls", ["J", "M"])
+def test_analyze_patch(ctx, pk, orc, cls):
+    _say(f"analyze patch in free space (n=8, 5 freqs), class {cls}")
+    strata, freqs, steer = pk.workloads.patch_free_space(n=8, nfreq=5, sheet_class=cls)
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    d = _max_dS(res, ref)
+    print(f"  max |dS| fp64 = {d:.3e}")
+    assert len(res) == len(ref) == 5
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
+def test_analyze_ring_oblique(ctx, pk, orc):
+    _say("analyze ring on slab (nr=2, nphi=24), theta=30, phi=0/45, 3 freqs")
+    strata, freqs, steer = pk.workloads.ring_on_slab(nr=2, nphi=24, nfreq=3)
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    d = _max_dS(res, ref)
+    print(f"  max |dS| fp64 = {d:.3e}")
+    assert len(res) == len(ref) == 6
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
+def _grating(kP, nterms=30):
+    """Collin's strip-grating closed form as used by test/full_test.jl:27-42."""
+    sum1 = 1.3862943611198906
+    sum3 = 7.2123414189575710
+    x = kP / (4 * math.pi)
+    th = x * sum1 + x ** 3 / 6 * sum3
+    for n in range(1, nterms + 1):
+        a = x / (n - 0.5)
+        b = x / n
+        th += math.asin(a) - (a + a ** 3 / 6) - (math.asin(b) - (b + b ** 3 / 6))
+    G = math.sin(th) * complex(math.cos(-math.pi / 2 - th), math.sin(-math.pi / 2 - th))
+    return G, 1 + G
+
+
+@pytest.mark.parametrize("cls", ["J", "M"])
+def test_grating_closed_form(ctx, pk, cls):
+    _say(f"symmetric strip grating vs Collin closed form, class {cls}")
+    FGHz = 0.8
+    r, t = _grating(2 * math.pi * FGHz)
+    period = 299792458.0 * 100 / 2.54 * 1e-9
+    Py = period
+    Ly = period / 2
+    Px = Lx = Ly / 10
+    Ny = 40
+    Nx = round(Ny * Lx / Ly)
+    sh = pk.rectstrip(Px=Px, Py=Py, Lx=Lx, Ly=Ly, Nx=Nx, Ny=Ny, units="inch", sheet_class=cls)
+    strata = [pk.Layer.make(), sh, pk.Layer.make()]
+    res = pk.analyze.analyze(strata, [FGHz], {"theta": [0.0], "phi": [0.0]}, ctx=ctx)
+    g = res[0]["gsm"]
+    ex = {"s11": np.array([[r, 0], [0, -t]]), "s22": np.array([[r, 0], [0, -t]]),
+          "s21": np.array([[t, 0], [0, -r]]), "s12": np.array([[t, 0], [0, -r]])}
+    for k, v in ex.items():
+        assert np.abs(getattr(g, k) - v).max() < 0.01, k

@@ -1,0 +1,133 @@
+"""Host-side logic (mesh, RWG maps, rings, GSM algebra) against the reference's
+exact fixtures, and the C-ABI library's exports (no GPU work)."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+
+def test_rings(pk):
+    """test/Rings_test.jl."""
+    from pssfss_amd.rings import ring
+    assert ring(0) == [(0, 0)]
+    assert ring(1) == [(1, -1), (1, 0), (1, 1), (-1, -1), (-1, 0), (-1, 1), (0, -1), (0, 1)]
+    assert ring(2) == [(2, -2), (2, -1), (2, 0), (2, 1), (2, 2), (-2, -2), (-2, -1), (-2, 0),
+                       (-2, 1), (-2, 2), (-1, -2), (0, -2), (1, -2), (-1, 2), (0, 2), (1, 2)]
+    assert len(ring(10)) == 80
+    assert all(max(abs(m), abs(n)) == 100 for m, n in ring(100))
+
+
+@pytest.mark.parametrize("fufp", [True, False])
+def test_rwgdata_maps(pk, fufp):
+    """test/RWGData_test.jl:7-37 (exact maps of a 4-face periodic strip)."""
+    sh = pk.rectstrip(Lx=1, Ly=1, Px=1, Py=1, Nx=2, Ny=1, units="mm", fufp=fufp)
+    d = pk.setup_rwg(sh)
+    assert (d.bfe + 1).tolist() == [[8, 6, 9, 1, 2, 5], [8, 6, 9, 3, 4, 7]]
+    assert (d.bff + 1).tolist() == [[1, 2, 3, 2, 4, 1], [2, 3, 4, 1, 3, 4]]
+    assert (d.ebf + 1).tolist() == [4, 5, 4, 5, 6, 2, 6, 1, 3]
+    assert d.eci.tolist() == [3, 3, 4, 4, 1, 0, 2, 0, 0]
+    if fufp:
+        assert (d.ufpm + 1).tolist() == [[1, 5, 9, 11], [2, 6, 10, 12], [3, 7, 1, 5],
+                                         [4, 8, 2, 6]]
+        classes = [(d.ufp_members[d.ufp_offsets[i]:d.ufp_offsets[i + 1]] + 1).tolist()
+                   for i in range(d.nufp)]
+        assert classes == [[1, 11], [2, 12], [3], [4], [5, 15], [6, 16], [7], [8], [9], [10],
+                           [13], [14]]
+        assert d.nufp == 12
+    else:
+        assert d.ufp_identity and d.nufp == 16
+
+
+def test_recttri_counts(pk):
+    """src/Sheets.jl:398-480 counts; fe[i] is opposite fv[i]."""
+    sh = pk.recttri([0, 0], [3, 2], 3, 2)
+    assert sh.rho.shape[0] == 12 and sh.nedge == 3 * 6 + 5 and sh.nface == 12
+    for f in range(sh.nface):
+        for i in range(3):
+            e = sh.fe[i, f]
+            assert sh.fv[i, f] not in (sh.e1[e], sh.e2[e])
+
+
+def test_annulus_mesh(pk):
+    sh = pk.annular_ring(a=3.0, b=3.6, P=9.0, nr=6, nphi=168, units="mm")
+    d = pk.setup_rwg(sh)
+    assert sh.nface == 2016 and d.nbf == (3 * 6 - 1) * 168
+    assert d.ufp_identity and d.nufp == sh.nface ** 2
+
+
+def test_structured_ufp_count(pk):
+    """Translation classes of the 14x14 patch mesh: far fewer than nface^2."""
+    strata, _, _ = pk.workloads.patch_free_space()
+    d = pk.setup_rwg(strata[1])
+    assert strata[1].nface == 392 and d.nbf == 560
+    assert d.nufp < 392 ** 2 // 20
+    # every face pair belongs to exactly one class
+    assert d.ufp_offsets[-1] == 392 ** 2
+    assert np.array_equal(np.sort(d.ufp_members), np.arange(392 ** 2))
+
+
+def test_cascade_identity_and_one_mode(pk):
+    """test/GSMs_test.jl:17-59: identity cascade and the 1-mode closed form."""
+    a = pk.GSM.new(10, 20)
+    b = pk.GSM.new(20, 15)
+    c = pk.cascade(a, b)
+    assert np.linalg.norm(c.s11) == 0 and np.linalg.norm(c.s22) == 0
+    assert np.array_equal(c.s12, np.eye(10, 15)) and np.array_equal(c.s21, np.eye(15, 10))
+    al, ga = 0.35, 0.23
+    be, de = math.sqrt(1 - al ** 2), math.sqrt(1 - ga ** 2)
+    a = pk.GSM.new(1, 1)
+    b = pk.GSM.new(1, 1)
+    b.s11[:] = b.s22[:] = ga
+    b.s12[:] = b.s21[:] = de
+    c = pk.cascade(a, b)
+    D = 1 / (1 - a.s22[0, 0] * b.s11[0, 0])
+    assert abs(c.s11[0, 0] - (a.s11[0, 0] + a.s12[0, 0] * a.s21[0, 0] * b.s11[0, 0] / D)) <= 1e-16
+    assert abs(c.s21[0, 0] - a.s21[0, 0] * b.s21[0, 0] / D) <= 1e-16
+    assert abs(c.s22[0, 0] - (b.s22[0, 0] + b.s12[0, 0] * b.s21[0, 0] * a.s22[0, 0] / D)) <= 1e-16
+
+
+def test_cascade_layer(pk):
+    """test/GSMs_test.jl:62-94: propagation through a slab."""
+    a = pk.GSM.new(1, 1)
+    a.s11[:] = a.s22[:] = 0.35
+    a.s12[:] = a.s21[:] = math.sqrt(1 - 0.35 ** 2)
+    s22 = a.s22.copy()
+    lay = pk.Layer.make(width=25, units="mm")
+    lay.gamma = np.array([1j])
+    p = np.exp(-1j * 0.025)
+    s12 = a.s12 * p
+    pk.cascade_layer(a, lay)
+    assert abs(a.s12[0, 0] - s12[0, 0]) <= 1e-16
+    assert abs(a.s22[0, 0] - s22[0, 0] * p * p) <= 1e-15
+
+
+def test_gblocks_and_modes(pk):
+    """Thin slab joins the sheet's Gblock; end regions carry the TE/TM pair."""
+    strata, freqs, steer = pk.workloads.ring_on_slab(nr=2, nphi=24, nfreq=3)
+    su = pk.analyze.prepare(strata, freqs, steer)
+    assert [(list(g.rng), g.j) for g in su.gbls] == [([1, 2], 1)]
+    assert len(su.layers[0].P) == 2 and len(su.layers[-1].P) == 2
+    pts = list(su.points())
+    assert len(pts) == 6 and pts[0].steer == {"theta": 30.0, "phi": 0.0}
+
+
+def test_library_exports(pk):
+    """libpssfss_hip.so loads and exports every entry point declared in the header."""
+    from pssfss_amd import hip
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "pssfss_hip.h")).read()
+    declared = set(re.findall(r"\b(pssfss_[a-z0-9_]+)\s*\(", hdr))
+    assert declared == set(hip.EXPORTS)
+    L = ctypes.CDLL(hip.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_struct_layout_matches_header(pk):
+    from pssfss_amd import hip
+    assert ctypes.sizeof(hip.Point) == 184
+    assert ctypes.sizeof(hip.SheetDesc) == 192
+    assert ctypes.sizeof(hip.PointInfo) == 32
