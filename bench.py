@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Frequency points per second of the per-point MoM path (spatial sums, spectral
+tables, fill, LU, RHS/solve, extraction, cascade) on one MI355X.
+
+Workload (default): PEC annular ring (a = 3.0 mm, b = 3.6 mm) in a 9 mm cell on a
+0.787 mm eps_r = 2.2 / tan_delta = 0.0009 slab, 2016 faces / 2856 unknowns,
+theta = 30 deg, phi in {0, 45} deg, 51 frequencies 5..25 GHz: 102 points.
+One step = the whole sweep (all 102 points) with the sheet resident on the
+device; meshing, setup_rwg and mode selection are outside the timed region.
+
+value = points/s of the mixed path (FP32 LU and smooth spatial sums, FP64
+refinement, singular terms and tables); the plain FP64 path is timed beside it
+and max|dS| between them is reported, as is max|dS| of both against the oracle
+on the CPU-baseline sample point.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _sub in ("pssfss.jl_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, _sub))
+
+PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 / FP32 matrix
+
+
+def max_dS(a, b):
+    import numpy as np
+    m = 0.0
+    for x, y in zip(a, b):
+        for blk in ("s11", "s12", "s21", "s22"):
+            m = max(m, float(np.abs(getattr(x["gsm"], blk) - getattr(y["gsm"], blk)).max()))
+    return m
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="ring", choices=["ring", "patch"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precisions", default="fp64,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(points shard across GPUs independently; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import pssfss_amd as pk
+    from pssfss_amd import hip
+
+    if args.workload == "ring":
+        strata, freqs, steer = pk.workloads.ring_on_slab()
+        wname = ("ring_on_slab: PEC annulus a=3.0 b=3.6 mm, 9 mm cell, 0.787 mm eps_r=2.2 "
+                 "tan_d=0.0009 slab, theta=30, phi={0,45}, 51 freqs 5-25 GHz")
+    else:
+        strata, freqs, steer = pk.workloads.patch_free_space()
+        wname = "patch_free_space: 5 mm PEC patch, 10 mm cell, 21 freqs 2-30 GHz, normal incidence"
+    npts, nface, nbf = pk.workloads.sheet_points(strata, freqs, steer)
+    ctx = hip.get_context()
+    dev = ctx.device_info()
+    print(f"[bench] device {dev}; workload {args.workload}: {npts} points, {nface} faces, "
+          f"{nbf} unknowns", flush=True)
+    setup = pk.analyze.prepare(strata, freqs, steer)
+    precs = [p for p in args.precisions.split(",") if p]
+    solvers = {}
+    shared = None
+    for p in precs:
+        s = hip.SheetSolver(ctx, setup, p)
+        if shared is None:
+            shared = s.dsheets
+        s.dsheets = shared
+        solvers[p] = s
+
+    timing, results, stages = {}, {}, {}
+    for p in precs:
+        for w in range(args.warmup):
+            t0 = time.perf_counter()
+            solvers[p].run()
+            print(f"[bench] warmup {p} step {w}: {time.perf_counter() - t0:.3f} s", flush=True)
+        ctx.stats_read(reset=True)
+        ctx.stats_enable(True)
+        times = []
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            res, info = solvers[p].run()
+            times.append(time.perf_counter() - t0)
+            print(f"[bench] {p} step {k}: {times[-1]:.3f} s", flush=True)
+        ctx.stats_enable(False)
+        stages[p] = ctx.stats_read(reset=True)
+        results[p] = res
+        timing[p] = times
+        iters = [i.refine_iters for i in info]
+        fb = sum(i.fallback_fp64 for i in info)
+        if p == "mixed":
+            print(f"[bench] mixed refinement sweeps min/max {min(iters)}/{max(iters)}, "
+                  f"FP64 fallbacks {fb}", flush=True)
+
+    head = "mixed" if "mixed" in timing else precs[0]
+    step_s = sum(timing[head]) / len(timing[head])
+    value = npts / step_s
+    extra = {}
+    for p in precs:
+        extra[f"points_per_s_{p}"] = npts / (sum(timing[p]) / len(timing[p]))
+    if "mixed" in results and "fp64" in results:
+        extra["max_dS_mixed_vs_fp64"] = max_dS(results["mixed"], results["fp64"])
+        extra["speedup_mixed_vs_fp64"] = extra["points_per_s_mixed"] / extra["points_per_s_fp64"]
+
+    # Roofline of the LU trailing-update GEMM (the unit of algorithmic flops).
+    st = stages[head]
+    g = st["lu_gemm"]
+    achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    roof = {"bound": "mfma", "kernel": "k_gemm_nn (LU trailing update)",
+            "achieved": round(achieved, 3), "peak": PEAK[head], "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK[head], 4),
+            "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4),
+            "traffic": None}
+    tot_ms = sum(v["ms"] for k, v in st.items() if k != "lu_total_flops")
+    stage_ms = {k: round(v["ms"] / args.steps, 2) for k, v in st.items() if k != "lu_total_flops"}
+    lu_flops = st["lu_total_flops"]["flops"] / args.steps
+    extra["stage_ms_per_step"] = stage_ms
+    extra["dominant_stage"] = max(stage_ms, key=stage_ms.get)
+    extra["lu_tflops_per_s_whole_step"] = round(lu_flops / step_s / 1e12, 3)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, errs = cpu_baseline(pk, strata, freqs, steer, results)
+        extra.update(errs)
+
+    line = {
+        "metric": "freq points/s (fill+LU+GSM) mixed vs FP64 at |dS|<=1e-6",
+        "value": round(value, 3), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": wname, "points": npts, "faces": nface, "unknowns": nbf,
+                   "precision": head},
+        "roofline": roof, "cpu_baseline": cpu, "detail": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(pk, strata, freqs, steer, gpu_results):
+    """Oracle (FP64 CPU restatement) on the first point of the sweep: points/s on
+    the host's cores, and max|dS| of each GPU path against it at that point."""
+    import oracle
+    f0 = [float(freqs[0])]
+    st0 = {k: [list(v)[0]] for k, v in steer.items()}
+    t0 = time.perf_counter()
+    ref = oracle.analyze(pk, strata, f0, st0)
+    dt = time.perf_counter() - t0
+    cores = oracle.lib().orc_num_threads()
+    errs = {}
+    for p, res in gpu_results.items():
+        errs[f"max_dS_{p}_vs_oracle_point0"] = max_dS(res[:1], ref)
+    keys = list(steer.keys())
+    sample = (f"1 point ({f0[0]} GHz, {keys[0]}={st0[keys[0]][0]}, {keys[1]}={st0[keys[1]][0]}) "
+              f"of the same workload, full mesh, {dt:.1f} s")
+    return ({"value": round(1.0 / dt, 5), "unit": "points/s", "cores": cores, "kind": "port",
+             "sample": sample}, errs)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,6 +161,22 @@ int pssfss_lu_solve_batch(pssfss_ctx *ctx, int64_t n, int nrhs, int nbatch, cons
 int pssfss_cascade_batch(pssfss_ctx *ctx, int nbatch, int n1, int n2, int n3,
                          const double *a, const double *b, double *c);
 
+/* ---- measurement (no counterpart in the reference) ---------------------------
+ * Stage timing with HIP events recorded on the context's stream around every
+ * launch of a stage.  Stages: 0 spatial sums, 1 spectral tables, 2 fill,
+ * 3 LU panels + interchanges + U12 solves, 4 LU trailing GEMM updates,
+ * 5 triangular solves and refinement, 6 transforms, RHS and projection.
+ * flops[] holds the algorithmic flop count of stage 4 (8 m n k per complex
+ * GEMM update) and of the whole LU (8/3 n^3 per system) in flops[7]. */
+typedef struct {
+    double ms[8];
+    double flops[8];
+    int64_t launches[8];
+} pssfss_stats;
+
+int pssfss_stats_enable(pssfss_ctx *ctx, int on);
+int pssfss_stats_read(pssfss_ctx *ctx, pssfss_stats *out, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,8 +72,12 @@ bool approx2(const double *a, const double *b) {
 
 }  // namespace
 
+thread_local StageRecorder *g_stage_rec = nullptr;
+
 struct pssfss_ctx {
     int device = 0;
+    bool stats_on = false;
+    StageRecorder rec;
     hipStream_t st = nullptr;
     std::string err;
     std::map<int, DBuf<cd>> dft;     // DFT matrices by size
@@ -127,8 +131,13 @@ namespace {
 template <class F>
 int guarded(pssfss_ctx *ctx, F &&f) {
     if (!ctx) return PSSFSS_EINVAL;
+    struct RecScope {
+        RecScope(pssfss_ctx *c) { g_stage_rec = c->stats_on ? &c->rec : nullptr; }
+        ~RecScope() { g_stage_rec = nullptr; }
+    } scope(ctx);
     try {
         f();
+        if (ctx->stats_on) ctx->rec.flush();
         ctx->err.clear();
         return PSSFSS_OK;
     } catch (const ArgError &e) {
@@ -294,9 +303,11 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
                 rcap = nr;
                 cap = ncap;
             }
+            stage_begin(1, st);
             launch_summands(d_tp.p, np, cap, computed + 1, need, t1.p, t2.p, st);
             launch_ringsums(d_tp.p, np, cap, computed + 1, need, rstride, t1.p, t2.p, convtest,
                             dflags.p, st);
+            stage_end(1, st);
             d2h(ctx, hflags.data(), dflags.p, hflags.size());
             sync(ctx);
             computed = need;
@@ -340,6 +351,7 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
     dmag.upload(hmag.data(), np, st);
     dpsi.upload(hpsi.data(), 2 * np, st);
     darea.upload(harea.data(), np, st);
+    stage_begin(1, st);
     launch_gather_square(np, dM.p, Mmax, cap, t1.p, t2.p, S.p, sq, st);
     const cd one = {1.0, 0.0}, zero = {0.0, 0.0};
     for (int p = 0; p < np; ++p) {
@@ -354,6 +366,7 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
     out.tabs.alloc((size_t)out.tab_stride * np);
     launch_phase_ghost(np, dM.p, dpsi.p, dmag.p, darea.p, F.p, sq, out.tabs.p, out.tab_stride,
                        Mmax, st);
+    stage_end(1, st);
     sync(ctx);   // temporaries die at return
 }
 
@@ -378,8 +391,10 @@ void compute_jk(pssfss_ctx *ctx, pssfss_sheet *sh, double ul, const std::vector<
     const int64_t stride = 6 * sh->nufp;
     for (int g = 0; g < nk; g += 8) {
         int nf = std::min(8, nk - g);
+        stage_begin(0, ctx->st);
         launch_jk(sh->dev(), sh->nufp, ul, us1, us2, dpsi.p + 2 * g, nf, dst + g * stride, stride,
                   mixed, ctx->st);
+        stage_end(0, ctx->st);
     }
     sync(ctx);
 }
@@ -520,7 +535,6 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     res.conv = tabs.converged;
     const int64_t nb2 = sh->nbf * sh->nbf;
     ctx->wZ.ensure((size_t)nb2 * np);
-    launch_zero(ctx->wZ.p, nb2 * np, st);
     std::vector<FillParams> fp(np);
     for (int p = 0; p < np; ++p) {
         fp[p] = make_fill_params(sh, &pts[p]);
@@ -532,7 +546,10 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     }
     DBuf<FillParams> dfp(np);
     dfp.upload(fp.data(), np, st);
+    stage_begin(2, st);
+    launch_zero(ctx->wZ.p, nb2 * np, st);
     launch_fill(sh->dev(), dfp.p, np, sh->rho_r.p, sh->rinv.p, sh->cls == 'M' ? 1 : 0, st);
+    stage_end(2, st);
     sync(ctx);
 }
 
@@ -545,7 +562,9 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wpiv.ensure((size_t)n * np);
     if (precision == PSSFSS_FP64) {
         lu_factor_batched(n, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, st);
+        stage_begin(5, st);
         lu_solve_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, ctx->wB.p, sB, st);
+        stage_end(5, st);
         sync(ctx);
         if (infos)
             for (int p = 0; p < np; ++p) { infos[p].refine_iters = 0; infos[p].refine_resid = 0; }
@@ -557,8 +576,11 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wR.ensure((size_t)sB * np);
     ctx->wRf.ensure((size_t)sB * np);
     ctx->wnorm.ensure((size_t)4 * np * ncol + 8);
+    stage_begin(3, st);
     convert_z_to_f32(ctx->wZ.p, ctx->wZf.p, nb2 * np, st);
+    stage_end(3, st);
     lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st);
+    stage_begin(5, st);
     double *rn = ctx->wnorm.p, *xn = ctx->wnorm.p + np * ncol;
     std::vector<double> hr(np * ncol), hx(np * ncol), hb(np * ncol);
     auto residual = [&](const cd *X) {
@@ -611,6 +633,7 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
         lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, ctx->wRf.p, sB, st);
         add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
     }
+    stage_end(5, st);
     // Points whose refinement did not reach an FP64-level residual: FP64 LU.
     std::vector<int> fb;
     for (int p = 0; p < np; ++p)
@@ -679,6 +702,7 @@ void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfs
     DBuf<BfftJob> djobs(njobs);
     djobs.upload(jobs.data(), njobs, st);
     ctx->wft.ensure((size_t)njobs * n * 2);
+    stage_begin(6, st);
     launch_bfft(sh->dev(), sh->bf_face.p, sh->bf_loc.p, djobs.p, njobs, ctx->wft.p, st);
     // Right-hand sides, eq. (7.39) / (7.64).
     std::vector<int32_t> coljob(np * ncol);
@@ -711,14 +735,17 @@ void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfs
     dcs.upload(colsrc.data(), colsrc.size(), st);
     ctx->wB.ensure((size_t)n * ncol * np);
     launch_rhs(n, np * ncol, ctx->wft.p, dcj.p, dcs.p, ctx->wB.p, n, st);
+    stage_end(6, st);
     solve_chunk(ctx, n, ncol, np, precision, infos);
     // Projection of the solutions onto the modes' transforms.
     ctx->wout.ensure((size_t)ncol * njobs * 2 * np);
     std::vector<cd> hout((size_t)np * ncol * njobs * 2);
     // one launch per point: columns of point p against all jobs
+    stage_begin(6, st);
     for (int p = 0; p < np; ++p)
         launch_project(n, ncol, njobs, ctx->wB.p + (int64_t)p * n * ncol, n, ctx->wft.p,
                        ctx->wout.p + (int64_t)p * ncol * njobs * 2, st);
+    stage_end(6, st);
     d2h(ctx, hout.data(), ctx->wout.p, hout.size());
     sync(ctx);
     // Scattered-field GSM entries, eqs. (6.18) / (6.37).
@@ -1088,6 +1115,24 @@ int pssfss_cascade_batch(pssfss_ctx *ctx, int nbatch, int n1, int n2, int n3, co
         cascade_batched_dev(nbatch, n1, n2, n3, da.p, db.p, dc.p, ctx->st);
         d2h(ctx, (cd *)c, dc.p, (size_t)lc * nbatch);
         sync(ctx);
+    });
+}
+
+int pssfss_stats_enable(pssfss_ctx *ctx, int on) {
+    return guarded(ctx, [&] { ctx->stats_on = on != 0; });
+}
+
+int pssfss_stats_read(pssfss_ctx *ctx, pssfss_stats *out, int reset) {
+    return guarded(ctx, [&] {
+        if (!out) throw ArgError("null output");
+        sync(ctx);
+        ctx->rec.flush();
+        for (int i = 0; i < NSTAGE; ++i) {
+            out->ms[i] = ctx->rec.ms[i];
+            out->flops[i] = ctx->rec.flops[i];
+            out->launches[i] = ctx->rec.launches[i];
+        }
+        if (reset) ctx->rec.reset();
     });
 }
 

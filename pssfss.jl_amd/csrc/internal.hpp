@@ -159,3 +159,75 @@ void launch_project(int64_t nbf, int ncol, int njobs, const cd *X, int64_t ldx, 
                     cd *out, hipStream_t st);
 void cascade_batched_dev(int nb, int n1, int n2, int n3, const cd *a, const cd *b, cd *c,
                          hipStream_t st);
+
+// ---- stage timing (HIP events on the launching stream)
+constexpr int NSTAGE = 8;
+struct StageRecorder {
+    struct Rec {
+        int stage;
+        hipEvent_t a, b;
+        double flops;
+    };
+    std::vector<Rec> open, pending;
+    std::vector<hipEvent_t> pool;
+    double ms[NSTAGE] = {0}, flops[NSTAGE] = {0};
+    int64_t launches[NSTAGE] = {0};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        return e;
+    }
+    void begin(int stage, hipStream_t st) {
+        Rec r{stage, get(), get(), 0.0};
+        HIPCHECK(hipEventRecord(r.a, st));
+        open.push_back(r);
+    }
+    void end(int stage, hipStream_t st, double fl) {
+        for (size_t i = open.size(); i-- > 0;) {
+            if (open[i].stage == stage) {
+                Rec r = open[i];
+                open.erase(open.begin() + i);
+                r.flops = fl;
+                HIPCHECK(hipEventRecord(r.b, st));
+                pending.push_back(r);
+                return;
+            }
+        }
+    }
+    void flush() {
+        for (Rec &r : pending) {
+            HIPCHECK(hipEventSynchronize(r.b));
+            float t = 0;
+            HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
+            ms[r.stage] += t;
+            flops[r.stage] += r.flops;
+            launches[r.stage] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        for (int i = 0; i < NSTAGE; ++i) { ms[i] = 0; flops[i] = 0; launches[i] = 0; }
+    }
+    ~StageRecorder() {
+        for (auto &r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        for (auto &r : open) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+extern thread_local StageRecorder *g_stage_rec;
+inline void stage_begin(int s, hipStream_t st) {
+    if (g_stage_rec) g_stage_rec->begin(s, st);
+}
+inline void stage_end(int s, hipStream_t st, double flops = 0.0) {
+    if (g_stage_rec) g_stage_rec->end(s, st, flops);
+}
+inline void stage_flops(int s, double flops) {
+    if (g_stage_rec) g_stage_rec->flops[s] += flops;
+}

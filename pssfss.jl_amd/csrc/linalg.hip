@@ -375,6 +375,7 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
     one.x = 1; one.y = 0;
     for (int64_t k = 0; k < n; k += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k);
+        stage_begin(3, st);
         hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(512), 0, st, n, k, kb, A, sA,
                            ipiv, sP);
         HIPCHECK(hipGetLastError());
@@ -389,15 +390,21 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
             dim3 g((unsigned)((rest + 63) / 64), (unsigned)batch);
             hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k, kb, A, sA);
             HIPCHECK(hipGetLastError());
+            stage_end(3, st);
             // Row-major A22 -= L21 U12, as a column-major GEMM on the transposes:
             // A22^T -= U12^T L21^T, every view having leading dimension n.
             C *u12 = A + k * n + (k + kb);
             C *l21t = A + (k + kb) * n + k;
             C *a22 = A + (k + kb) * n + (k + kb);
+            stage_begin(4, st);
             gemm_t<C>((int)rest, (int)rest, kb, mone, u12, n, sA, l21t, n, sA, one, a22, n, sA,
                       batch, st);
+            stage_end(4, st, 8.0 * (double)rest * (double)rest * kb * batch);
+        } else {
+            stage_end(3, st);
         }
     }
+    stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
 }
 
 template <class C>

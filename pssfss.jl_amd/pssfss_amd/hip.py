@@ -53,6 +53,14 @@ class Gsm(C.Structure):
     _fields_ = [("s11", _dp), ("s12", _dp), ("s21", _dp), ("s22", _dp)]
 
 
+class Stats(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("flops", C.c_double * 8), ("launches", C.c_int64 * 8)]
+
+
+STAGES = ("spatial_sums", "tables", "fill", "lu_panel", "lu_gemm", "solve", "transforms_rhs",
+          "lu_total_flops")
+
+
 class PointInfo(C.Structure):
     _fields_ = [("mmax", C.c_int32), ("table_converged", C.c_int32),
                 ("jk_rings_max", C.c_int32), ("refine_iters", C.c_int32),
@@ -90,11 +98,13 @@ def lib():
         L.pssfss_lu_solve_batch.argtypes = [vp, C.c_int64, C.c_int, C.c_int, _dp, _dp, C.c_int,
                                             C.POINTER(C.c_int), _dp]
         L.pssfss_cascade_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.pssfss_stats_enable.argtypes = [vp, C.c_int]
+        L.pssfss_stats_read.argtypes = [vp, C.POINTER(Stats), C.c_int]
         for name in ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_device_info",
                      "pssfss_sheet_upload", "pssfss_sheet_free", "pssfss_sheet_gsm",
                      "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk",
                      "pssfss_modal_tables", "pssfss_rwgbfft", "pssfss_lu_solve_batch",
-                     "pssfss_cascade_batch"):
+                     "pssfss_cascade_batch", "pssfss_stats_enable", "pssfss_stats_read"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -103,7 +113,8 @@ def lib():
 EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssfss_device_info",
            "pssfss_sheet_upload", "pssfss_sheet_free", "pssfss_sheet_gsm",
            "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk", "pssfss_modal_tables",
-           "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch")
+           "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
+           "pssfss_stats_enable", "pssfss_stats_read")
 
 
 def _d(a):
@@ -133,6 +144,15 @@ class Context:
         gib = C.c_double()
         self.check(lib().pssfss_device_info(self.h, name, 256, C.byref(cu), C.byref(gib)))
         return name.value.decode(), cu.value, gib.value
+
+    def stats_enable(self, on: bool = True):
+        self.check(lib().pssfss_stats_enable(self.h, 1 if on else 0))
+
+    def stats_read(self, reset: bool = True) -> dict:
+        st = Stats()
+        self.check(lib().pssfss_stats_read(self.h, C.byref(st), 1 if reset else 0))
+        return {name: {"ms": st.ms[i], "flops": st.flops[i], "launches": st.launches[i]}
+                for i, name in enumerate(STAGES)}
 
     def close(self):
         if self.h:

@@ -131,3 +131,4 @@ def test_struct_layout_matches_header(pk):
     assert ctypes.sizeof(hip.Point) == 184
     assert ctypes.sizeof(hip.SheetDesc) == 192
     assert ctypes.sizeof(hip.PointInfo) == 32
+    assert ctypes.sizeof(hip.Stats) == 192
