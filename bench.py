@@ -95,8 +95,8 @@ def main():
         stages[p] = ctx.stats_read(reset=True)
         results[p] = res
         timing[p] = times
-        iters = [i.refine_iters for i in info]
-        fb = sum(i.fallback_fp64 for i in info)
+        iters = [i.refine_iters for i in info["infos"]]
+        fb = sum(i.fallback_fp64 for i in info["infos"])
         if p == "mixed":
             print(f"[bench] mixed refinement sweeps min/max {min(iters)}/{max(iters)}, "
                   f"FP64 fallbacks {fb}", flush=True)

@@ -347,7 +347,8 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                     } else {
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
-                            double a2 = (double)m * psis[2 * f] + (double)n * psis[2 * f + 1];
+                            const int fi = f < nf ? f : 0;
+                            double a2 = (double)m * psis[2 * fi] + (double)n * psis[2 * fi + 1];
                             float sn, cs;
                             __sincosf((float)(-a2), &sn, &cs);
                             cf pf = cf{cs, sn};
@@ -371,7 +372,8 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                     } else {
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
-                            double a2 = (double)m * psis[2 * f] + (double)n * psis[2 * f + 1];
+                            const int fi = f < nf ? f : 0;
+                            double a2 = (double)m * psis[2 * fi] + (double)n * psis[2 * fi + 1];
                             cd term = cd{e * cos(-a2), e * sin(-a2)};
                             jr[f] += cd{term.x * ia, term.y * ia};
                             kr[f] += term;

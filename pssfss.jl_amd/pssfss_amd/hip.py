@@ -122,8 +122,8 @@ def _d(a):
 
 
 def _cplx(a) -> np.ndarray:
-    """complex array -> contiguous float64 (re, im) view holder."""
-    return np.ascontiguousarray(np.asarray(a, np.complex128)).view(np.float64)
+    """Private contiguous float64 (re, im) copy of a complex array."""
+    return np.array(a, dtype=np.complex128, order="C", copy=True).view(np.float64)
 
 
 class Context:
@@ -168,8 +168,10 @@ class Context:
             A, B = A[None], B[None]
         nb, n, _ = A.shape
         nrhs = B.shape[2] if B.ndim == 3 else 1
-        Af = np.ascontiguousarray(np.transpose(A, (0, 2, 1))).view(np.float64)  # col-major
-        Bf = np.ascontiguousarray(np.transpose(B.reshape(nb, n, nrhs), (0, 2, 1))).view(np.float64)
+        # private column-major copies (the library overwrites B with X)
+        Af = np.array(np.transpose(A, (0, 2, 1)), order="C", copy=True).view(np.float64)
+        Bf = np.array(np.transpose(B.reshape(nb, n, nrhs), (0, 2, 1)), order="C",
+                      copy=True).view(np.float64)
         it = (C.c_int * nb)()
         rs = np.zeros(nb)
         self.check(lib().pssfss_lu_solve_batch(self.h, n, nrhs, nb, _d(Af), _d(Bf),
@@ -317,8 +319,9 @@ class PointBuilder:
             for r, l in enumerate(ends):
                 nm = len(l.P)
                 p.nmodes[r] = nm
-                b = np.ascontiguousarray(l.beta[:nm], np.float64).reshape(-1)
-                t = np.ascontiguousarray(l.tvec[:nm], np.float64).reshape(-1)
+                # private copies: setup_modes overwrites the layer arrays per point
+                b = np.array(l.beta[:nm], dtype=np.float64, order="C", copy=True).reshape(-1)
+                t = np.array(l.tvec[:nm], dtype=np.float64, order="C", copy=True).reshape(-1)
                 c = _cplx(l.c[:nm])
                 y = _cplx(l.Y[:nm])
                 self.arrs += [b, t, c, y]
