@@ -84,7 +84,9 @@ struct pssfss_ctx {
     // reusable workspaces
     DBuf<cd> wZ, wB, wX, wR, wft, wout, wjk;
     DBuf<cf> wZf, wRf;
-    DBuf<int32_t> wpiv;
+    DBuf<int32_t> wpiv, wperm;
+    DBuf<cd> wT;
+    DBuf<cf> wTf;
     DBuf<double> wnorm;
     const cd *dft_matrix(int M) {
         auto it = dft.find(M);
@@ -389,8 +391,9 @@ void compute_jk(pssfss_ctx *ctx, pssfss_sheet *sh, double ul, const std::vector<
     DBuf<double> dpsi(psis.size());
     dpsi.upload(psis.data(), psis.size(), ctx->st);
     const int64_t stride = 6 * sh->nufp;
-    for (int g = 0; g < nk; g += 8) {
-        int nf = std::min(8, nk - g);
+    const int per = 1024;      // phasings per launch (grid y covers them in tiles)
+    for (int g = 0; g < nk; g += per) {
+        int nf = std::min(per, nk - g);
         stage_begin(0, ctx->st);
         launch_jk(sh->dev(), sh->nufp, ul, us1, us2, dpsi.p + 2 * g, nf, dst + g * stride, stride,
                   mixed, ctx->st);
@@ -560,10 +563,13 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     hipStream_t st = ctx->st;
     const int64_t nb2 = n * n, sB = n * ncol;
     ctx->wpiv.ensure((size_t)n * np);
+    ctx->wperm.ensure((size_t)n * np);
     if (precision == PSSFSS_FP64) {
+        ctx->wT.ensure((size_t)sB * np);
         lu_factor_batched(n, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, st);
         stage_begin(5, st);
-        lu_solve_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, ctx->wB.p, sB, st);
+        lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
+        lu_solve_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wperm.p, n, ctx->wB.p, sB, ctx->wT.p, st);
         stage_end(5, st);
         sync(ctx);
         if (infos)
@@ -581,6 +587,9 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     stage_end(3, st);
     lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st);
     stage_begin(5, st);
+    ctx->wTf.ensure((size_t)sB * np);
+    ctx->wT.ensure((size_t)sB * np);
+    lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
     double *rn = ctx->wnorm.p, *xn = ctx->wnorm.p + np * ncol;
     std::vector<double> hr(np * ncol), hx(np * ncol), hb(np * ncol);
     auto residual = [&](const cd *X) {
@@ -605,7 +614,8 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     residual(ctx->wX.p);
     hb = hr;
     convert_z_to_f32(ctx->wB.p, ctx->wRf.p, sB * np, st);
-    lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, ctx->wRf.p, sB, st);
+    lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wperm.p, n, ctx->wRf.p, sB,
+                         ctx->wTf.p, st);
     add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
     std::vector<double> prev(np, 1e300), best(np, 1e300);
     std::vector<int> iters(np, 0), done(np, 0), diverged(np, 0);
@@ -621,7 +631,7 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
             }
             if (!done[p]) {
                 iters[p] = it + 1;
-                if (r <= 1e-15 || r >= 0.5 * prev[p]) done[p] = 1;
+                if (r <= 1e-13 || r >= 0.5 * prev[p]) done[p] = 1;
                 if (r > 2 * best[p] && it > 0) diverged[p] = 1;
                 best[p] = std::min(best[p], r);
                 prev[p] = r;
@@ -630,7 +640,8 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
         }
         if (all) break;
         residual_to_f32(ctx->wR.p, ctx->wRf.p, sB * np, st);
-        lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, ctx->wRf.p, sB, st);
+        lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wperm.p, n, ctx->wRf.p, sB,
+                         ctx->wTf.p, st);
         add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
     }
     stage_end(5, st);
@@ -640,10 +651,11 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
         if (diverged[p] || best[p] > 1e-10) fb.push_back(p);
     for (int p : fb) {
         lu_factor_batched(n, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n, st);
+        lu_make_perm(n, 1, ctx->wpiv.p + p * n, n, ctx->wperm.p + p * n, st);
         HIPCHECK(hipMemcpyAsync(ctx->wX.p + p * sB, ctx->wB.p + p * sB, sB * sizeof(cd),
                                 hipMemcpyDeviceToDevice, st));
-        lu_solve_batched(n, ncol, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n,
-                         ctx->wX.p + p * sB, sB, st);
+        lu_solve_batched(n, ncol, 1, ctx->wZ.p + p * nb2, nb2, ctx->wperm.p + p * n, n,
+                         ctx->wX.p + p * sB, sB, ctx->wT.p + p * sB, st);
     }
     HIPCHECK(hipMemcpyAsync(ctx->wB.p, ctx->wX.p, (size_t)sB * np * sizeof(cd),
                             hipMemcpyDeviceToDevice, st));
@@ -799,7 +811,7 @@ int chunk_size(pssfss_ctx *ctx, const pssfss_sheet *sh, int npts, int precision)
     double budget = 0.45 * (double)freeb;
     int c = (int)std::max(1.0, std::floor(budget / per));
     (void)ctx;
-    return std::min(std::min(c, 32), npts);
+    return std::min(std::min(c, 128), npts);
 }
 
 void upload_sheet(pssfss_ctx *ctx, const pssfss_sheet_desc *d, pssfss_sheet *sh) {
@@ -1026,12 +1038,22 @@ int pssfss_filljk(pssfss_ctx *ctx, pssfss_sheet *sheet, double u_local, double p
         ensure_singular(ctx, sheet, u_local);
         std::vector<cd> h((size_t)6 * nu);
         d2h(ctx, h.data(), buf.p, h.size());
-        d2h(ctx, rho_r, sheet->rho_r.p, 2 * nu);
-        d2h(ctx, rinv, sheet->rinv.p, nu);
+        sync(ctx);
+        // device order -> reference class order (ufp2fp: ifm + nface*ifs)
+        const int64_t nf = sheet->nface;
+        std::vector<double> rr(2 * nu), ri(nu);
+        d2h(ctx, rr.data(), sheet->rho_r.p, 2 * nu);
+        d2h(ctx, ri.data(), sheet->rinv.p, nu);
         sync(ctx);
         cd *o = (cd *)jk;
-        for (int64_t i = 0; i < nu; ++i)
-            for (int q = 0; q < 6; ++q) o[6 * i + q] = h[q * nu + i];
+        for (int64_t i = 0; i < nu; ++i) {
+            int64_t j = i;
+            if (sheet->identity) j = (i / nf) + nf * (i % nf);
+            for (int q = 0; q < 6; ++q) o[6 * j + q] = h[q * nu + i];
+            rho_r[2 * j] = rr[2 * i];
+            rho_r[2 * j + 1] = rr[2 * i + 1];
+            rinv[j] = ri[i];
+        }
     });
 }
 

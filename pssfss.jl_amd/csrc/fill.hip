@@ -71,8 +71,8 @@ __global__ __launch_bounds__(256) void k_fill(SheetDev sh, const FillParams *fps
     if (iu >= sh.nufp) return;
     const FillParams &P = fps[blockIdx.y];
     const double EPS0 = 1.0 / (MU0 * (C0 * C0));
-    const int64_t fp0 = sh.identity ? iu : sh.umem[sh.uoff[iu]];
-    const int64_t ifm = fp0 % sh.nface, ifs = fp0 / sh.nface;
+    int64_t ifm, ifs;
+    class_faces(sh, iu, ifm, ifs);
     double rs[3][2], rm[3][2];
     {
         const double *a = sh.fxm + 6 * ifs, *b = sh.fxm + 6 * ifm;
@@ -130,8 +130,12 @@ __global__ __launch_bounds__(256) void k_fill(SheetDev sh, const FillParams *fps
     const int64_t nmem = sh.identity ? 1 : sh.uoff[iu + 1] - sh.uoff[iu];
     const int64_t nbf = sh.nbf;
     for (int64_t im_ = 0; im_ < nmem; ++im_) {
-        const int64_t fp = sh.identity ? iu : sh.umem[sh.uoff[iu] + im_];
-        const int64_t fm = fp % sh.nface, fs = fp / sh.nface;
+        int64_t fm = ifm, fs = ifs;
+        if (!sh.identity) {
+            const int64_t fp = sh.umem[sh.uoff[iu] + im_];
+            fm = fp % sh.nface;
+            fs = fp / sh.nface;
+        }
         for (int isl = 0; isl < 3; ++isl) {
             const int sbf = sh.fbf[3 * fs + isl];
             if (sbf < 0) continue;

@@ -62,6 +62,7 @@ struct DBuf {
 constexpr int MAXL = 16;        // layers per Gblock region
 constexpr int JK_RT = 10;       // rings whose phases are tabulated in LDS
 constexpr int JK_RINGMAX = 65;  // src/PGF.jl:15
+constexpr int JK_TILED_MIN = 24; // phasings from which launch_jk uses the tiled kernel
 
 // Geometry of a sheet on the device (structure of arrays, 0-based indices).
 struct SheetDev {
@@ -77,6 +78,22 @@ struct SheetDev {
     const int64_t *uoff;        // CSR offsets (nufp+1) or null
     const int64_t *umem;        // CSR members (0-based face pairs)
 };
+
+// Faces (match ifm, source ifs) of a face-pair class.  With every face pair its
+// own class, classes are numbered row-major (ifm * nface + ifs) so that a
+// wavefront's classes share the match face and scatter into one row of Z; with
+// translation classes the representative is the first member (ufp2fp order).
+__device__ __forceinline__ void class_faces(const SheetDev &sh, int64_t iu, int64_t &ifm,
+                                            int64_t &ifs) {
+    if (sh.identity) {
+        ifm = iu / sh.nface;
+        ifs = iu - ifm * sh.nface;
+    } else {
+        const int64_t fp = sh.umem[sh.uoff[iu]];
+        ifm = fp % sh.nface;
+        ifs = fp / sh.nface;
+    }
+}
 
 // Per-point spectral-table parameters.
 struct TabParams {
@@ -132,12 +149,17 @@ void zgemm_batched(int m, int n, int k, cd alpha, const cd *A, int64_t lda, int6
                    int64_t sC, int batch, hipStream_t st);
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
                        hipStream_t st);
+// perm from lu_make_perm; scratch holds batch * sB entries.
 void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
-                      const int32_t *ipiv, int64_t sP, cd *B, int64_t sB, hipStream_t st);
+                      const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
+                      hipStream_t st);
+void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t *perm,
+                  hipStream_t st);
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv,
                            int64_t sP, hipStream_t st);
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
-                          const int32_t *ipiv, int64_t sP, cf *B, int64_t sB, hipStream_t st);
+                          const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
+                          hipStream_t st);
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st);
 // r = b - A x (FP64), batched; then per-(system, rhs) norms of r and of x.
 void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,

@@ -81,26 +81,27 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int m, int n, int k, C alpha,
 // ------------------------------------------------------------------- LU panel
 // Factor columns [k, k+kb) of rows [k, n) of a row-major matrix with partial
 // pivoting; swaps are applied inside the panel only (k_rowswap does the rest).
+// The pivot search of column j+1 is fused into the rank-1 update pass of column j.
 template <class C>
-__global__ __launch_bounds__(512) void k_panel(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
-                                               int32_t *ipiv0, int64_t sP) {
+__global__ __launch_bounds__(1024) void k_panel(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
+                                                int32_t *ipiv0, int64_t sP) {
     typedef typename CTraits<C>::R R;
     C *A = A0 + (int64_t)blockIdx.x * sA;
     int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
     const int64_t lda = n;
     __shared__ C prow[NB];
-    __shared__ R rv[8];
-    __shared__ int ri[8];
+    __shared__ R rv[16];
+    __shared__ int ri[16];
     __shared__ int pivs;
     const int tid = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+    R best = (R)-1;
+    int bi = 0x7fffffff;
+    for (int64_t i = k + tid; i < n; i += nt) {
+        R v = cabs1(A[i * lda + k]);
+        if (v > best) { best = v; bi = (int)i; }
+    }
     for (int j = 0; j < kb; ++j) {
         const int64_t col = k + j;
-        R best = (R)-1;
-        int bi = 0x7fffffff;
-        for (int64_t i = col + tid; i < n; i += nt) {
-            R v = cabs1(A[i * lda + col]);
-            if (v > best) { best = v; bi = (int)i; }
-        }
         for (int off = 32; off > 0; off >>= 1) {
             R ov = __shfl_xor(best, off, 64);
             int oi = __shfl_xor(bi, off, 64);
@@ -131,12 +132,19 @@ __global__ __launch_bounds__(512) void k_panel(int64_t n, int64_t k, int kb, C *
         const C piv = prow[j];
         const bool nz = !(piv.x == 0 && piv.y == 0);
         const C rp = nz ? CTraits<C>::one() / piv : CTraits<C>::zero();
+        best = (R)-1;
+        bi = 0x7fffffff;
+        const bool next = j + 1 < kb;
         for (int64_t i = col + 1 + tid; i < n; i += nt) {
             C *row = A + i * lda + k;
             C l = nz ? row[j] * rp : row[j];
             row[j] = l;
             C ml = -l;
             for (int c = j + 1; c < kb; ++c) cfma(row[c], ml, prow[c]);
+            if (next) {
+                R v = cabs1(row[j + 1]);
+                if (v > best) { best = v; bi = (int)i; }
+            }
         }
         __syncthreads();
     }
@@ -189,106 +197,117 @@ __global__ __launch_bounds__(64) void k_trsm_u12(int64_t n, int64_t k, int kb, C
     }
 }
 
-// Forward (unit lower) and backward (upper) substitution for nrhs columns of a
-// column-major B, one workgroup per system, after the row interchanges.
-template <class C, int NR>
-__global__ __launch_bounds__(256) void k_trsv(int64_t n, int nrhs, const C *LU0, int64_t sA,
-                                              const int32_t *ipiv0, int64_t sP, C *B0,
-                                              int64_t sB) {
-    const C *LU = LU0 + (int64_t)blockIdx.x * sA;
+// Row permutation of the factorisation: perm[i] = original row now at row i,
+// from LAPACK's sequential interchanges (one workgroup per system, LDS).
+__global__ __launch_bounds__(256) void k_make_perm(int64_t n, const int32_t *ipiv0, int64_t sP,
+                                                   int32_t *perm0) {
+    extern __shared__ int32_t pm[];
     const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
-    C *B = B0 + (int64_t)blockIdx.x * sB;
-    __shared__ C D[NB][NB + 1];
-    __shared__ C Y[NR][NB];
-    const int tid = threadIdx.x;
-    // B <- P B
-    if (tid < nrhs) {
+    int32_t *perm = perm0 + (int64_t)blockIdx.x * sP;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) pm[i] = (int32_t)i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
         for (int64_t j = 0; j < n; ++j) {
-            int64_t p = ipiv[j];
-            if (p != j) {
-                C t = B[j + n * tid];
-                B[j + n * tid] = B[p + n * tid];
-                B[p + n * tid] = t;
-            }
+            int32_t p = ipiv[j];
+            if (p != j) { int32_t t = pm[j]; pm[j] = pm[p]; pm[p] = t; }
         }
     }
     __syncthreads();
-    // L y = b
-    for (int64_t b0 = 0; b0 < n; b0 += NB) {
-        const int bs = (int)min((int64_t)NB, n - b0);
-        for (int t = tid; t < bs * bs; t += blockDim.x) {
-            int r = t / bs, l = t % bs;
-            D[r][l] = LU[(b0 + r) * n + b0 + l];
-        }
-        for (int t = tid; t < NR * bs; t += blockDim.x) {
-            int c = t / bs, r = t % bs;
-            Y[c][r] = c < nrhs ? B[b0 + r + n * c] : CTraits<C>::zero();
-        }
-        __syncthreads();
-        if (tid < nrhs) {
-            for (int r = 0; r < bs; ++r) {
-                C v = Y[tid][r];
-                for (int l = 0; l < r; ++l) cfma(v, -D[r][l], Y[tid][l]);
-                Y[tid][r] = v;
-                B[b0 + r + n * tid] = v;
-            }
-        }
-        __syncthreads();
-        for (int64_t i = b0 + bs + tid; i < n; i += blockDim.x) {
-            C acc[NR];
-#pragma unroll
-            for (int c = 0; c < NR; ++c) acc[c] = CTraits<C>::zero();
-            const C *row = LU + i * n + b0;
-            for (int l = 0; l < bs; ++l) {
-                C a = row[l];
-#pragma unroll
-                for (int c = 0; c < NR; ++c) cfma(acc[c], a, Y[c][l]);
-            }
-#pragma unroll
-            for (int c = 0; c < NR; ++c)
-                if (c < nrhs) B[i + n * c] -= acc[c];
-        }
-        __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) perm[i] = pm[i];
+}
+
+template <class C>
+__device__ __forceinline__ C shfl_c(C v, int src) {
+    C r;
+    r.x = __shfl(v.x, src, 64);
+    r.y = __shfl(v.y, src, 64);
+    return r;
+}
+
+template <class C>
+__device__ __forceinline__ C wave_sum(C v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
     }
-    // U x = y
+    return v;
+}
+
+// P B, then L y = b (unit lower) and U x = y for nrhs <= NR columns of a
+// column-major B, one workgroup per system.  Diagonal 64x64 blocks are solved
+// one wavefront per right-hand side; the off-diagonal updates take one wavefront
+// per matrix row, its lanes reading the row's 64 block entries at once.
+template <class C, int NR>
+__global__ __launch_bounds__(1024) void k_trsv(int64_t n, int nrhs, const C *LU0, int64_t sA,
+                                               const int32_t *perm0, int64_t sP, C *B0,
+                                               int64_t sB, C *T0) {
+    const C *LU = LU0 + (int64_t)blockIdx.x * sA;
+    const int32_t *perm = perm0 + (int64_t)blockIdx.x * sP;
+    C *B = B0 + (int64_t)blockIdx.x * sB;
+    C *T = T0 + (int64_t)blockIdx.x * sB;
+    __shared__ C D[NB][NB + 1];
+    __shared__ C Y[NR][NB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    const C zero = CTraits<C>::zero();
+    for (int64_t t = tid; t < n * nrhs; t += blockDim.x) {
+        int64_t i = t % n, c = t / n;
+        T[i + n * c] = B[perm[i] + n * c];
+    }
+    __syncthreads();
+    for (int64_t t = tid; t < n * nrhs; t += blockDim.x) B[t] = T[t];
+    __syncthreads();
     const int64_t nblk = (n + NB - 1) / NB;
-    for (int64_t bb = nblk - 1; bb >= 0; --bb) {
-        const int64_t b0 = bb * NB;
-        const int bs = (int)min((int64_t)NB, n - b0);
-        for (int t = tid; t < bs * bs; t += blockDim.x) {
-            int r = t / bs, l = t % bs;
-            D[r][l] = LU[(b0 + r) * n + b0 + l];
-        }
-        for (int t = tid; t < NR * bs; t += blockDim.x) {
-            int c = t / bs, r = t % bs;
-            Y[c][r] = c < nrhs ? B[b0 + r + n * c] : CTraits<C>::zero();
-        }
-        __syncthreads();
-        if (tid < nrhs) {
-            for (int r = bs - 1; r >= 0; --r) {
-                C v = Y[tid][r];
-                for (int l = r + 1; l < bs; ++l) cfma(v, -D[r][l], Y[tid][l]);
-                v = v / D[r][r];
-                Y[tid][r] = v;
-                B[b0 + r + n * tid] = v;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool fwd = pass == 0;
+        for (int64_t q = 0; q < nblk; ++q) {
+            const int64_t bb = fwd ? q : nblk - 1 - q;
+            const int64_t b0 = bb * NB;
+            const int bs = (int)min((int64_t)NB, n - b0);
+            for (int t = tid; t < bs * bs; t += blockDim.x) {
+                int r = t / bs, l = t % bs;
+                D[r][l] = LU[(b0 + r) * n + b0 + l];
             }
-        }
-        __syncthreads();
-        for (int64_t i = tid; i < b0; i += blockDim.x) {
-            C acc[NR];
-#pragma unroll
-            for (int c = 0; c < NR; ++c) acc[c] = CTraits<C>::zero();
-            const C *row = LU + i * n + b0;
-            for (int l = 0; l < bs; ++l) {
-                C a = row[l];
-#pragma unroll
-                for (int c = 0; c < NR; ++c) cfma(acc[c], a, Y[c][l]);
+            for (int t = tid; t < NR * NB; t += blockDim.x) {
+                int c = t / NB, r = t % NB;
+                Y[c][r] = (c < nrhs && r < bs) ? B[b0 + r + n * c] : zero;
             }
+            __syncthreads();
+            if (wave < nrhs) {
+                C y = Y[wave][lane];
+                if (fwd) {
+                    for (int r = 0; r < bs; ++r) {
+                        C xr = shfl_c(y, r);
+                        if (lane > r && lane < bs) cfma(y, -D[lane][r], xr);
+                    }
+                } else {
+                    for (int r = bs - 1; r >= 0; --r) {
+                        if (lane == r) y = y / D[r][r];
+                        C xr = shfl_c(y, r);
+                        if (lane < r) cfma(y, -D[lane][r], xr);
+                    }
+                }
+                Y[wave][lane] = y;
+                if (lane < bs) B[b0 + lane + n * wave] = y;
+            }
+            __syncthreads();
+            const int64_t r0 = fwd ? b0 + bs : 0, r1 = fwd ? n : b0;
+            for (int64_t i = r0 + wave; i < r1; i += nw) {
+                C a = lane < bs ? LU[i * n + b0 + lane] : zero;
+                C acc[NR];
 #pragma unroll
-            for (int c = 0; c < NR; ++c)
-                if (c < nrhs) B[i + n * c] -= acc[c];
+                for (int c = 0; c < NR; ++c) {
+                    acc[c] = zero;
+                    if (c < nrhs) cfma(acc[c], a, Y[c][lane]);
+                    acc[c] = wave_sum(acc[c]);
+                }
+                C mine = zero;
+#pragma unroll
+                for (int c = 0; c < NR; ++c)
+                    if (c == lane) mine = acc[c];
+                if (lane < nrhs) B[i + n * lane] -= mine;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -376,7 +395,7 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
     for (int64_t k = 0; k < n; k += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k);
         stage_begin(3, st);
-        hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(512), 0, st, n, k, kb, A, sA,
+        hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(1024), 0, st, n, k, kb, A, sA,
                            ipiv, sP);
         HIPCHECK(hipGetLastError());
         const int64_t other = n - kb;
@@ -408,17 +427,18 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
 }
 
 template <class C>
-void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *ipiv,
-                int64_t sP, C *B, int64_t sB, hipStream_t st) {
+void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *perm,
+                int64_t sP, C *B, int64_t sB, C *scratch, hipStream_t st) {
     for (int c0 = 0; c0 < nrhs; c0 += 8) {
         int nr = std::min(8, nrhs - c0);
         C *Bc = B + (int64_t)c0 * n;
+        C *Tc = scratch + (int64_t)c0 * n;
         if (nr <= 4)
-            hipLaunchKernelGGL((k_trsv<C, 4>), dim3((unsigned)batch), dim3(256), 0, st, n, nr, LU,
-                               sA, ipiv, sP, Bc, sB);
+            hipLaunchKernelGGL((k_trsv<C, 4>), dim3((unsigned)batch), dim3(1024), 0, st, n, nr,
+                               LU, sA, perm, sP, Bc, sB, Tc);
         else
-            hipLaunchKernelGGL((k_trsv<C, 8>), dim3((unsigned)batch), dim3(256), 0, st, n, nr, LU,
-                               sA, ipiv, sP, Bc, sB);
+            hipLaunchKernelGGL((k_trsv<C, 8>), dim3((unsigned)batch), dim3(1024), 0, st, n, nr,
+                               LU, sA, perm, sP, Bc, sB, Tc);
         HIPCHECK(hipGetLastError());
     }
 }
@@ -437,8 +457,17 @@ void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, i
 }
 
 void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
-                      const int32_t *ipiv, int64_t sP, cd *B, int64_t sB, hipStream_t st) {
-    lu_solve_t<cd>(n, nrhs, batch, LU, sA, ipiv, sP, B, sB, st);
+                      const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
+                      hipStream_t st) {
+    lu_solve_t<cd>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st);
+}
+
+void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t *perm,
+                  hipStream_t st) {
+    size_t lds = (size_t)n * sizeof(int32_t);
+    if (lds > 160 * 1024) throw ArgError("lu_make_perm: n too large for the LDS permutation");
+    hipLaunchKernelGGL(k_make_perm, dim3((unsigned)batch), dim3(256), lds, st, n, ipiv, sP, perm);
+    HIPCHECK(hipGetLastError());
 }
 
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv, int64_t sP,
@@ -447,8 +476,9 @@ void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipi
 }
 
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
-                          const int32_t *ipiv, int64_t sP, cf *B, int64_t sB, hipStream_t st) {
-    lu_solve_t<cf>(n, nrhs, batch, LU, sA, ipiv, sP, B, sB, st);
+                          const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
+                          hipStream_t st) {
+    lu_solve_t<cf>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st);
 }
 
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {

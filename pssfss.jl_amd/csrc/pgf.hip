@@ -5,6 +5,8 @@
 //   * spatial lattice sums integrated over the source triangle for several
 //     phasings at once (filljk! / jksums) -- src/Zint.jl:54-125, src/PGF.jl:67-106
 //   * analytic singular integrals          -- src/Zint.jl:126-165
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace {
@@ -254,25 +256,35 @@ __global__ void k_phase_ghost(const int *Ms, const double *psis, const int *mags
 
 // ---------------------------------------------------------------- spatial sums
 // One lane per (face-pair class, quadrature point); 9 classes x 7 points per
-// wavefront.  Every lane sums the lattice rings for F phasings at once: the
-// frequency-independent distance, exp and 1/distance are shared by the F
-// phasings, whose lattice phases for |m|,|n| <= JK_RT come from LDS.
+// wavefront.  Every lane sums the lattice rings of jksums (src/PGF.jl:67-106) for
+// F phasings at once: the frequency-independent distance, exp(-d) and 1/d are
+// shared by the F phasings, whose lattice phases for |m|,|n| <= JK_RT come from
+// LDS.  Each phasing keeps its own ring count (break test at 1e-8 after ring 5).
+// MIXED: ring terms (r >= 1) and ring partial sums in FP32, the extracted (0,0)
+// term and the running sums in FP64.
 template <int F, bool MIXED>
 __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul, double us1x,
                                             double us1y, double us2x, double us2y,
                                             const double *psis, int nf, cd *jk,
-                                            int64_t jk_stride, int *ringmax) {
+                                            int64_t jk_stride) {
 #pragma clang fp contract(off)
     constexpr int RT = JK_RT, W = 2 * JK_RT + 1;
-    __shared__ cd ph[F][W * W];
+    typedef typename std::conditional<MIXED, cf, cd>::type PT;
+    __shared__ PT ph[F][W * W];
+    __shared__ double spsi[2 * F];
+    for (int t = threadIdx.x; t < 2 * F; t += blockDim.x) spsi[t] = t < 2 * nf ? psis[t] : 0.0;
     for (int t = threadIdx.x; t < F * W * W; t += blockDim.x) {
         int f = t / (W * W), e = t % (W * W);
         int m = e / W - RT, n = e % W - RT;
-        cd v = {0.0, 0.0};
+        double c = 1.0, sn = 0.0;
         if (f < nf) {
             double a = (double)m * psis[2 * f] + (double)n * psis[2 * f + 1];
-            v = cd{cos(-a), sin(-a)};
+            c = cos(-a);
+            sn = sin(-a);
         }
+        PT v;
+        v.x = c;
+        v.y = sn;
         ph[f][e] = v;
     }
     __syncthreads();
@@ -283,8 +295,8 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
     const bool active = lane < 63 && iu < nufp;
     double urx = 0.0, ury = 0.0;
     if (active) {
-        int64_t fp = sh.identity ? iu : sh.umem[sh.uoff[iu]];
-        int64_t ifm = fp % sh.nface, ifs = fp / sh.nface;
+        int64_t ifm, ifs;
+        class_faces(sh, iu, ifm, ifs);
         const double *rs = sh.fxs + 6 * ifs;
         const double *rm = sh.fxs + 6 * ifm;
         double rs12x = rs[0] - rs[2], rs12y = rs[1] - rs[3];
@@ -298,8 +310,11 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
         ury = ul * (rmcy - rty);
     }
     cd js[F], ks[F];
-    unsigned done = 0;
-    int rsave = 0;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        js[f] = cd{0.0, 0.0};
+        ks[f] = cd{0.0, 0.0};
+    }
     if (active) {
         double arg = sqrt(urx * urx + ury * ury);
         double rterm = exp(-arg);
@@ -311,18 +326,17 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
             js[f] = cd{j0, 0.0};
             ks[f] = cd{rterm, 0.0};
         }
-        for (int f = nf; f < F; ++f) done |= 1u << f;
-        const unsigned all = (1u << F) - 1u;
+        unsigned done = 0;
+#pragma unroll
+        for (int f = 0; f < F; ++f)
+            if (f >= nf) done |= 1u << f;
+        const unsigned all = (F == 32) ? 0xffffffffu : ((1u << F) - 1u);
         for (int r = 1; r <= JK_RINGMAX && done != all; ++r) {
-            rsave = r;
-            cd jr[F], kr[F];
-            cf jrf[F], krf[F];
+            PT jr[F], kr[F];
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                jr[f] = cd{0.0, 0.0};
-                kr[f] = cd{0.0, 0.0};
-                jrf[f] = cf{0.0f, 0.0f};
-                krf[f] = cf{0.0f, 0.0f};
+                jr[f].x = 0; jr[f].y = 0;
+                kr[f].x = 0; kr[f].y = 0;
             }
             const int len = 8 * r;
             for (int k = 0; k < len; ++k) {
@@ -339,21 +353,22 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                         const int off = (m + RT) * W + (n + RT);
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
-                            cd p = ph[f][off];
-                            cf pf = cf{(float)p.x, (float)p.y};
-                            jrf[f] = jrf[f] + ea * pf;
-                            krf[f] = krf[f] + e * pf;
+                            const PT p = ph[f][off];
+                            jr[f].x = fmaf(ea, p.x, jr[f].x);
+                            jr[f].y = fmaf(ea, p.y, jr[f].y);
+                            kr[f].x = fmaf(e, p.x, kr[f].x);
+                            kr[f].y = fmaf(e, p.y, kr[f].y);
                         }
                     } else {
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
-                            const int fi = f < nf ? f : 0;
-                            double a2 = (double)m * psis[2 * fi] + (double)n * psis[2 * fi + 1];
+                            double a2 = (double)m * spsi[2 * f] + (double)n * spsi[2 * f + 1];
                             float sn, cs;
                             __sincosf((float)(-a2), &sn, &cs);
-                            cf pf = cf{cs, sn};
-                            jrf[f] = jrf[f] + ea * pf;
-                            krf[f] = krf[f] + e * pf;
+                            jr[f].x = fmaf(ea, cs, jr[f].x);
+                            jr[f].y = fmaf(ea, sn, jr[f].y);
+                            kr[f].x = fmaf(e, cs, kr[f].x);
+                            kr[f].y = fmaf(e, sn, kr[f].y);
                         }
                     }
                 } else {
@@ -364,75 +379,282 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                         const int off = (m + RT) * W + (n + RT);
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
-                            cd p = ph[f][off];
-                            cd term = cd{e * p.x, e * p.y};
-                            jr[f] += cd{term.x * ia, term.y * ia};
-                            kr[f] += term;
+                            const PT p = ph[f][off];
+                            double tx = e * p.x, ty = e * p.y;
+                            jr[f].x += tx * ia;
+                            jr[f].y += ty * ia;
+                            kr[f].x += tx;
+                            kr[f].y += ty;
                         }
                     } else {
 #pragma unroll
                         for (int f = 0; f < F; ++f) {
-                            const int fi = f < nf ? f : 0;
-                            double a2 = (double)m * psis[2 * fi] + (double)n * psis[2 * fi + 1];
-                            cd term = cd{e * cos(-a2), e * sin(-a2)};
-                            jr[f] += cd{term.x * ia, term.y * ia};
-                            kr[f] += term;
+                            double a2 = (double)m * spsi[2 * f] + (double)n * spsi[2 * f + 1];
+                            double tx = e * cos(-a2), ty = e * sin(-a2);
+                            jr[f].x += tx * ia;
+                            jr[f].y += ty * ia;
+                            kr[f].x += tx;
+                            kr[f].y += ty;
                         }
                     }
                 }
             }
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                if (done & (1u << f)) continue;
-                cd a = MIXED ? cd{(double)jrf[f].x, (double)jrf[f].y} : jr[f];
-                cd b = MIXED ? cd{(double)krf[f].x, (double)krf[f].y} : kr[f];
-                js[f] += a;
-                ks[f] += b;
-                if (r > 5) {
-                    double c1 = cabs(a / js[f]), c2 = cabs(b / ks[f]);
-                    double conv = c1 > c2 ? c1 : c2;
-                    if (conv < 1e-8) done |= 1u << f;
+                if (!(done & (1u << f))) {
+                    cd a = cd{(double)jr[f].x, (double)jr[f].y};
+                    cd b = cd{(double)kr[f].x, (double)kr[f].y};
+                    js[f] += a;
+                    ks[f] += b;
+                    if (r > 5) {
+                        double c1 = cabs(a / js[f]), c2 = cabs(b / ks[f]);
+                        double conv = c1 > c2 ? c1 : c2;
+                        if (conv < 1e-8) done |= 1u << f;
+                    }
                 }
             }
         }
     }
     // Combine the 7 quadrature points of each class in order (src/Zint.jl:117-124).
     const int base_lane = (lane / 7) * 7;
-    for (int f = 0; f < F; ++f) {
-        if (f >= nf) break;
-        double jx[7], jy[7], kx[7], ky[7];
 #pragma unroll
-        for (int q = 0; q < 7; ++q) {
-            int src = base_lane + q;
-            if (src > 63) src = 63;
-            jx[q] = __shfl(js[f].x, src, 64);
-            jy[q] = __shfl(js[f].y, src, 64);
-            kx[q] = __shfl(ks[f].x, src, 64);
-            ky[q] = __shfl(ks[f].y, src, 64);
-        }
-        if (active && qp == 0) {
+    for (int f = 0; f < F; ++f) {
+        if (f < nf) {
             cd J = {0, 0}, Jx = {0, 0}, Je = {0, 0}, K = {0, 0}, Kx = {0, 0}, Ke = {0, 0};
 #pragma unroll
             for (int q = 0; q < 7; ++q) {
-                cd jw = cd{jx[q], jy[q]} * QW[q];
+                int src = base_lane + q;
+                if (src > 63) src = 63;
+                cd jq = cd{__shfl(js[f].x, src, 64), __shfl(js[f].y, src, 64)};
+                cd kq = cd{__shfl(ks[f].x, src, 64), __shfl(ks[f].y, src, 64)};
+                cd jw = jq * QW[q];
                 Jx += jw * QXI[q];
                 Je += jw * QETA[q];
                 J += jw;
-                cd kw = cd{kx[q], ky[q]} * QW[q];
+                cd kw = kq * QW[q];
                 Kx += kw * QXI[q];
                 Ke += kw * QETA[q];
                 K += kw;
             }
-            cd *o = jk + f * jk_stride;
-            o[0 * nufp + iu] = J;
-            o[1 * nufp + iu] = Jx;
-            o[2 * nufp + iu] = Je;
-            o[3 * nufp + iu] = K;
-            o[4 * nufp + iu] = Kx;
-            o[5 * nufp + iu] = Ke;
+            if (active && qp == 0) {
+                cd *o = jk + f * jk_stride;
+                o[0 * nufp + iu] = J;
+                o[1 * nufp + iu] = Jx;
+                o[2 * nufp + iu] = Je;
+                o[3 * nufp + iu] = K;
+                o[4 * nufp + iu] = Kx;
+                o[5 * nufp + iu] = Ke;
+            }
         }
     }
-    if (active && ringmax) atomicMax(ringmax, rsave);
+}
+
+// Tiled form of the spatial sums for many phasings: a workgroup takes 64 rows
+// (9 classes x 7 quadrature points) and 16*TN phasings.  Lattice points are
+// visited ring by ring in chunks of 8 (ring r has 8r points), the lattice term
+// tile exp(-d)/d, exp(-d) (64 rows x 8 points) is computed once into LDS and
+// multiplies the phase tile (8 points x 16*TN phasings) read from a table, so
+// the distance/exp/reciprocal work is shared by every phasing of the tile.
+// Ring sums are folded and the break test of jksums applied per (row, phasing)
+// at each ring end, exactly as src/PGF.jl:86-103 does per call.
+// T = double: FP64 terms and ring sums; T = float: FP32 terms and ring sums
+// (FP64 running sums and extracted (0,0) term).
+template <class T, int TN>
+__global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, double ul,
+                                                  double us1x, double us1y, double us2x,
+                                                  double us2y, const T *phase, int ldp, int nf,
+                                                  cd *jk, int64_t jk_stride) {
+#pragma clang fp contract(off)
+    constexpr int FT = 16 * TN;              // phasings per workgroup
+    __shared__ T Aj[8][64], Ak[8][64];
+    __shared__ T Bre[8][FT], Bim[8][FT];
+    __shared__ double urxs[64], urys[64];
+    __shared__ cd Cs[64][FT][2];
+    const int tid = threadIdx.x, rg = tid >> 4, pg = tid & 15;
+    const int64_t u0 = (int64_t)blockIdx.x * 9;
+    const int f0 = blockIdx.y * FT;
+    if (tid < 64) {
+        const int r = tid;
+        const int64_t iu = u0 + r / 7;
+        double urx = 0.0, ury = 0.0;
+        if (r < 63 && iu < nufp) {
+            const int qp = r % 7;
+            int64_t ifm, ifs;
+            class_faces(sh, iu, ifm, ifs);
+            const double *rs = sh.fxs + 6 * ifs;
+            const double *rm = sh.fxs + 6 * ifm;
+            double rs12x = rs[0] - rs[2], rs12y = rs[1] - rs[3];
+            double rs31x = rs[4] - rs[0], rs31y = rs[5] - rs[1];
+            double rs21x = -rs12x, rs21y = -rs12y;
+            double rmcx = ((rm[0] + rm[2]) + rm[4]) / 3.0;
+            double rmcy = ((rm[1] + rm[3]) + rm[5]) / 3.0;
+            double rtx = (rs[0] + rs21x * QXI[qp]) + rs31x * QETA[qp];
+            double rty = (rs[1] + rs21y * QXI[qp]) + rs31y * QETA[qp];
+            urx = ul * (rmcx - rtx);
+            ury = ul * (rmcy - rty);
+        }
+        urxs[r] = urx;
+        urys[r] = ury;
+    }
+    __syncthreads();
+    // Rows of this thread: 4*rg .. 4*rg+3; phasings: f0 + pg + 16*j.
+    cd js[4][TN], ks[4][TN];
+    unsigned done = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = 4 * rg + i;
+        const bool live = r < 63 && u0 + r / 7 < nufp;
+        double arg = sqrt(urxs[r] * urxs[r] + urys[r] * urys[r]);
+        double rterm = exp(-arg);
+        double j0 = arg <= 1e-5 ? (-arg / 6 + 0.5) * arg - 1 : (rterm - 1) / arg;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            js[i][j] = cd{j0, 0.0};
+            ks[i][j] = cd{rterm, 0.0};
+            if (!live || f0 + pg + 16 * j >= nf) done |= 1u << (i * TN + j);
+        }
+    }
+    const unsigned all = (1u << (4 * TN)) - 1u;
+    for (int r = 1; r <= JK_RINGMAX; ++r) {
+        if (!__syncthreads_or(done != all)) break;
+        T jr[4][TN][2], kr[4][TN][2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                jr[i][j][0] = 0; jr[i][j][1] = 0;
+                kr[i][j][0] = 0; kr[i][j][1] = 0;
+            }
+        const int64_t ebase = ring_base(r);
+        for (int c = 0; c < r; ++c) {
+            // lattice term tile: 64 rows x 8 points
+            for (int t = tid; t < 512; t += 256) {
+                const int row = t & 63, kk = t >> 6;
+                int m, n;
+                ring_mn(r, 8 * c + kk, m, n);
+                double dx = urxs[row] - ((double)m * us1x + (double)n * us2x);
+                double dy = urys[row] - ((double)m * us1y + (double)n * us2y);
+                T a, e, ea;
+                if (sizeof(T) == 4) {
+                    float fx = (float)dx, fy = (float)dy;
+                    float af = __fsqrt_rn(fx * fx + fy * fy);
+                    float ef = __expf(-af);
+                    a = af;
+                    e = ef;
+                    ea = __fdividef(ef, af);
+                } else {
+                    a = sqrt(dx * dx + dy * dy);
+                    e = exp(-(double)a);
+                    ea = e * (1.0 / (double)a);
+                }
+                Aj[kk][row] = ea;
+                Ak[kk][row] = e;
+            }
+            // phase tile: 8 points x FT phasings
+            for (int t = tid; t < 8 * FT; t += 256) {
+                const int kk = t / FT, f = t % FT;
+                const T *pp = phase + (size_t)(ebase + 8 * c + kk) * (2 * ldp) + 2 * (f0 + f);
+                Bre[kk][f] = pp[0];
+                Bim[kk][f] = pp[1];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                T aj[4], ak[4], pr[TN], pi[TN];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    aj[i] = Aj[kk][4 * rg + i];
+                    ak[i] = Ak[kk][4 * rg + i];
+                }
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    pr[j] = Bre[kk][pg + 16 * j];
+                    pi[j] = Bim[kk][pg + 16 * j];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) {
+                        jr[i][j][0] += aj[i] * pr[j];
+                        jr[i][j][1] += aj[i] * pi[j];
+                        kr[i][j][0] += ak[i] * pr[j];
+                        kr[i][j][1] += ak[i] * pi[j];
+                    }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const unsigned bit = 1u << (i * TN + j);
+                if (!(done & bit)) {
+                    cd a = cd{(double)jr[i][j][0], (double)jr[i][j][1]};
+                    cd b = cd{(double)kr[i][j][0], (double)kr[i][j][1]};
+                    js[i][j] += a;
+                    ks[i][j] += b;
+                    if (r > 5) {
+                        double c1 = cabs(a / js[i][j]), c2 = cabs(b / ks[i][j]);
+                        if ((c1 > c2 ? c1 : c2) < 1e-8) done |= bit;
+                    }
+                }
+            }
+    }
+    // Combine the 7 quadrature points of each class (src/Zint.jl:117-124).
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            Cs[4 * rg + i][pg + 16 * j][0] = js[i][j];
+            Cs[4 * rg + i][pg + 16 * j][1] = ks[i][j];
+        }
+    __syncthreads();
+    for (int t = tid; t < 9 * FT; t += 256) {
+        const int u = t / FT, f = t % FT;
+        const int64_t iu = u0 + u;
+        if (iu >= nufp || f0 + f >= nf) continue;
+        cd J = {0, 0}, Jx = {0, 0}, Je = {0, 0}, K = {0, 0}, Kx = {0, 0}, Ke = {0, 0};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            cd jw = Cs[7 * u + q][f][0] * QW[q];
+            Jx += jw * QXI[q];
+            Je += jw * QETA[q];
+            J += jw;
+            cd kw = Cs[7 * u + q][f][1] * QW[q];
+            Kx += kw * QXI[q];
+            Ke += kw * QETA[q];
+            K += kw;
+        }
+        cd *o = jk + (int64_t)(f0 + f) * jk_stride;
+        o[0 * nufp + iu] = J;
+        o[1 * nufp + iu] = Jx;
+        o[2 * nufp + iu] = Je;
+        o[3 * nufp + iu] = K;
+        o[4 * nufp + iu] = Kx;
+        o[5 * nufp + iu] = Ke;
+    }
+}
+
+// Lattice phase table exp(-i (m psi1 + n psi2)) over rings 0..JK_RINGMAX in ring
+// order, one column per phasing (zero beyond nf), interleaved (re, im).
+template <class T>
+__global__ void k_phase_table(const double *psis, int nf, int ldp, T *phase) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
+    if (e >= E) return;
+    int r = ring_of(e);
+    int m, n;
+    ring_mn(r, (int)(e - ring_base(r)), m, n);
+    double c = 0.0, sn = 0.0;
+    if (f < nf) {
+        double a = (double)m * psis[2 * f] + (double)n * psis[2 * f + 1];
+        c = cos(-a);
+        sn = sin(-a);
+    }
+    T *p = phase + (size_t)e * (2 * ldp) + 2 * f;
+    p[0] = (T)c;
+    p[1] = (T)sn;
 }
 
 // Closed-form singular integrals of eqs. (B.3) over the source triangle,
@@ -441,8 +663,8 @@ __global__ void k_singular(SheetDev sh, int64_t nufp, double ul, double *rho_r, 
 #pragma clang fp contract(off)
     int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (iu >= nufp) return;
-    int64_t fp = sh.identity ? iu : sh.umem[sh.uoff[iu]];
-    int64_t ifm = fp % sh.nface, ifs = fp / sh.nface;
+    int64_t ifm, ifs;
+    class_faces(sh, iu, ifm, ifs);
     const double *rsp = sh.fxs + 6 * ifs, *rm = sh.fxs + 6 * ifm;
     double rs[3][2] = {{rsp[0], rsp[1]}, {rsp[2], rsp[3]}, {rsp[4], rsp[5]}};
     double rs32x = rs[2][0] - rs[1][0], rs32y = rs[2][1] - rs[1][1];
@@ -535,25 +757,60 @@ void launch_phase_ghost(int npts, const int *d_M, const double *d_psi, const int
     HIPCHECK(hipGetLastError());
 }
 
+// Few phasings: the per-lane kernel (phases in LDS, F per launch).  Many
+// phasings: the tiled kernel, whose lattice-term tile is shared by 32 phasings.
 void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us1,
                const double *us2, const double *d_psi, int nf, cd *jk, int64_t jk_stride,
                int mixed, hipStream_t st) {
-    if (nf < 1 || nf > 8) throw ArgError("launch_jk: 1..8 phasings per launch");
-    int64_t per_block = 36;
-    dim3 grid((unsigned)((nufp + per_block - 1) / per_block));
-    int *rm = nullptr;
+    if (nf < 1) return;
+    if (nf < JK_TILED_MIN) {
+        const int per = mixed ? 16 : 8;
+        dim3 grid((unsigned)((nufp + 35) / 36));
+        for (int g = 0; g < nf; g += per) {
+            const int n = nf - g < per ? nf - g : per;
+            const double *ps = d_psi + 2 * g;
+            cd *o = jk + (int64_t)g * jk_stride;
 #define JKL(FF, MX)                                                                        \
     hipLaunchKernelGGL((k_jk<FF, MX>), grid, dim3(256), 0, st, sh, nufp, ulocal, us1[0],  \
-                       us1[1], us2[0], us2[1], d_psi, nf, jk, jk_stride, rm)
-    if (mixed) {
-        if (nf <= 1) JKL(1, true); else if (nf <= 2) JKL(2, true);
-        else if (nf <= 4) JKL(4, true); else JKL(8, true);
-    } else {
-        if (nf <= 1) JKL(1, false); else if (nf <= 2) JKL(2, false);
-        else if (nf <= 4) JKL(4, false); else JKL(8, false);
-    }
+                       us1[1], us2[0], us2[1], ps, n, o, jk_stride)
+            if (mixed) {
+                if (n <= 1) JKL(1, true); else if (n <= 2) JKL(2, true);
+                else if (n <= 4) JKL(4, true); else if (n <= 8) JKL(8, true);
+                else JKL(16, true);
+            } else {
+                if (n <= 1) JKL(1, false); else if (n <= 2) JKL(2, false);
+                else if (n <= 4) JKL(4, false); else JKL(8, false);
+            }
 #undef JKL
+            HIPCHECK(hipGetLastError());
+        }
+        return;
+    }
+    constexpr int FT = 32;                         // phasings per workgroup (16 * TN)
+    const int ldp = (nf + FT - 1) / FT * FT;
+    const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
+    const size_t elt = mixed ? sizeof(float) : sizeof(double);
+    void *ph = nullptr;
+    if (hipMallocAsync(&ph, (size_t)E * 2 * ldp * elt, st) != hipSuccess) {
+        (void)hipGetLastError();
+        throw HipError("hipMallocAsync of the lattice phase table failed");
+    }
+    dim3 pg((unsigned)((E + 255) / 256), (unsigned)ldp);
+    dim3 grid((unsigned)((nufp + 8) / 9), (unsigned)(ldp / FT));
+    if (mixed) {
+        hipLaunchKernelGGL(k_phase_table<float>, pg, dim3(256), 0, st, d_psi, nf, ldp, (float *)ph);
+        hipLaunchKernelGGL((k_jk_tiled<float, 2>), grid, dim3(256), 0, st, sh, nufp, ulocal,
+                           us1[0], us1[1], us2[0], us2[1], (const float *)ph, ldp, nf, jk,
+                           jk_stride);
+    } else {
+        hipLaunchKernelGGL(k_phase_table<double>, pg, dim3(256), 0, st, d_psi, nf, ldp,
+                           (double *)ph);
+        hipLaunchKernelGGL((k_jk_tiled<double, 2>), grid, dim3(256), 0, st, sh, nufp, ulocal,
+                           us1[0], us1[1], us2[0], us2[1], (const double *)ph, ldp, nf, jk,
+                           jk_stride);
+    }
     HIPCHECK(hipGetLastError());
+    HIPCHECK(hipFreeAsync(ph, st));
 }
 
 void launch_singular(const SheetDev &sh, int64_t nufp, double ulocal, double *rho_r,

@@ -249,6 +249,23 @@ def test_analyze_ring_oblique(ctx, pk, orc):
     assert dm <= 1e-6
 
 
+def test_analyze_ring_many_phasings(ctx, pk, orc):
+    """30 distinct (psi1, psi2) keys in one batch: the spatial sums run through the
+    tiled many-phasing kernel, whose columns must land on the right points."""
+    _say("analyze ring on slab (nr=2, nphi=24), theta=30, phi=0/45, 15 freqs (30 phasings)")
+    strata, freqs, steer = pk.workloads.ring_on_slab(nr=2, nphi=24, nfreq=15)
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    assert len(res) == len(ref) == 30
+    d = _max_dS(res, ref)
+    print(f"  max |dS| fp64 = {d:.3e}")
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
 def _grating(kP, nterms=30):
     """Collin's strip-grating closed form as used by test/full_test.jl:27-42."""
     sum1 = 1.3862943611198906
