@@ -84,7 +84,7 @@ struct pssfss_ctx {
     // reusable workspaces
     DBuf<cd> wZ, wB, wX, wR, wft, wout, wjk;
     DBuf<cf> wZf, wRf;
-    DBuf<int32_t> wpiv, wperm;
+    DBuf<int32_t> wpiv, wperm, wsel;
     DBuf<cd> wT;
     DBuf<cf> wTf;
     DBuf<double> wnorm;
@@ -558,6 +558,9 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
 
 // Solve the np systems in ctx->wZ for the right-hand sides in ctx->wB (nbf x ncol
 // column-major per point), leaving X in ctx->wB.
+// Refinement stops at this max-norm residual relative to |b| (or on stagnation).
+constexpr double IR_TOL = 1e-11;
+
 void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
                  pssfss_point_info *infos) {
     hipStream_t st = ctx->st;
@@ -592,17 +595,24 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
     double *rn = ctx->wnorm.p, *xn = ctx->wnorm.p + np * ncol;
     std::vector<double> hr(np * ncol), hx(np * ncol), hb(np * ncol);
+    // systems still refining; the residuals and solves run on these only
+    std::vector<int32_t> act(np);
+    for (int p = 0; p < np; ++p) act[p] = p;
+    ctx->wsel.ensure((size_t)np);
+    HIPCHECK(hipMemcpyAsync(ctx->wsel.p, act.data(), np * sizeof(int32_t),
+                            hipMemcpyHostToDevice, st));
     auto residual = [&](const cd *X) {
+        const int na = (int)act.size();
         for (int c0 = 0; c0 < ncol; c0 += 8) {
             int nr = std::min(8, ncol - c0);
             // norms are per (system, rhs) over a strided sub-block
             residual_batched(n, nr, np, ctx->wZ.p, nb2, X + c0 * n, ctx->wB.p + c0 * n, sB,
-                             ctx->wR.p + c0 * n, rn, xn, st);
+                             ctx->wR.p + c0 * n, rn, xn, st, ctx->wsel.p, na);
             std::vector<double> tr(np * nr), tx(np * nr);
             d2h(ctx, tr.data(), rn, np * nr);
             d2h(ctx, tx.data(), xn, np * nr);
             sync(ctx);
-            for (int p = 0; p < np; ++p)
+            for (int p : act)
                 for (int c = 0; c < nr; ++c) {
                     hr[p * ncol + c0 + c] = tr[p * nr + c];
                     hx[p * ncol + c0 + c] = tx[p * nr + c];
@@ -622,27 +632,30 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     const int maxit = 10;
     for (int it = 0; it < maxit; ++it) {
         residual(ctx->wX.p);
-        bool all = true;
-        for (int p = 0; p < np; ++p) {
+        for (int p : act) {
             double r = 0;
             for (int c = 0; c < ncol; ++c) {
                 double bb = hb[p * ncol + c] > 0 ? hb[p * ncol + c] : 1.0;
                 r = std::max(r, hr[p * ncol + c] / bb);
             }
-            if (!done[p]) {
-                iters[p] = it + 1;
-                if (r <= 1e-13 || r >= 0.5 * prev[p]) done[p] = 1;
-                if (r > 2 * best[p] && it > 0) diverged[p] = 1;
-                best[p] = std::min(best[p], r);
-                prev[p] = r;
-            }
-            if (!done[p]) all = false;
+            iters[p] = it + 1;
+            if (r <= IR_TOL || r >= 0.5 * prev[p]) done[p] = 1;
+            if (r > 2 * best[p] && it > 0) diverged[p] = 1;
+            best[p] = std::min(best[p], r);
+            prev[p] = r;
         }
-        if (all) break;
+        std::vector<int32_t> next;
+        for (int p : act)
+            if (!done[p]) next.push_back(p);
+        if (next.empty()) break;
+        act.swap(next);
+        HIPCHECK(hipMemcpyAsync(ctx->wsel.p, act.data(), act.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, st));
+        // only the listed systems are solved and corrected
         residual_to_f32(ctx->wR.p, ctx->wRf.p, sB * np, st);
-        lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wperm.p, n, ctx->wRf.p, sB,
-                         ctx->wTf.p, st);
-        add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
+        lu_solve_batched_f32(n, ncol, (int)act.size(), ctx->wZf.p, nb2, ctx->wperm.p, n,
+                             ctx->wRf.p, sB, ctx->wTf.p, st, ctx->wsel.p);
+        add_correction_sel(ctx->wX.p, ctx->wRf.p, sB, ctx->wsel.p, (int)act.size(), st);
     }
     stage_end(5, st);
     // Points whose refinement did not reach an FP64-level residual: FP64 LU.

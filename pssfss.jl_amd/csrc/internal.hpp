@@ -157,16 +157,21 @@ void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t
                   hipStream_t st);
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv,
                            int64_t sP, hipStream_t st);
+// sel (optional): the batch lists system indices, one per entry.
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
                           const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
-                          hipStream_t st);
+                          hipStream_t st, const int32_t *sel = nullptr);
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st);
 // r = b - A x (FP64), batched; then per-(system, rhs) norms of r and of x.
+// With sel, only the nsel listed systems are computed (norms of the others are 0).
 void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
                       const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
-                      hipStream_t st);
+                      hipStream_t st, const int32_t *sel = nullptr, int nsel = 0);
 void residual_to_f32(const cd *R, cf *Rf, int64_t count, hipStream_t st);
 void add_correction(cd *X, const cf *D, int64_t count, hipStream_t st);
+// X[sys] += D[sys] for the nsel systems listed in sel, each of sB entries.
+void add_correction_sel(cd *X, const cf *D, int64_t sB, const int32_t *sel, int nsel,
+                        hipStream_t st);
 
 // ---- gsmk.hip
 struct BfftJob {                // one Fourier transform: wavevector and phases

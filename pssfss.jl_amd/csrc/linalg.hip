@@ -150,6 +150,201 @@ __global__ __launch_bounds__(1024) void k_panel(int64_t n, int64_t k, int kb, C 
     }
 }
 
+// Panel factorisation with the rows resident in registers, W columns at a time.
+// Interchanges inside the panel are logical: perm[i] (LDS) is the physical row
+// (relative to k) holding logical row i, so a pivot swap exchanges two perm
+// entries and two register rows and moves no memory.  After each W-column
+// sub-block is factored, its U rows for the remaining panel columns are solved
+// against the W x W unit-lower block, and the remaining panel columns of the
+// rows below take the rank-W update.  At the end the panel rows are written back
+// in logical order, so the matrix holds exactly what LAPACK's sequential
+// interchanges (recorded in ipiv) produce.  Rows per thread: R (m <= 1024 R).
+template <class C, int R, int W>
+__global__ __launch_bounds__(1024) void k_panel_reg(int64_t n, int64_t k, int kb, C *A0,
+                                                    int64_t sA, int32_t *ipiv0, int64_t sP) {
+    typedef typename CTraits<C>::R Rl;
+    constexpr int NT = 1024;
+    C *A = A0 + (int64_t)blockIdx.x * sA;
+    int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
+    const int m = (int)(n - k);
+    __shared__ int perm[R * NT];
+    __shared__ C prow[W];
+    __shared__ C Ld[W][W];
+    __shared__ C Us[W][NB];
+    __shared__ C xch[2][W];
+    __shared__ Rl rv[16];
+    __shared__ int ri[16];
+    __shared__ int pivs;
+    const int tid = threadIdx.x;
+    const C zero = CTraits<C>::zero();
+    for (int i = tid; i < m; i += NT) perm[i] = i;
+    __syncthreads();
+    C x[R][W];
+    for (int c0 = 0; c0 < kb; c0 += W) {
+        const int wb = kb - c0 < W ? kb - c0 : W;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tid + NT * r;
+            const bool live = i >= c0 && i < m;
+            const C *row = A + (k + (live ? perm[i] : 0)) * n + k + c0;
+#pragma unroll
+            for (int c = 0; c < W; ++c) x[r][c] = (live && c < wb) ? row[c] : zero;
+        }
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (j >= wb) break;
+            const int col = c0 + j;
+            Rl best = (Rl)-1;
+            int bi = 0x7fffffff;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tid + NT * r;
+                if (i >= col && i < m) {
+                    Rl v = cabs1(x[r][j]);
+                    if (v > best) { best = v; bi = i; }
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                Rl ov = __shfl_xor(best, off, 64);
+                int oi = __shfl_xor(bi, off, 64);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
+            __syncthreads();
+            if (tid == 0) {
+                Rl b = rv[0];
+                int bx = ri[0];
+                for (int w = 1; w < NT / 64; ++w)
+                    if (rv[w] > b || (rv[w] == b && ri[w] < bx)) { b = rv[w]; bx = ri[w]; }
+                pivs = bx;
+                ipiv[k + col] = (int32_t)(k + bx);
+            }
+            __syncthreads();
+            const int p = pivs;
+            const int oc = col % NT, sc = col / NT, op = p % NT, sp = p / NT;
+            if (p != col) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (tid == op && r == sp)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) xch[0][c] = x[r][c];
+                    if (tid == oc && r == sc)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) xch[1][c] = x[r][c];
+                }
+                if (tid == 0) { int t = perm[col]; perm[col] = perm[p]; perm[p] = t; }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (tid == oc && r == sc)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) x[r][c] = xch[0][c];
+                    if (tid == op && r == sp)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) x[r][c] = xch[1][c];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (tid == oc && r == sc)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) prow[c] = x[r][c];
+            __syncthreads();
+            const C piv = prow[j];
+            const bool nz = !(piv.x == 0 && piv.y == 0);
+            const C rp = nz ? CTraits<C>::one() / piv : zero;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tid + NT * r;
+                if (i > col && i < m) {
+                    C l = nz ? x[r][j] * rp : x[r][j];
+                    x[r][j] = l;
+                    C ml = -l;
+#pragma unroll
+                    for (int c = j + 1; c < W; ++c) cfma(x[r][c], ml, prow[c]);
+                }
+            }
+        }
+        // write the sub-block back; the diagonal block goes to LDS as well
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tid + NT * r;
+            if (i >= c0 && i < m) {
+                C *row = A + (k + perm[i]) * n + k + c0;
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (c < wb) row[c] = x[r][c];
+                if (i < c0 + wb)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) Ld[i - c0][c] = x[r][c];
+            }
+        }
+        __syncthreads();
+        const int cr0 = c0 + wb;
+        if (cr0 < kb) {
+            // U rows of this sub-block for panel columns [cr0, kb): unit-lower solve
+            if (tid < kb - cr0) {
+                const int cc = cr0 + tid;
+                C u[W];
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                    if (j < wb) {
+                        C v = A[(k + perm[c0 + j]) * n + k + cc];
+#pragma unroll
+                        for (int l = 0; l < j; ++l) cfma(v, -Ld[j][l], u[l]);
+                        u[j] = v;
+                        A[(k + perm[c0 + j]) * n + k + cc] = v;
+                        Us[j][cc] = v;
+                    }
+                }
+            }
+            __syncthreads();
+            // rank-wb update of the remaining panel columns of the rows below
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tid + NT * r;
+                if (i >= cr0 && i < m) {
+                    C *row = A + (k + perm[i]) * n + k;
+                    for (int cc = cr0; cc < kb; ++cc) {
+                        C v = row[cc];
+#pragma unroll
+                        for (int j = 0; j < W; ++j)
+                            if (j < wb) cfma(v, -x[r][j], Us[j][cc]);
+                        row[cc] = v;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // Panel rows into logical order, W columns at a time.
+    for (int c0 = 0; c0 < kb; c0 += W) {
+        const int wb = kb - c0 < W ? kb - c0 : W;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tid + NT * r;
+            if (i < m) {
+                const C *row = A + (k + perm[i]) * n + k + c0;
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (c < wb) x[r][c] = row[c];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tid + NT * r;
+            if (i < m) {
+                C *row = A + (k + i) * n + k + c0;
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (c < wb) row[c] = x[r][c];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Apply the panel's interchanges to the rest of each row (columns outside the
 // panel), in pivot order.
 template <class C>
@@ -234,19 +429,26 @@ __device__ __forceinline__ C wave_sum(C v) {
 }
 
 // P B, then L y = b (unit lower) and U x = y for nrhs <= NR columns of a
-// column-major B, one workgroup per system.  Diagonal 64x64 blocks are solved
-// one wavefront per right-hand side; the off-diagonal updates take one wavefront
-// per matrix row, its lanes reading the row's 64 block entries at once.
-template <class C, int NR>
-__global__ __launch_bounds__(1024) void k_trsv(int64_t n, int nrhs, const C *LU0, int64_t sA,
-                                               const int32_t *perm0, int64_t sP, C *B0,
-                                               int64_t sB, C *T0) {
-    const C *LU = LU0 + (int64_t)blockIdx.x * sA;
-    const int32_t *perm = perm0 + (int64_t)blockIdx.x * sP;
-    C *B = B0 + (int64_t)blockIdx.x * sB;
-    C *T = T0 + (int64_t)blockIdx.x * sB;
+// column-major B, one workgroup per system, left-looking: the rows of block b
+// first take the whole dot product with the already solved part (lanes stride
+// the contiguous row, RW rows per wavefront, U column chunks in flight, one
+// cross-lane reduction per row and right-hand side), then the 64x64 diagonal
+// block is solved one wavefront per right-hand side.  XL: the solution is also
+// kept in LDS (n * NR entries of dynamic shared memory) for the dot products.
+// sel (optional) lists the systems to solve, one per workgroup.
+template <class C, int NR, int RW, int U, bool XL>
+__global__ __launch_bounds__(1024) void k_trsv_ll(int64_t n, int nrhs, const C *LU0, int64_t sA,
+                                                  const int32_t *perm0, int64_t sP, C *B0,
+                                                  int64_t sB, C *T0, const int32_t *sel) {
+    const int64_t sys = sel ? sel[blockIdx.x] : blockIdx.x;
+    const C *LU = LU0 + sys * sA;
+    const int32_t *perm = perm0 + sys * sP;
+    C *B = B0 + sys * sB;
+    C *T = T0 + sys * sB;
     __shared__ C D[NB][NB + 1];
     __shared__ C Y[NR][NB];
+    extern __shared__ char xlds_raw[];
+    C *XS = reinterpret_cast<C *>(xlds_raw);      // [NR][n] when XL
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
     const C zero = CTraits<C>::zero();
     for (int64_t t = tid; t < n * nrhs; t += blockDim.x) {
@@ -256,6 +458,7 @@ __global__ __launch_bounds__(1024) void k_trsv(int64_t n, int nrhs, const C *LU0
     __syncthreads();
     for (int64_t t = tid; t < n * nrhs; t += blockDim.x) B[t] = T[t];
     __syncthreads();
+    const C *X = XL ? XS : B;
     const int64_t nblk = (n + NB - 1) / NB;
     for (int pass = 0; pass < 2; ++pass) {
         const bool fwd = pass == 0;
@@ -267,13 +470,56 @@ __global__ __launch_bounds__(1024) void k_trsv(int64_t n, int nrhs, const C *LU0
                 int r = t / bs, l = t % bs;
                 D[r][l] = LU[(b0 + r) * n + b0 + l];
             }
-            for (int t = tid; t < NR * NB; t += blockDim.x) {
-                int c = t / NB, r = t % NB;
-                Y[c][r] = (c < nrhs && r < bs) ? B[b0 + r + n * c] : zero;
+            const int64_t j0 = fwd ? 0 : b0 + bs, j1 = fwd ? b0 : n;
+            for (int r0 = wave * RW; r0 < bs; r0 += nw * RW) {
+                C acc[RW][NR];
+#pragma unroll
+                for (int a = 0; a < RW; ++a)
+#pragma unroll
+                    for (int c = 0; c < NR; ++c) acc[a][c] = zero;
+                const C *rows[RW];
+#pragma unroll
+                for (int a = 0; a < RW; ++a)
+                    rows[a] = LU + (b0 + (r0 + a < bs ? r0 + a : r0)) * n;
+                int64_t j = j0 + lane;
+                for (; j + 64 * (U - 1) < j1; j += 64 * U) {
+                    C l[U][RW];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int a = 0; a < RW; ++a) l[u][a] = rows[a][j + 64 * u];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int c = 0; c < NR; ++c) {
+                            if (c < nrhs) {
+                                const C xv = X[j + 64 * u + n * c];
+#pragma unroll
+                                for (int a = 0; a < RW; ++a) cfma(acc[a][c], l[u][a], xv);
+                            }
+                        }
+                }
+                for (; j < j1; j += 64) {
+#pragma unroll
+                    for (int a = 0; a < RW; ++a) {
+                        const C lv = rows[a][j];
+#pragma unroll
+                        for (int c = 0; c < NR; ++c)
+                            if (c < nrhs) cfma(acc[a][c], lv, X[j + n * c]);
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < RW; ++a)
+#pragma unroll
+                    for (int c = 0; c < NR; ++c) {
+                        C v = wave_sum(acc[a][c]);
+                        if (lane == c && c < nrhs && r0 + a < bs)
+                            Y[c][r0 + a] = B[b0 + r0 + a + n * c] - v;
+                    }
             }
             __syncthreads();
             if (wave < nrhs) {
-                C y = Y[wave][lane];
+                C y = lane < bs ? Y[wave][lane] : zero;
                 if (fwd) {
                     for (int r = 0; r < bs; ++r) {
                         C xr = shfl_c(y, r);
@@ -286,25 +532,10 @@ __global__ __launch_bounds__(1024) void k_trsv(int64_t n, int nrhs, const C *LU0
                         if (lane < r) cfma(y, -D[lane][r], xr);
                     }
                 }
-                Y[wave][lane] = y;
-                if (lane < bs) B[b0 + lane + n * wave] = y;
-            }
-            __syncthreads();
-            const int64_t r0 = fwd ? b0 + bs : 0, r1 = fwd ? n : b0;
-            for (int64_t i = r0 + wave; i < r1; i += nw) {
-                C a = lane < bs ? LU[i * n + b0 + lane] : zero;
-                C acc[NR];
-#pragma unroll
-                for (int c = 0; c < NR; ++c) {
-                    acc[c] = zero;
-                    if (c < nrhs) cfma(acc[c], a, Y[c][lane]);
-                    acc[c] = wave_sum(acc[c]);
+                if (lane < bs) {
+                    B[b0 + lane + n * wave] = y;
+                    if (XL) XS[b0 + lane + n * wave] = y;
                 }
-                C mine = zero;
-#pragma unroll
-                for (int c = 0; c < NR; ++c)
-                    if (c == lane) mine = acc[c];
-                if (lane < nrhs) B[i + n * lane] -= mine;
             }
             __syncthreads();
         }
@@ -317,8 +548,8 @@ template <int NR>
 __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd *A0, int64_t sA,
                                                   const cd *X0, const cd *B0, int64_t sB,
                                                   cd *R0, unsigned long long *rmax,
-                                                  unsigned long long *xmax) {
-    const int64_t b = blockIdx.y;
+                                                  unsigned long long *xmax, const int32_t *sel) {
+    const int64_t b = sel ? sel[blockIdx.y] : blockIdx.y;
     const cd *A = A0 + b * sA;
     const cd *X = X0 + b * sB;
     const cd *Bv = B0 + b * sB;
@@ -364,6 +595,15 @@ __global__ void k_cd_to_cf(const cd *a, cf *b, int64_t n) {
     for (; i < n; i += st) b[i] = cf{(float)a[i].x, (float)a[i].y};
 }
 
+__global__ void k_add_corr_sel(cd *x0, const cf *d0, int64_t sB, const int32_t *sel) {
+    const int64_t sys = sel[blockIdx.y];
+    cd *x = x0 + sys * sB;
+    const cf *d = d0 + sys * sB;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sB;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = x[i] + cd{(double)d[i].x, (double)d[i].y};
+}
+
 __global__ void k_add_corr(cd *x, const cf *d, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -386,6 +626,34 @@ void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, c
     HIPCHECK(hipGetLastError());
 }
 
+// Register-resident panel for panels of up to 12288 rows, the streaming panel
+// beyond.  W is chosen so the R x W register block needs no scratch.
+template <class C>
+void launch_panel(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                  int batch, hipStream_t st) {
+    const int64_t m = n - k;
+    constexpr bool F = sizeof(C) == sizeof(cf);
+    dim3 g((unsigned)batch), b(1024);
+#define PR(RR, WW) hipLaunchKernelGGL((k_panel_reg<C, RR, WW>), g, b, 0, st, n, k, kb, A, sA, ipiv, sP)
+    if constexpr (F) {
+        if (m <= 1024) PR(1, 8);
+        else if (m <= 2048) PR(2, 8);
+        else if (m <= 3072) PR(3, 8);
+        else if (m <= 6144) PR(6, 4);
+        else if (m <= 12288) PR(12, 2);
+        else hipLaunchKernelGGL(k_panel<C>, g, b, 0, st, n, k, kb, A, sA, ipiv, sP);
+    } else {
+        if (m <= 1024) PR(1, 8);
+        else if (m <= 2048) PR(2, 4);
+        else if (m <= 3072) PR(3, 4);
+        else if (m <= 6144) PR(6, 2);
+        else if (m <= 12288) PR(12, 1);
+        else hipLaunchKernelGGL(k_panel<C>, g, b, 0, st, n, k, kb, A, sA, ipiv, sP);
+    }
+#undef PR
+    HIPCHECK(hipGetLastError());
+}
+
 template <class C>
 void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
                  hipStream_t st) {
@@ -395,9 +663,7 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
     for (int64_t k = 0; k < n; k += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k);
         stage_begin(3, st);
-        hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(1024), 0, st, n, k, kb, A, sA,
-                           ipiv, sP);
-        HIPCHECK(hipGetLastError());
+        launch_panel<C>(n, k, kb, A, sA, ipiv, sP, batch, st);
         const int64_t other = n - kb;
         if (other > 0) {
             dim3 g((unsigned)((other + 255) / 256), (unsigned)batch);
@@ -426,19 +692,39 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
     stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
 }
 
+// Largest shared memory per workgroup on the current device (bytes).
+size_t max_lds_bytes() {
+    int dev = 0, v = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return (size_t)v;
+}
+
 template <class C>
 void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *perm,
-                int64_t sP, C *B, int64_t sB, C *scratch, hipStream_t st) {
+                int64_t sP, C *B, int64_t sB, C *scratch, hipStream_t st,
+                const int32_t *sel = nullptr) {
+    if (batch <= 0) return;
+    const size_t static_lds = sizeof(C) * (NB * (NB + 1) + 8 * NB);
     for (int c0 = 0; c0 < nrhs; c0 += 8) {
         int nr = std::min(8, nrhs - c0);
         C *Bc = B + (int64_t)c0 * n;
         C *Tc = scratch + (int64_t)c0 * n;
-        if (nr <= 4)
-            hipLaunchKernelGGL((k_trsv<C, 4>), dim3((unsigned)batch), dim3(1024), 0, st, n, nr,
-                               LU, sA, perm, sP, Bc, sB, Tc);
-        else
-            hipLaunchKernelGGL((k_trsv<C, 8>), dim3((unsigned)batch), dim3(1024), 0, st, n, nr,
-                               LU, sA, perm, sP, Bc, sB, Tc);
+        const int NRk = nr <= 4 ? 4 : 8;
+        const size_t xl = sizeof(C) * (size_t)n * NRk;
+        const bool useX = static_lds + xl <= max_lds_bytes();
+        dim3 g((unsigned)batch), b(1024);
+#define TL(NRR, RWW, UU, XLL) hipLaunchKernelGGL((k_trsv_ll<C, NRR, RWW, UU, XLL>), g, b,      \
+                                                 XLL ? xl : 0, st, n, nr, LU, sA, perm, sP,  \
+                                                 Bc, sB, Tc, sel)
+        if constexpr (sizeof(C) == sizeof(cf)) {
+            if (NRk == 4) { if (useX) TL(4, 2, 4, true); else TL(4, 2, 4, false); }
+            else { if (useX) TL(8, 2, 4, true); else TL(8, 2, 2, false); }
+        } else {
+            if (NRk == 4) { if (useX) TL(4, 2, 4, true); else TL(4, 2, 2, false); }
+            else { if (useX) TL(8, 1, 2, true); else TL(8, 1, 2, false); }
+        }
+#undef TL
         HIPCHECK(hipGetLastError());
     }
 }
@@ -477,8 +763,8 @@ void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipi
 
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
                           const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
-                          hipStream_t st) {
-    lu_solve_t<cf>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st);
+                          hipStream_t st, const int32_t *sel) {
+    lu_solve_t<cf>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
 }
 
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {
@@ -488,17 +774,19 @@ void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {
 
 void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
                       const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
-                      hipStream_t st) {
+                      hipStream_t st, const int32_t *sel, int nsel) {
     if (nrhs > 8) throw ArgError("residual_batched: at most 8 right-hand sides per call");
     launch_zero((cd *)rnorm, (batch * nrhs + 1) / 2, st);
     launch_zero((cd *)xnorm, (batch * nrhs + 1) / 2, st);
-    dim3 g((unsigned)((n + 3) / 4), (unsigned)batch);
+    const int ns = sel ? nsel : batch;
+    if (ns <= 0) return;
+    dim3 g((unsigned)((n + 3) / 4), (unsigned)ns);
     if (nrhs <= 4)
         hipLaunchKernelGGL(k_residual<4>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
-                           (unsigned long long *)rnorm, (unsigned long long *)xnorm);
+                           (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
     else
         hipLaunchKernelGGL(k_residual<8>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
-                           (unsigned long long *)rnorm, (unsigned long long *)xnorm);
+                           (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
     HIPCHECK(hipGetLastError());
 }
 
@@ -508,5 +796,14 @@ void residual_to_f32(const cd *R, cf *Rf, int64_t count, hipStream_t st) {
 
 void add_correction(cd *X, const cf *D, int64_t count, hipStream_t st) {
     hipLaunchKernelGGL(k_add_corr, dim3(grid1(count)), dim3(256), 0, st, X, D, count);
+    HIPCHECK(hipGetLastError());
+}
+
+void add_correction_sel(cd *X, const cf *D, int64_t sB, const int32_t *sel, int nsel,
+                        hipStream_t st) {
+    if (nsel <= 0) return;
+    int64_t bx = (sB + 255) / 256;
+    dim3 g((unsigned)(bx > 64 ? 64 : bx), (unsigned)nsel);
+    hipLaunchKernelGGL(k_add_corr_sel, g, dim3(256), 0, st, X, D, sB, sel);
     HIPCHECK(hipGetLastError());
 }

@@ -34,7 +34,7 @@ def _fillzy_case(pk, cls):
 
 # ------------------------------------------------------------------ linear algebra
 @pytest.mark.parametrize("n,nrhs,nb", [(1, 1, 1), (63, 2, 2), (64, 4, 3), (65, 3, 1),
-                                       (130, 5, 2), (300, 4, 2)])
+                                       (130, 5, 2), (300, 4, 2), (1500, 6, 1)])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_lu_solve(ctx, n, nrhs, nb, prec):
     _say(f"lu_solve n={n} nrhs={nrhs} batch={nb} {prec}")
@@ -57,6 +57,23 @@ def test_lu_pivoting(ctx):
     B = rng.standard_normal((n, 2)) + 0j
     X, _, _ = ctx.lu_solve(A, B, "fp64")
     assert _rel(X[0], np.linalg.solve(A, B)) < 1e-11
+
+
+@pytest.mark.parametrize("n", [1100, 2100, 3100])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lu_pivoting_large(ctx, n, prec):
+    """Unscaled Gaussian matrices: pivots land anywhere in tall panels of 1-6 rows per
+    thread (register panels of R = 2, 3 and 6) and solutions come from both the
+    LDS-resident and the streaming triangular-solve variants."""
+    _say(f"lu_solve random Gaussian n={n} {prec}")
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    X, _, _ = ctx.lu_solve(A, B, prec)
+    ref = np.linalg.solve(A, B)
+    err = _rel(X[0], ref)
+    print(f"  rel err {err:.2e}")
+    assert err < (1e-10 if prec == "fp64" else 1e-6)
 
 
 @pytest.mark.parametrize("n1,n2,n3", [(1, 1, 1), (2, 2, 2), (2, 6, 3), (10, 20, 15), (34, 34, 34)])
