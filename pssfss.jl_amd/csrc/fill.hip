@@ -33,10 +33,10 @@ __device__ cd sigma_interp(const cd *tab, int M, const double *b1, const double 
     if (fabs(x1o) < 1e-8) x1o = 0.0;
     double x2o = (b2[0] * rx + b2[1] * ry) / TWOPI;
     if (fabs(x2o) < 1e-8) x2o = 0.0;
-    double x1 = fmod(x1o, 1.0);
-    if (x1 == 0) x1 = 0.0; else if (x1 < 0) x1 = x1 + 1.0;
-    double x2 = fmod(x2o, 1.0);
-    if (x2 == 0) x2 = 0.0; else if (x2 < 0) x2 = x2 + 1.0;
+    // Julia's mod(x, 1): x - floor(x) rounds the same real number once, as
+    // fmod(x, 1) (exact) followed by +1 for negative x does.
+    double x1 = x1o - floor(x1o);
+    double x2 = x2o - floor(x2o);
     double msh = rint(x1o - x1), nsh = rint(x2o - x2);
     int m = (int)(M * x1);
     double p = M * x1 - m;
@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void k_fill(SheetDev sh, const FillParams *fps
     if (!MAGNETIC) phi_core = ((FOURPI * I2) + u * (ri + J)) + (P.d3 / u) * K;
     else phi_core = ((P.I2fact * I2) + u * (ri + J)) + P.KPfact * K;
     const cd iw = MAGNETIC ? cd{0.0, -P.omega} : cd{0.0, P.omega};
+    const cd psf_scale = P.psf_base * (1.0 / ((TWOPI * P.omega) * EPS0));
     const int64_t nmem = sh.identity ? 1 : sh.uoff[iu + 1] - sh.uoff[iu];
     const int64_t nbf = sh.nbf;
     for (int64_t im_ = 0; im_ < nmem; ++im_) {
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(256) void k_fill(SheetDev sh, const FillParams *fps
             cd Ai[2], Phi;
             if (!MAGNETIC) {
                 cd asf = ((MU0 / FOURPI) * sflag) * P.mut;
-                cd psf = P.psf_base * (sflag / ((TWOPI * P.omega) * EPS0));
+                cd psf = psf_scale * sflag;
 #pragma unroll
                 for (int d = 0; d < 2; ++d)
                     Ai[d] = asf * ((((FOURPI * I1i[d]) + sing[d]) + u * Ji[d]) + (P.c3 / u) * Ki[d]);

@@ -635,6 +635,267 @@ __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, dou
     }
 }
 
+// MFMA form of the many-phasing spatial sums.  The lattice-term tile (64 rows x 8
+// points) is the A operand and the phase tile (8 points x 32 phasings, cos and
+// sin columns) the B operand of 16x16x4 MFMAs (FP32 for T = float, FP64 for
+// T = double).  Wavefront w owns rows 16w..16w+15 and each lane owns 4 rows of
+// one phasing column in each of two 16-phasing groups, so it folds the ring sums
+// of its 8 (row, phasing) pairs of J and K into FP64 running sums and applies the
+// break test of src/PGF.jl:86-103 at every ring end.  Tiles are double-buffered
+// in LDS and the next phase tile is prefetched into registers, one barrier per
+// 8-point chunk.
+typedef float mf4 __attribute__((ext_vector_type(4)));
+typedef double md4 __attribute__((ext_vector_type(4)));
+
+template <class T> struct MfmaT;
+template <> struct MfmaT<float> {
+    typedef mf4 V;
+    static __device__ __forceinline__ V mma(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // row (within the 16-row tile) of element e of lane l
+    static __device__ __forceinline__ int row(int l, int e) { return 4 * (l >> 4) + e; }
+};
+template <> struct MfmaT<double> {
+    typedef md4 V;
+    static __device__ __forceinline__ V mma(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int e) { return (l >> 4) + 4 * e; }
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
+                                                 double us1x, double us1y, double us2x,
+                                                 double us2y, const T *phase, int ldp,
+                                                 int nf, cd *jk, int64_t jk_stride) {
+#pragma clang fp contract(off)
+    typedef MfmaT<T> MT;
+    typedef typename MT::V V;
+    constexpr int FT = 32;
+    // the ring tiles and the final per-row sums share one LDS region
+    struct Tiles {
+        T Aj[2][8][64], Ak[2][8][64];
+        T Bre[2][8][FT], Bim[2][8][FT];
+    };
+    constexpr size_t LBYTES = sizeof(Tiles) > sizeof(cd) * 64 * FT ? sizeof(Tiles)
+                                                                  : sizeof(cd) * 64 * FT;
+    __shared__ __align__(16) char lraw[LBYTES];
+    __shared__ double urxs[64], urys[64];
+    __shared__ double offs[2][8][2];      // lattice offsets m s1 + n s2 of the chunk points
+    __shared__ int pmn[2][8][2];          // (m, n) of the chunk points
+    constexpr int PW = 2 * JK_RT + 1;
+    __shared__ T P1[2][FT][PW], P2[2][FT][PW];   // [cos/sin][phasing][m + JK_RT]
+    Tiles &TL = *reinterpret_cast<Tiles *>(lraw);
+    auto &Aj = TL.Aj;
+    auto &Ak = TL.Ak;
+    auto &Bre = TL.Bre;
+    auto &Bim = TL.Bim;
+    cd(*Cs)[FT] = reinterpret_cast<cd(*)[FT]>(lraw);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t u0 = (int64_t)blockIdx.x * 9;
+    const int f0 = blockIdx.y * FT;
+    if (tid < 64) {
+        const int r = tid;
+        const int64_t iu = u0 + r / 7;
+        double urx = 0.0, ury = 0.0;
+        if (r < 63 && iu < nufp) {
+            const int qp = r % 7;
+            int64_t ifm, ifs;
+            class_faces(sh, iu, ifm, ifs);
+            const double *rs = sh.fxs + 6 * ifs;
+            const double *rm = sh.fxs + 6 * ifm;
+            double rs12x = rs[0] - rs[2], rs12y = rs[1] - rs[3];
+            double rs31x = rs[4] - rs[0], rs31y = rs[5] - rs[1];
+            double rs21x = -rs12x, rs21y = -rs12y;
+            double rmcx = ((rm[0] + rm[2]) + rm[4]) / 3.0;
+            double rmcy = ((rm[1] + rm[3]) + rm[5]) / 3.0;
+            double rtx = (rs[0] + rs21x * QXI[qp]) + rs31x * QETA[qp];
+            double rty = (rs[1] + rs21y * QXI[qp]) + rs31y * QETA[qp];
+            urx = ul * (rmcx - rtx);
+            ury = ul * (rmcy - rty);
+        }
+        urxs[r] = urx;
+        urys[r] = ury;
+    }
+    __syncthreads();
+    const int pc = lane & 15;
+    cd js[2][4], ks[2][4];
+    unsigned done = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int r = 16 * wave + MT::row(lane, e);
+        const bool live = r < 63 && u0 + r / 7 < nufp;
+        double arg = sqrt(urxs[r] * urxs[r] + urys[r] * urys[r]);
+        double rterm = exp(-arg);
+        double j0 = arg <= 1e-5 ? (-arg / 6 + 0.5) * arg - 1 : (rterm - 1) / arg;
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            js[g][e] = cd{j0, 0.0};
+            ks[g][e] = cd{rterm, 0.0};
+            if (!live || f0 + 16 * g + pc >= nf) done |= 1u << (4 * g + e);
+        }
+    }
+    // this thread's slot of the phase tile
+    const int pkk = tid / FT, pf = tid % FT;
+    const T *pcol = phase + 2 * (f0 + pf);
+    const int64_t pstride = 2 * (int64_t)ldp;
+    // exp(-i m psi1) and exp(-i n psi2) for |m|, |n| <= JK_RT, from the phase table's
+    // (m, 0) and (0, n) entries; phases of rings <= JK_RT are their products
+    for (int t = tid; t < 2 * FT * PW; t += 256) {
+        const int which = t / (FT * PW), f = (t / PW) % FT, j = t % PW - JK_RT;
+        const int64_t e = which == 0 ? ring_index(j, 0) : ring_index(0, j);
+        const T *pp = phase + e * pstride + 2 * (f0 + f);
+        if (which == 0) { P1[0][f][j + JK_RT] = pp[0]; P1[1][f][j + JK_RT] = pp[1]; }
+        else { P2[0][f][j + JK_RT] = pp[0]; P2[1][f][j + JK_RT] = pp[1]; }
+    }
+    if (pf == 0) {                        // ring 1's first chunk
+        int m, n;
+        ring_mn(1, pkk, m, n);
+        offs[0][pkk][0] = (double)m * us1x + (double)n * us2x;
+        offs[0][pkk][1] = (double)m * us1y + (double)n * us2y;
+        pmn[0][pkk][0] = m;
+        pmn[0][pkk][1] = n;
+    }
+    int buf = 0;
+    for (int r = 1; r <= JK_RINGMAX; ++r) {
+        if (!__syncthreads_or(done != 0xffu)) break;
+        V aJ[2][2], aK[2][2];          // [g][cos/sin]
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                aJ[g][c] = V{0, 0, 0, 0};
+                aK[g][c] = V{0, 0, 0, 0};
+            }
+        for (int c = 0; c < r; ++c) {
+            // terms of chunk c (offsets staged one chunk ahead) into buffer buf
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int t = tid + 256 * h;
+                const int row = t & 63, kk = t >> 6;
+                double dx = urxs[row] - offs[buf][kk][0];
+                double dy = urys[row] - offs[buf][kk][1];
+                T a, e, ea;
+                if constexpr (sizeof(T) == 4) {
+                    float fx = (float)dx, fy = (float)dy;
+                    a = __builtin_amdgcn_sqrtf(fx * fx + fy * fy);
+                    e = __expf(-a);
+                    ea = __fdividef(e, a);
+                } else {
+                    a = sqrt(dx * dx + dy * dy);
+                    e = exp(-a);
+                    ea = e * (1.0 / a);
+                }
+                Aj[buf][kk][row] = ea;
+                Ak[buf][kk][row] = e;
+            }
+            {
+                T br, bi;
+                if (r <= JK_RT) {
+                    const int m = pmn[buf][pkk][0] + JK_RT, n = pmn[buf][pkk][1] + JK_RT;
+                    const T c1 = P1[0][pf][m], s1 = P1[1][pf][m];
+                    const T c2 = P2[0][pf][n], s2 = P2[1][pf][n];
+                    br = c1 * c2 - s1 * s2;
+                    bi = c1 * s2 + s1 * c2;
+                } else {
+                    const int64_t e = ring_base(r) + 8 * c + pkk;
+                    br = pcol[e * pstride];
+                    bi = pcol[e * pstride + 1];
+                }
+                Bre[buf][pkk][pf] = br;
+                Bim[buf][pkk][pf] = bi;
+            }
+            // offsets of the next chunk (the next ring's first chunk at a ring end)
+            if (pf == 0) {
+                const bool same = c + 1 < r;
+                const int nr_ = same ? r : (r < JK_RINGMAX ? r + 1 : r);
+                const int nidx = same ? 8 * (c + 1) : 0;
+                int m, n;
+                ring_mn(nr_, nidx + pkk, m, n);
+                offs[buf ^ 1][pkk][0] = (double)m * us1x + (double)n * us2x;
+                offs[buf ^ 1][pkk][1] = (double)m * us1y + (double)n * us2y;
+                pmn[buf ^ 1][pkk][0] = m;
+                pmn[buf ^ 1][pkk][1] = n;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k4 = 0; k4 < 2; ++k4) {
+                const int kr = 4 * k4 + (lane >> 4);
+                const T a_j = Aj[buf][kr][16 * wave + (lane & 15)];
+                const T a_k = Ak[buf][kr][16 * wave + (lane & 15)];
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    const T bre = Bre[buf][kr][16 * g + (lane & 15)];
+                    const T bim = Bim[buf][kr][16 * g + (lane & 15)];
+                    aJ[g][0] = MT::mma(a_j, bre, aJ[g][0]);
+                    aJ[g][1] = MT::mma(a_j, bim, aJ[g][1]);
+                    aK[g][0] = MT::mma(a_k, bre, aK[g][0]);
+                    aK[g][1] = MT::mma(a_k, bim, aK[g][1]);
+                }
+            }
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const unsigned bit = 1u << (4 * g + e);
+                if (!(done & bit)) {
+                    cd a = cd{(double)aJ[g][0][e], (double)aJ[g][1][e]};
+                    cd b = cd{(double)aK[g][0][e], (double)aK[g][1][e]};
+                    js[g][e] += a;
+                    ks[g][e] += b;
+                    if (r > 5) {
+                        bool conv;
+                        if constexpr (sizeof(T) == 4) {
+                            // |ring/sum| < 1e-8 for J and K, as squared magnitudes
+                            const double aj = a.x * a.x + a.y * a.y;
+                            const double sj = js[g][e].x * js[g][e].x + js[g][e].y * js[g][e].y;
+                            const double ak = b.x * b.x + b.y * b.y;
+                            const double sk = ks[g][e].x * ks[g][e].x + ks[g][e].y * ks[g][e].y;
+                            conv = aj < 1e-16 * sj && ak < 1e-16 * sk;
+                        } else {
+                            double c1 = cabs(a / js[g][e]), c2 = cabs(b / ks[g][e]);
+                            conv = (c1 > c2 ? c1 : c2) < 1e-8;
+                        }
+                        if (conv) done |= bit;
+                    }
+                }
+            }
+    }
+    // J then K: per-row sums through LDS, combined over the 7 quadrature points
+    // (src/Zint.jl:117-124)
+    for (int which = 0; which < 2; ++which) {
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * wave + MT::row(lane, e);
+                Cs[r][16 * g + pc] = which == 0 ? js[g][e] : ks[g][e];
+            }
+        __syncthreads();
+        for (int t = tid; t < 9 * FT; t += 256) {
+            const int u = t / FT, f = t % FT;
+            const int64_t iu = u0 + u;
+            if (iu >= nufp || f0 + f >= nf) continue;
+            cd S = {0, 0}, Sx = {0, 0}, Se = {0, 0};
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+                cd w = Cs[7 * u + q][f] * QW[q];
+                Sx += w * QXI[q];
+                Se += w * QETA[q];
+                S += w;
+            }
+            cd *o = jk + (int64_t)(f0 + f) * jk_stride + (int64_t)(3 * which) * nufp;
+            o[iu] = S;
+            o[nufp + iu] = Sx;
+            o[2 * nufp + iu] = Se;
+        }
+    }
+}
+
 // Lattice phase table exp(-i (m psi1 + n psi2)) over rings 0..JK_RINGMAX in ring
 // order, one column per phasing (zero beyond nf), interleaved (re, im).
 template <class T>
@@ -799,9 +1060,8 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
     dim3 grid((unsigned)((nufp + 8) / 9), (unsigned)(ldp / FT));
     if (mixed) {
         hipLaunchKernelGGL(k_phase_table<float>, pg, dim3(256), 0, st, d_psi, nf, ldp, (float *)ph);
-        hipLaunchKernelGGL((k_jk_tiled<float, 2>), grid, dim3(256), 0, st, sh, nufp, ulocal,
-                           us1[0], us1[1], us2[0], us2[1], (const float *)ph, ldp, nf, jk,
-                           jk_stride);
+        hipLaunchKernelGGL(k_jk_mfma<float>, grid, dim3(256), 0, st, sh, nufp, ulocal, us1[0],
+                           us1[1], us2[0], us2[1], (const float *)ph, ldp, nf, jk, jk_stride);
     } else {
         hipLaunchKernelGGL(k_phase_table<double>, pg, dim3(256), 0, st, d_psi, nf, ldp,
                            (double *)ph);
