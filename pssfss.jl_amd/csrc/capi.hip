@@ -88,6 +88,7 @@ struct pssfss_ctx {
     DBuf<cd> wT;
     DBuf<cf> wTf;
     DBuf<double> wnorm;
+    DBuf<cd> wG;                 // fill class terms
     const cd *dft_matrix(int M) {
         auto it = dft.find(M);
         if (it != dft.end()) return it->second.p;
@@ -106,6 +107,8 @@ struct pssfss_sheet {
     double s1[2] = {0, 0}, s2[2] = {0, 0}, b1[2] = {0, 0}, b2[2] = {0, 0};
     DBuf<double> fxs, fxm, fr;
     DBuf<int32_t> fbf, fsg, feci, fedge, bf_face, bf_loc;
+    DBuf<int32_t> fpcls;        // class of face pair ifm + nface*ifs (translation classes)
+    bool has_fr = false;        // any face with nonzero sheet resistance
     DBuf<int64_t> uoff, umem;
     // J/K cache with the key semantics of src/FillZY.jl:97-108
     bool jk_valid = false;
@@ -519,6 +522,9 @@ struct ChunkResult {
     std::vector<int> M, conv;
 };
 
+// Class terms of the fill are staged for at most this many bytes of points at once.
+constexpr int64_t FILL_G_BYTES = 4ll << 30;
+
 // Fill np points into ctx->wZ (row-major, one nbf^2 block per point).
 void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int np, int mixed,
                 ChunkResult &res) {
@@ -549,9 +555,20 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     }
     DBuf<FillParams> dfp(np);
     dfp.upload(fp.data(), np, st);
+    // class terms of up to npg points at a time (9 complex per class and point)
+    const int64_t gstride = 9 * sh->nufp;
+    const int64_t gbytes = gstride * (int64_t)sizeof(cd);
+    int npg = (int)std::max<int64_t>(1, std::min<int64_t>(np, FILL_G_BYTES / gbytes));
+    ctx->wG.ensure((size_t)gstride * npg);
+    const int mag = sh->cls == 'M' ? 1 : 0;
     stage_begin(2, st);
-    launch_zero(ctx->wZ.p, nb2 * np, st);
-    launch_fill(sh->dev(), dfp.p, np, sh->rho_r.p, sh->rinv.p, sh->cls == 'M' ? 1 : 0, st);
+    for (int p0 = 0; p0 < np; p0 += npg) {
+        const int n = std::min(npg, np - p0);
+        launch_fill_classes(sh->dev(), dfp.p + p0, n, sh->rho_r.p, sh->rinv.p, mag, ctx->wG.p,
+                            gstride, st);
+        launch_fill_gather(sh->dev(), dfp.p + p0, n, sh->bf_face.p, sh->bf_loc.p, sh->fpcls.p,
+                           ctx->wG.p, gstride, sh->has_fr ? 1 : 0, mag, st);
+    }
     stage_end(2, st);
     sync(ctx);
 }
@@ -904,13 +921,19 @@ void upload_sheet(pssfss_ctx *ctx, const pssfss_sheet_desc *d, pssfss_sheet *sh)
         }
         for (int64_t i = 0; i < d->nufp; ++i)
             if (off[i + 1] <= off[i]) throw ArgError("sheet: empty ufp class");
+        std::vector<int32_t> fpc(nm);
+        for (int64_t i = 0; i < d->nufp; ++i)
+            for (int64_t j = off[i]; j < off[i + 1]; ++j) fpc[mem[j]] = (int32_t)i;
         sh->uoff.upload(off.data(), off.size(), ctx->st);
         sh->umem.upload(mem.data(), mem.size(), ctx->st);
+        sh->fpcls.upload(fpc.data(), fpc.size(), ctx->st);
     } else {
         if (d->nufp != nf * nf) throw ArgError("sheet: nufp must be nface^2 without classes");
         sh->identity = true;
         sh->nufp = nf * nf;
     }
+    sh->has_fr = false;
+    for (double v : fr) if (v != 0) sh->has_fr = true;
     hipStream_t st = ctx->st;
     sh->fxs.upload(fxs.data(), fxs.size(), st);
     sh->fxm.upload(fxm.data(), fxm.size(), st);

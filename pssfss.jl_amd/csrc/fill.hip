@@ -1,8 +1,8 @@
-// MoM matrix fill for gfx950: one lane per unique-face-pair class.  Each lane
-// interpolates the two spectral tables at the 7 quadrature points (zint), combines
-// them with the cached spatial integrals and the analytic singular terms, and
-// adds the <= 9 edge-pair contributions of every member face pair of the class
-// into Z with FP64 atomics.  fillz: src/FillZY.jl:50-245; filly: :273-457;
+// MoM matrix fill for gfx950 in two phases without atomics.  Phase 1: one lane
+// per unique-face-pair class interpolates the two spectral tables at the 7
+// quadrature points (zint), combines them with the cached spatial integrals and
+// the analytic singular terms, and stores the 9 flag-free local edge-pair terms.
+// Phase 2: one lane per Z entry gathers the terms of its <= 4 face pairs.  fillz: src/FillZY.jl:50-245; filly: :273-457;
 // zint: src/Zint.jl:218-245; Sigma interpolation: src/PGF.jl:392-443.
 #include "internal.hpp"
 
@@ -58,18 +58,21 @@ __device__ cd sigma_interp(const cd *tab, int M, const double *b1, const double 
     return sm;
 }
 
-__device__ __forceinline__ void atomic_add_cd(cd *p, cd v) {
-    unsafeAtomicAdd(&p->x, v.x);
-    unsafeAtomicAdd(&p->y, v.y);
-}
-
+// Phase 1: one lane per (class, point).  The contribution of local edge pair
+// (iml of the match face, isl of the source face) to Z is
+//     conj(h_m) h_s w[iml][isl]
+// where h = +-ff[cell code] is the Floquet/sign flag of each basis-function half
+// and w is flag-free and the same for every member of a translation class.  The
+// nine w are written to plane (3 iml + isl) of G, coalesced in the class index.
 template <bool MAGNETIC>
-__global__ __launch_bounds__(256) void k_fill(SheetDev sh, const FillParams *fps,
-                                              const double *rho_r, const double *rinv_) {
+__global__ __launch_bounds__(256) void k_fill_classes(SheetDev sh, const FillParams *fps,
+                                                      const double *rho_r, const double *rinv_,
+                                                      cd *G0, int64_t gstride) {
 #pragma clang fp contract(off)
     const int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (iu >= sh.nufp) return;
     const FillParams &P = fps[blockIdx.y];
+    cd *G = G0 + (int64_t)blockIdx.y * gstride;
     const double EPS0 = 1.0 / (MU0 * (C0 * C0));
     int64_t ifm, ifs;
     class_faces(sh, iu, ifm, ifs);
@@ -81,18 +84,6 @@ __global__ __launch_bounds__(256) void k_fill(SheetDev sh, const FillParams *fps
             rs[i][0] = a[2 * i]; rs[i][1] = a[2 * i + 1];
             rm[i][0] = b[2 * i]; rm[i][1] = b[2 * i + 1];
         }
-    }
-    const double rs32x = rs[2][0] - rs[1][0], rs32y = rs[2][1] - rs[1][1];
-    const double rs12x = rs[0][0] - rs[1][0], rs12y = rs[0][1] - rs[1][1];
-    const double area = 0.5 * (rs32x * rs12y - rs32y * rs12x);
-    const double area48 = 48 * fabs(area);
-    const bool self_tri = (ifm == ifs);
-    double ls[3] = {0, 0, 0};
-    if (!MAGNETIC && self_tri) {
-        double r13x = rs[0][0] - rs[2][0], r13y = rs[0][1] - rs[2][1];
-        ls[0] = sqrt(rs32x * rs32x + rs32y * rs32y);
-        ls[1] = sqrt(r13x * r13x + r13y * r13y);
-        ls[2] = sqrt(rs12x * rs12x + rs12y * rs12y);
     }
     const double rmcx = ((rm[0][0] + rm[1][0]) + rm[2][0]) / 3;
     const double rmcy = ((rm[0][1] + rm[1][1]) + rm[2][1]) / 3;
@@ -121,80 +112,99 @@ __global__ __launch_bounds__(256) void k_fill(SheetDev sh, const FillParams *fps
         rc2[i][0] = 0.5 * (rmcx - rm[i][0]);
         rc2[i][1] = 0.5 * (rmcy - rm[i][1]);
     }
-    // Edge-independent parts of A_i and Phi_i.
+    // Edge-independent parts of A_i and Phi_i (src/FillZY.jl:183-200, :385-400).
     const double u = P.u;
     cd phi_core;
     if (!MAGNETIC) phi_core = ((FOURPI * I2) + u * (ri + J)) + (P.d3 / u) * K;
     else phi_core = ((P.I2fact * I2) + u * (ri + J)) + P.KPfact * K;
     const cd iw = MAGNETIC ? cd{0.0, -P.omega} : cd{0.0, P.omega};
-    const cd psf_scale = P.psf_base * (1.0 / ((TWOPI * P.omega) * EPS0));
-    const int64_t nmem = sh.identity ? 1 : sh.uoff[iu + 1] - sh.uoff[iu];
-    const int64_t nbf = sh.nbf;
-    for (int64_t im_ = 0; im_ < nmem; ++im_) {
-        int64_t fm = ifm, fs = ifs;
-        if (!sh.identity) {
-            const int64_t fp = sh.umem[sh.uoff[iu] + im_];
-            fm = fp % sh.nface;
-            fs = fp / sh.nface;
+    cd Phi, asf;
+    if (!MAGNETIC) {
+        Phi = (P.psf_base * (1.0 / ((TWOPI * P.omega) * EPS0))) * phi_core;
+        asf = (MU0 / FOURPI) * P.mut;
+    } else {
+        Phi = P.psf_base * phi_core;
+        asf = (-EPS0 / PI_) * P.epsb;
+    }
+#pragma unroll
+    for (int isl = 0; isl < 3; ++isl) {
+        const double singx = rrx + ((u * ri) * (rmcx - rs[isl][0]));
+        const double singy = rry + ((u * ri) * (rmcy - rs[isl][1]));
+        const double sing[2] = {singx, singy};
+        cd Ai[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            cd I1i = ((rs[0][d] * I1x + rs[1][d] * I1e) + rs[2][d] * I1z) - rs[isl][d] * I1;
+            cd Ji = ((rs[0][d] * Jx + rs[1][d] * Je) + rs[2][d] * Jz) - rs[isl][d] * J;
+            cd Ki = ((rs[0][d] * Kx + rs[1][d] * Ke) + rs[2][d] * Kz) - rs[isl][d] * K;
+            if (!MAGNETIC)
+                Ai[d] = asf * ((((FOURPI * I1i) + sing[d]) + u * Ji) + (P.c3 / u) * Ki);
+            else
+                Ai[d] = asf * ((((P.I1fact * I1i) + sing[d]) + u * Ji) + P.KFfact * Ki);
         }
-        for (int isl = 0; isl < 3; ++isl) {
-            const int sbf = sh.fbf[3 * fs + isl];
-            if (sbf < 0) continue;
-            const cd ffs = P.ff[sh.feci[3 * fs + isl]];
-            const cd sflag = sh.fsg[3 * fs + isl] > 0 ? ffs : -ffs;
-            const double singx = rrx + ((u * ri) * (rmcx - rs[isl][0]));
-            const double singy = rry + ((u * ri) * (rmcy - rs[isl][1]));
-            cd I1i[2], Ji[2], Ki[2];
 #pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                I1i[d] = ((rs[0][d] * I1x + rs[1][d] * I1e) + rs[2][d] * I1z) - rs[isl][d] * I1;
-                Ji[d] = ((rs[0][d] * Jx + rs[1][d] * Je) + rs[2][d] * Jz) - rs[isl][d] * J;
-                Ki[d] = ((rs[0][d] * Kx + rs[1][d] * Ke) + rs[2][d] * Kz) - rs[isl][d] * K;
-            }
-            const double sing[2] = {singx, singy};
-            cd Ai[2], Phi;
-            if (!MAGNETIC) {
-                cd asf = ((MU0 / FOURPI) * sflag) * P.mut;
-                cd psf = psf_scale * sflag;
-#pragma unroll
-                for (int d = 0; d < 2; ++d)
-                    Ai[d] = asf * ((((FOURPI * I1i[d]) + sing[d]) + u * Ji[d]) + (P.c3 / u) * Ki[d]);
-                Phi = psf * phi_core;
-            } else {
-                cd fsf = ((-EPS0 / PI_) * P.epsb) * sflag;
-                cd ysf = P.psf_base * sflag;
-#pragma unroll
-                for (int d = 0; d < 2; ++d)
-                    Ai[d] = fsf * ((((P.I1fact * I1i[d]) + sing[d]) + u * Ji[d]) + P.KFfact * Ki[d]);
-                Phi = ysf * phi_core;
-            }
-            const int se = sh.fedge[3 * fs + isl];
-            for (int iml = 0; iml < 3; ++iml) {
-                const int mbf = sh.fbf[3 * fm + iml];
-                if (mbf < 0) continue;
-                const cd ffm = conj(P.ff[sh.feci[3 * fm + iml]]);
-                const cd mflag = sh.fsg[3 * fm + iml] > 0 ? ffm : -ffm;
-                const cd dot = rc2[iml][0] * Ai[0] + rc2[iml][1] * Ai[1];
-                cd zc = mflag * (iw * dot - Phi);
-                if (!MAGNETIC && self_tri && sh.fr[fm] != 0) {
-                    static const int nxt[3] = {1, 2, 0}, prv[3] = {2, 0, 1};
-                    static const int third[3][3] = {{-1, 2, 1}, {2, -1, 0}, {1, 0, -1}};
-                    if (sh.fedge[3 * fm + iml] == se) {
-                        double zl = sh.fr[fs] / area48 *
-                                    (3 * (ls[nxt[isl]] * ls[nxt[isl]] + ls[prv[isl]] * ls[prv[isl]]) -
-                                     ls[isl] * ls[isl]);
-                        zc = zc + zl;
-                    } else {
-                        int t3 = third[isl][iml];
-                        cd zlc = (((sh.fr[fs] / area48) * sflag) * mflag) *
-                                 ((ls[isl] * ls[isl] + ls[iml] * ls[iml]) - 3 * ls[t3] * ls[t3]);
-                        zc = zc + zlc;
-                    }
-                }
-                atomic_add_cd(P.Z + (int64_t)mbf * nbf + (int64_t)sbf, zc);  // row-major Z[m][s]
-            }
+        for (int iml = 0; iml < 3; ++iml) {
+            const cd dot = rc2[iml][0] * Ai[0] + rc2[iml][1] * Ai[1];
+            G[(int64_t)(3 * iml + isl) * nu + iu] = iw * dot - Phi;
         }
     }
+}
+
+// Phase 2: one lane per Z entry (point, row m, column s).  Z[m][s] sums the
+// contributions of the (<= 4) face pairs formed by the two halves of m and the
+// two halves of s (the scatter of src/FillZY.jl:201-240 turned into a gather),
+// plus the surface-resistance term of self-triangle pairs (src/FillZY.jl:221-231).
+template <bool MAGNETIC>
+__global__ __launch_bounds__(256) void k_fill_gather(SheetDev sh, const FillParams *fps,
+                                                     const int32_t *bf_face,
+                                                     const int32_t *bf_loc,
+                                                     const int32_t *fpcls, const cd *G0,
+                                                     int64_t gstride, int has_fr) {
+#pragma clang fp contract(off)
+    const int64_t nbf = sh.nbf;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nbf * nbf) return;
+    const FillParams &P = fps[blockIdx.y];
+    const cd *G = G0 + (int64_t)blockIdx.y * gstride;
+    const int64_t m = e / nbf, s = e - m * nbf;
+    const int64_t nu = sh.nufp, nf = sh.nface;
+    cd acc = {0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int fm = bf_face[2 * m + a], lm = bf_loc[2 * m + a];
+        const cd ffm = conj(P.ff[sh.feci[3 * fm + lm]]);
+        const cd hm = sh.fsg[3 * fm + lm] > 0 ? ffm : -ffm;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int fs = bf_face[2 * s + b], ls = bf_loc[2 * s + b];
+            const cd ffs = P.ff[sh.feci[3 * fs + ls]];
+            const cd hs = sh.fsg[3 * fs + ls] > 0 ? ffs : -ffs;
+            const int64_t cls = sh.identity ? (int64_t)fm * nf + fs
+                                            : (int64_t)fpcls[(int64_t)fm + nf * fs];
+            cd zc = hm * (hs * G[(int64_t)(3 * lm + ls) * nu + cls]);
+            if (!MAGNETIC && has_fr && fm == fs && sh.fr[fm] != 0) {
+                const double *v = sh.fxm + 6 * fs;
+                const double r32x = v[4] - v[2], r32y = v[5] - v[3];
+                const double r12x = v[0] - v[2], r12y = v[1] - v[3];
+                const double r13x = v[0] - v[4], r13y = v[1] - v[5];
+                const double area48 = 48 * fabs(0.5 * (r32x * r12y - r32y * r12x));
+                double l2[3];
+                l2[0] = r32x * r32x + r32y * r32y;
+                l2[1] = r13x * r13x + r13y * r13y;
+                l2[2] = r12x * r12x + r12y * r12y;
+                const double q = sh.fr[fs] / area48;
+                if (lm == ls) {
+                    const int nx = ls == 2 ? 0 : ls + 1, pv = ls == 0 ? 2 : ls - 1;
+                    zc = zc + q * (3 * (l2[nx] + l2[pv]) - l2[ls]);
+                } else {
+                    const int t3 = 3 - lm - ls;
+                    zc = zc + ((q * hs) * hm) * ((l2[ls] + l2[lm]) - 3 * l2[t3]);
+                }
+            }
+            acc += zc;
+        }
+    }
+    P.Z[e] = acc;     // row-major Z[m][s]
 }
 
 __global__ void k_zero(cd *p, int64_t n) {
@@ -213,12 +223,29 @@ void launch_zero(cd *p, int64_t n, hipStream_t st) {
     HIPCHECK(hipGetLastError());
 }
 
-void launch_fill(const SheetDev &sh, const FillParams *d_fp, int npts, const double *rho_r,
-                 const double *rinv, int magnetic, hipStream_t st) {
+void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
+                         const double *rho_r, const double *rinv, int magnetic, cd *G,
+                         int64_t gstride, hipStream_t st) {
     dim3 grid((unsigned)((sh.nufp + 255) / 256), (unsigned)npts);
     if (magnetic)
-        hipLaunchKernelGGL(k_fill<true>, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv);
+        hipLaunchKernelGGL(k_fill_classes<true>, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv,
+                           G, gstride);
     else
-        hipLaunchKernelGGL(k_fill<false>, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv);
+        hipLaunchKernelGGL(k_fill_classes<false>, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv,
+                           G, gstride);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_fill_gather(const SheetDev &sh, const FillParams *d_fp, int npts,
+                        const int32_t *bf_face, const int32_t *bf_loc, const int32_t *fpcls,
+                        const cd *G, int64_t gstride, int has_fr, int magnetic, hipStream_t st) {
+    const int64_t n2 = sh.nbf * sh.nbf;
+    dim3 grid((unsigned)((n2 + 255) / 256), (unsigned)npts);
+    if (magnetic)
+        hipLaunchKernelGGL(k_fill_gather<true>, grid, dim3(256), 0, st, sh, d_fp, bf_face, bf_loc,
+                           fpcls, G, gstride, has_fr);
+    else
+        hipLaunchKernelGGL(k_fill_gather<false>, grid, dim3(256), 0, st, sh, d_fp, bf_face,
+                           bf_loc, fpcls, G, gstride, has_fr);
     HIPCHECK(hipGetLastError());
 }

@@ -140,8 +140,14 @@ void launch_singular(const SheetDev &sh, int64_t nufp, double ulocal, double *rh
 
 // ---- fill.hip
 void launch_zero(cd *p, int64_t n, hipStream_t st);
-void launch_fill(const SheetDev &sh, const FillParams *d_fp, int npts, const double *rho_r,
-                 const double *rinv, int magnetic, hipStream_t st);
+// Phase 1: G[p] holds 9 planes of nufp class terms (plane 3*iml + isl).
+void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
+                         const double *rho_r, const double *rinv, int magnetic, cd *G,
+                         int64_t gstride, hipStream_t st);
+// Phase 2: Z of each point from G (fpcls: class of face pair ifm + nface*ifs, CSR only).
+void launch_fill_gather(const SheetDev &sh, const FillParams *d_fp, int npts,
+                        const int32_t *bf_face, const int32_t *bf_loc, const int32_t *fpcls,
+                        const cd *G, int64_t gstride, int has_fr, int magnetic, hipStream_t st);
 
 // ---- linalg.hip
 void zgemm_batched(int m, int n, int k, cd alpha, const cd *A, int64_t lda, int64_t sA,
