@@ -7,75 +7,119 @@
 // pivot of largest |re|+|im| (izamax), applied to whole rows, so P*A = L*U with
 // the usual packed L\U storage -- the lu! / ldiv! pair of src/PSSFSS.jl:401,439.
 #include "internal.hpp"
+#include "mfma.hpp"
 
 namespace {
 
 constexpr int NB = 64;  // panel width
 
 // ------------------------------------------------------------------ GEMM (NN)
-// C = alpha*A*B + beta*C, column-major, batched by blockIdx.z.
+// C = alpha*A*B + beta*C, column-major complex, batched by blockIdx.z, on the
+// matrix cores.  A complex product is a real product with K doubled: with
+// (re, im) interleaved along k, Re C = [Ar, -Ai] [Br; Bi] and
+// Im C = [Ai, Ar] [Br; Bi], so B feeds the 16x16x4 MFMA in its memory layout
+// and each 4-deep real step covers two complex k.  64x64 tile per workgroup,
+// 32x32 per wavefront, BK = 16 complex k per LDS stage, double-buffered.
 template <class C>
-__global__ __launch_bounds__(256) void k_gemm_nn(int m, int n, int k, C alpha,
-                                                 const C *__restrict__ A, int64_t lda,
-                                                 int64_t sA, const C *__restrict__ B,
-                                                 int64_t ldb, int64_t sB, C beta, C *Cm,
-                                                 int64_t ldc, int64_t sC) {
+__global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
+                                                   const C *__restrict__ A, int64_t lda,
+                                                   int64_t sA, const C *__restrict__ B,
+                                                   int64_t ldb, int64_t sB, C beta, C *Cm,
+                                                   int64_t ldc, int64_t sC) {
+    typedef typename CTraits<C>::R R;
+    typedef MfmaT<R> MT;
+    typedef typename MT::V V;
     constexpr int BM = 64, BN = 64, BK = 16;
-    __shared__ C As[BK][BM];
-    __shared__ C Bs[BK][BN];
+    __shared__ C As[2][BK][BM];
+    __shared__ C Bs[2][BK][BN];
     const int64_t bz = blockIdx.z;
     A += bz * sA;
     B += bz * sB;
     Cm += bz * sC;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
     const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
-    C acc[4][4];
+    const C zero = CTraits<C>::zero();
+    V acc[2][2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = CTraits<C>::zero();
-    for (int k0 = 0; k0 < k; k0 += BK) {
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int t = tid; t < BM * BK; t += 256) {
-            int i = t % BM, kk = t / BM;
-            As[kk][i] = (m0 + i < m && k0 + kk < k) ? A[(m0 + i) + lda * (k0 + kk)]
-                                                    : CTraits<C>::zero();
+            for (int c = 0; c < 2; ++c) acc[a][b][c] = V{0, 0, 0, 0};
+    C ra[4], rb[4];
+#define GEMM_LOAD(K0)                                                                          \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                           \
+        const int t = tid + 256 * q;                                                           \
+        const int i = t % BM, kk = t / BM;                                                     \
+        ra[q] = zero;                                                                          \
+        if (m0 + i < m && (K0) + kk < k) ra[q] = A[(m0 + i) + lda * ((K0) + kk)];              \
+        const int kb = t % BK, j = t / BK;                                                     \
+        rb[q] = zero;                                                                          \
+        if ((K0) + kb < k && n0 + j < n) rb[q] = B[((K0) + kb) + ldb * (n0 + j)];              \
+    }
+#define GEMM_STORE(BUF)                                                                        \
+    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                           \
+        const int t = tid + 256 * q;                                                           \
+        As[BUF][t / BM][t % BM] = ra[q];                                                       \
+        Bs[BUF][t % BK][t / BK] = rb[q];                                                       \
+    }
+    const int nk = (k + BK - 1) / BK;
+    GEMM_LOAD(0)
+    GEMM_STORE(0)
+    __syncthreads();
+    const int c = (lane >> 4) & 1;        // re / im half of the doubled k
+    const int kh = lane >> 5;             // which of the two complex k of a step
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) { GEMM_LOAD((kt + 1) * BK) }
+#pragma unroll
+        for (int s2 = 0; s2 < BK / 2; ++s2) {
+            const int kc = 2 * s2 + kh;
+            R are[2], aim[2], bop[2];
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti) {
+                const C a = As[buf][kc][32 * wm + 16 * ti + (lane & 15)];
+                are[ti] = c == 0 ? a.x : -a.y;
+                aim[ti] = c == 0 ? a.y : a.x;
+            }
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                const C b = Bs[buf][kc][32 * wn + 16 * tj + (lane & 15)];
+                bop[tj] = c == 0 ? b.x : b.y;
+            }
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj) {
+                    acc[ti][tj][0] = MT::mma(are[ti], bop[tj], acc[ti][tj][0]);
+                    acc[ti][tj][1] = MT::mma(aim[ti], bop[tj], acc[ti][tj][1]);
+                }
         }
-#pragma unroll
-        for (int t = tid; t < BK * BN; t += 256) {
-            int kk = t % BK, j = t / BK;
-            Bs[kk][j] = (k0 + kk < k && n0 + j < n) ? B[(k0 + kk) + ldb * (n0 + j)]
-                                                    : CTraits<C>::zero();
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; ++kk) {
-            C a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) cfma(acc[i][j], a[i], b[j]);
-        }
+        if (kt + 1 < nk) { GEMM_STORE(buf ^ 1) }
         __syncthreads();
     }
+#undef GEMM_LOAD
+#undef GEMM_STORE
+    const bool use_beta = !(beta.x == 0 && beta.y == 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        int64_t r = m0 + tx + 16 * i;
-        if (r >= m) continue;
+    for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            int64_t c = n0 + ty + 16 * j;
-            if (c >= n) continue;
-            C *p = Cm + r + ldc * c;
-            C v = alpha * acc[i][j];
-            if (!(beta.x == 0 && beta.y == 0)) v += beta * (*p);
-            *p = v;
-        }
-    }
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t i = m0 + 32 * wm + 16 * ti + MT::row(lane, e);
+                const int64_t j = n0 + 32 * wn + 16 * tj + (lane & 15);
+                if (i < m && j < n) {
+                    C v;
+                    v.x = acc[ti][tj][0][e];
+                    v.y = acc[ti][tj][1][e];
+                    C *p = Cm + i + ldc * j;
+                    C o = alpha * v;
+                    if (use_beta) o += beta * (*p);
+                    *p = o;
+                }
+            }
 }
 
 // ------------------------------------------------------------------- LU panel
@@ -621,7 +665,7 @@ void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, c
             hipStream_t st) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
     dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
-    hipLaunchKernelGGL(k_gemm_nn<C>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA, B, ldb,
+    hipLaunchKernelGGL(k_gemm_mfma<C>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA, B, ldb,
                        sB, beta, Cm, ldc, sC);
     HIPCHECK(hipGetLastError());
 }

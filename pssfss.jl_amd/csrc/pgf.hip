@@ -8,6 +8,7 @@
 #include <type_traits>
 
 #include "internal.hpp"
+#include "mfma.hpp"
 
 namespace {
 
@@ -644,26 +645,6 @@ __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, dou
 // break test of src/PGF.jl:86-103 at every ring end.  Tiles are double-buffered
 // in LDS and the next phase tile is prefetched into registers, one barrier per
 // 8-point chunk.
-typedef float mf4 __attribute__((ext_vector_type(4)));
-typedef double md4 __attribute__((ext_vector_type(4)));
-
-template <class T> struct MfmaT;
-template <> struct MfmaT<float> {
-    typedef mf4 V;
-    static __device__ __forceinline__ V mma(float a, float b, V c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    // row (within the 16-row tile) of element e of lane l
-    static __device__ __forceinline__ int row(int l, int e) { return 4 * (l >> 4) + e; }
-};
-template <> struct MfmaT<double> {
-    typedef md4 V;
-    static __device__ __forceinline__ V mma(double a, double b, V c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int l, int e) { return (l >> 4) + 4 * e; }
-};
-
 template <class T>
 __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
                                                  double us1x, double us1y, double us2x,
