@@ -166,8 +166,10 @@ int pssfss_cascade_batch(pssfss_ctx *ctx, int nbatch, int n1, int n2, int n3,
  * launch of a stage.  Stages: 0 spatial sums, 1 spectral tables, 2 fill,
  * 3 LU panels + interchanges + U12 solves, 4 LU trailing GEMM updates,
  * 5 triangular solves and refinement, 6 transforms, RHS and projection.
- * flops[] holds the algorithmic flop count of stage 4 (8 m n k per complex
- * GEMM update) and of the whole LU (8/3 n^3 per system) in flops[7]. */
+ * flops[] holds algorithmic flop counts: stage 0, 8 per lattice point that the
+ * reference's per-(class, quadrature point, phasing) ring rule sums (counted by
+ * the many-phasing kernels, 24 or more phasings per launch); stage 4, 8 m n k
+ * per complex GEMM update; flops[7], 8/3 n^3 per LU (whole factorisation). */
 typedef struct {
     double ms[8];
     double flops[8];

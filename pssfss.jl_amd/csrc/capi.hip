@@ -395,14 +395,23 @@ void compute_jk(pssfss_ctx *ctx, pssfss_sheet *sh, double ul, const std::vector<
     dpsi.upload(psis.data(), psis.size(), ctx->st);
     const int64_t stride = 6 * sh->nufp;
     const int per = 1024;      // phasings per launch (grid y covers them in tiles)
+    // with stats on: lattice points of the reference's sums, 8 flops each (J and K,
+    // real term times complex phase)
+    DBuf<unsigned long long> cnt(g_stage_rec ? 2 : 0);    // 16 bytes: cleared as one cd
+    if (g_stage_rec) launch_zero((cd *)cnt.p, 1, ctx->st);
     for (int g = 0; g < nk; g += per) {
         int nf = std::min(per, nk - g);
         stage_begin(0, ctx->st);
         launch_jk(sh->dev(), sh->nufp, ul, us1, us2, dpsi.p + 2 * g, nf, dst + g * stride, stride,
-                  mixed, ctx->st);
+                  mixed, ctx->st, cnt.p);
         stage_end(0, ctx->st);
     }
     sync(ctx);
+    if (g_stage_rec) {
+        unsigned long long npts = 0;
+        HIPCHECK(hipMemcpy(&npts, cnt.p, sizeof(npts), hipMemcpyDeviceToHost));
+        stage_flops(0, 8.0 * (double)npts);
+    }
 }
 
 bool key_matches(const pssfss_sheet *sh, bool valid, double k1, double k2, double ku,

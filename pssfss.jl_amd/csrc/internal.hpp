@@ -132,9 +132,11 @@ void launch_dft_matrix(int M, cd *W, hipStream_t st);
 void launch_phase_ghost(int npts, const int *d_M, const double *d_psi, const int *d_mag,
                         const double *d_area, const cd *F, int64_t f_stride, cd *tab,
                         int64_t tab_stride, int Mmax, hipStream_t st);
+// total (optional, device): adds the lattice points the reference's per-pair
+// ring rule sums (tiled / MFMA paths only; untouched for fewer phasings).
 void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us1,
                const double *us2, const double *d_psi, int nf, cd *jk, int64_t jk_stride,
-               int mixed, hipStream_t st);
+               int mixed, hipStream_t st, unsigned long long *total = nullptr);
 void launch_singular(const SheetDev &sh, int64_t nufp, double ulocal, double *rho_r,
                      double *rinv, hipStream_t st);
 

@@ -450,6 +450,33 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
     }
 }
 
+// Lattice points summed by the pairs of one lane up to ring r: 8 per ring index.
+__device__ __forceinline__ unsigned long long lattice_points(int r) {
+    return 4ull * (unsigned long long)r * (unsigned long long)(r + 1);
+}
+
+// Sum of a per-lane count over the workgroup, written to out[linear block id].
+__device__ void block_count(unsigned long long v, unsigned long long *out) {
+    __shared__ unsigned long long part[16];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < nw; ++i) t += part[i];
+        out[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+__global__ void k_sum_u64(const unsigned long long *a, int64_t n, unsigned long long *total) {
+    unsigned long long v = 0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v += a[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(total, v);
+}
+
 // Tiled form of the spatial sums for many phasings: a workgroup takes 64 rows
 // (9 classes x 7 quadrature points) and 16*TN phasings.  Lattice points are
 // visited ring by ring in chunks of 8 (ring r has 8r points), the lattice term
@@ -464,7 +491,8 @@ template <class T, int TN>
 __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, double ul,
                                                   double us1x, double us1y, double us2x,
                                                   double us2y, const T *phase, int ldp, int nf,
-                                                  cd *jk, int64_t jk_stride) {
+                                                  cd *jk, int64_t jk_stride,
+                                                  unsigned long long *bcnt) {
 #pragma clang fp contract(off)
     constexpr int FT = 16 * TN;              // phasings per workgroup
     __shared__ T Aj[8][64], Ak[8][64];
@@ -516,8 +544,12 @@ __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, dou
         }
     }
     const unsigned all = (1u << (4 * TN)) - 1u;
+    const unsigned live0 = ~done & all;
+    unsigned long long npts = 0;     // lattice points of the reference's sums
+    int rlast = 0;
     for (int r = 1; r <= JK_RINGMAX; ++r) {
         if (!__syncthreads_or(done != all)) break;
+        rlast = r;
         T jr[4][TN][2], kr[4][TN][2];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -596,10 +628,17 @@ __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, dou
                     ks[i][j] += b;
                     if (r > 5) {
                         double c1 = cabs(a / js[i][j]), c2 = cabs(b / ks[i][j]);
-                        if ((c1 > c2 ? c1 : c2) < 1e-8) done |= bit;
+                        if ((c1 > c2 ? c1 : c2) < 1e-8) {
+                            done |= bit;
+                            npts += lattice_points(r);
+                        }
                     }
                 }
             }
+    }
+    if (bcnt) {
+        npts += (unsigned long long)__popc(live0 & ~done) * lattice_points(rlast);
+        block_count(npts, bcnt);
     }
     // Combine the 7 quadrature points of each class (src/Zint.jl:117-124).
 #pragma unroll
@@ -649,7 +688,8 @@ template <class T>
 __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
                                                  double us1x, double us1y, double us2x,
                                                  double us2y, const T *phase, int ldp,
-                                                 int nf, cd *jk, int64_t jk_stride) {
+                                                 int nf, cd *jk, int64_t jk_stride,
+                                                 unsigned long long *bcnt) {
 #pragma clang fp contract(off)
     typedef MfmaT<T> MT;
     typedef typename MT::V V;
@@ -739,8 +779,12 @@ __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, doub
         pmn[0][pkk][1] = n;
     }
     int buf = 0;
+    const unsigned live0 = ~done & 0xffu;
+    unsigned long long npts = 0;     // lattice points of the reference's sums
+    int rlast = 0;
     for (int r = 1; r <= JK_RINGMAX; ++r) {
         if (!__syncthreads_or(done != 0xffu)) break;
+        rlast = r;
         V aJ[2][2], aK[2][2];          // [g][cos/sin]
 #pragma unroll
         for (int g = 0; g < 2; ++g)
@@ -840,10 +884,17 @@ __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, doub
                             double c1 = cabs(a / js[g][e]), c2 = cabs(b / ks[g][e]);
                             conv = (c1 > c2 ? c1 : c2) < 1e-8;
                         }
-                        if (conv) done |= bit;
+                        if (conv) {
+                            done |= bit;
+                            npts += lattice_points(r);
+                        }
                     }
                 }
             }
+    }
+    if (bcnt) {
+        npts += (unsigned long long)__popc(live0 & ~done) * lattice_points(rlast);
+        block_count(npts, bcnt);
     }
     // J then K: per-row sums through LDS, combined over the 7 quadrature points
     // (src/Zint.jl:117-124)
@@ -1003,7 +1054,7 @@ void launch_phase_ghost(int npts, const int *d_M, const double *d_psi, const int
 // phasings: the tiled kernel, whose lattice-term tile is shared by 32 phasings.
 void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us1,
                const double *us2, const double *d_psi, int nf, cd *jk, int64_t jk_stride,
-               int mixed, hipStream_t st) {
+               int mixed, hipStream_t st, unsigned long long *total) {
     if (nf < 1) return;
     if (nf < JK_TILED_MIN) {
         const int per = mixed ? 16 : 8;
@@ -1032,25 +1083,35 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
     const int ldp = (nf + FT - 1) / FT * FT;
     const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
     const size_t elt = mixed ? sizeof(float) : sizeof(double);
+    dim3 pg((unsigned)((E + 255) / 256), (unsigned)ldp);
+    dim3 grid((unsigned)((nufp + 8) / 9), (unsigned)(ldp / FT));
+    const int64_t nblk = (int64_t)grid.x * grid.y;
+    const size_t phase_bytes = (size_t)E * 2 * ldp * elt;
+    const size_t cnt_bytes = total ? (size_t)nblk * sizeof(unsigned long long) : 0;
     void *ph = nullptr;
-    if (hipMallocAsync(&ph, (size_t)E * 2 * ldp * elt, st) != hipSuccess) {
+    if (hipMallocAsync(&ph, phase_bytes + cnt_bytes, st) != hipSuccess) {
         (void)hipGetLastError();
         throw HipError("hipMallocAsync of the lattice phase table failed");
     }
-    dim3 pg((unsigned)((E + 255) / 256), (unsigned)ldp);
-    dim3 grid((unsigned)((nufp + 8) / 9), (unsigned)(ldp / FT));
+    unsigned long long *bcnt =
+        total ? reinterpret_cast<unsigned long long *>((char *)ph + phase_bytes) : nullptr;
     if (mixed) {
         hipLaunchKernelGGL(k_phase_table<float>, pg, dim3(256), 0, st, d_psi, nf, ldp, (float *)ph);
         hipLaunchKernelGGL(k_jk_mfma<float>, grid, dim3(256), 0, st, sh, nufp, ulocal, us1[0],
-                           us1[1], us2[0], us2[1], (const float *)ph, ldp, nf, jk, jk_stride);
+                           us1[1], us2[0], us2[1], (const float *)ph, ldp, nf, jk, jk_stride,
+                           bcnt);
     } else {
         hipLaunchKernelGGL(k_phase_table<double>, pg, dim3(256), 0, st, d_psi, nf, ldp,
                            (double *)ph);
         hipLaunchKernelGGL((k_jk_tiled<double, 2>), grid, dim3(256), 0, st, sh, nufp, ulocal,
                            us1[0], us1[1], us2[0], us2[1], (const double *)ph, ldp, nf, jk,
-                           jk_stride);
+                           jk_stride, bcnt);
     }
     HIPCHECK(hipGetLastError());
+    if (total) {
+        hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(1024), 0, st, bcnt, nblk, total);
+        HIPCHECK(hipGetLastError());
+    }
     HIPCHECK(hipFreeAsync(ph, st));
 }
 
