@@ -264,6 +264,7 @@ def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False):
             extras.append(ex)
             return g
         gsm = setup.cascade_point(k0, sheet_solver)
-        results.append({"gsm": gsm, "FGHz": point.fghz, "steer": point.steer, "beta00": b00,
-                        "extras": extras})
+        res = pk.outputs.make_result(gsm, point.steer, b00, point.fghz, setup.layers)
+        res.extras = extras
+        results.append(res)
     return results

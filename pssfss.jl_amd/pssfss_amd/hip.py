@@ -13,6 +13,7 @@ from typing import List
 import numpy as np
 
 from . import gsm as G
+from . import outputs as O
 from . import modes as Md
 from .constants import TWOPI
 
@@ -426,7 +427,7 @@ class SheetSolver:
                         return solved[(_ip, ig)].copy()
                 raise RuntimeError("sheet GSM missing from the batched solve")
             g = su.cascade_point(pt.k0, solver)
-            results.append({"gsm": g, "FGHz": pt.fghz, "steer": pt.steer, "beta00": pt.beta00})
+            results.append(O.make_result(g, pt.steer, pt.beta00, pt.fghz, su.layers))
         stats = {"npoints": len(live), "infos": infos_all}
         return results, stats
 
