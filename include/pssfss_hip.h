@@ -124,6 +124,24 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
                            const pssfss_point *pts, pssfss_gsm *gsms, int precision,
                            pssfss_point_info *infos);
 
+/* ---- sensitivities (no counterpart in the reference) ---------------------------
+ * Derivatives of the sheet GSM with respect to mesh geometry, for gradient-based
+ * design through a ChainRulesCore.rrule on the S-parameter output.  For each of
+ * npts points (as pssfss_sheet_gsm_batch) and nobj weight sets, returns for every
+ * direction k
+ *     grad[p][o][k] = sum_e wbar[p][o][e] * d S_e / d t_k   (complex)
+ * where e runs over the entries of the Gblock GSM packed as in pssfss_gsm
+ * (s11, s12, s21, s22 blocks, column-major complex, (n1+n2)^2 entries) and the
+ * mesh nodes move as rho(t) = rho + t_k vtan[k] (vtan: nparam blocks of nnode
+ * (x, y) pairs, sheet units).  An objective f(S) enters through wbar = df/dS (holomorphic
+ * derivative); Re(grad) is then the derivative of a real objective.  The forward
+ * solve is repeated in FP64; adjoint systems use the same LU factors; the fill is
+ * recomputed with derivatives and contracted, Z's derivative is never stored.
+ * 'J' sheets without surface resistance.  grad: npts x nobj x nparam complex. */
+int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
+                         const pssfss_point *pts, int nparam, const double *vtan, int nobj,
+                         const double *wbar, double *grad);
+
 /* ---- single stages, each a parity boundary ---------------------------------- */
 /* fillz (src/FillZY.jl:50-245) / filly (src/FillZY.jl:273-457): the nbf x nbf
  * impedance (admittance) matrix, written to zmat (complex, column-major). */

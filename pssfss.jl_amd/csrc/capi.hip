@@ -108,6 +108,7 @@ struct pssfss_sheet {
     DBuf<double> fxs, fxm, fr;
     DBuf<int32_t> fbf, fsg, feci, fedge, bf_face, bf_loc;
     DBuf<int32_t> fpcls;        // class of face pair ifm + nface*ifs (translation classes)
+    std::vector<int32_t> hfv;   // 3*nface 0-based face vertices (adjoint directions)
     bool has_fr = false;        // any face with nonzero sheet resistance
     DBuf<int64_t> uoff, umem;
     // J/K cache with the key semantics of src/FillZY.jl:97-108
@@ -536,7 +537,8 @@ constexpr int64_t FILL_G_BYTES = 4ll << 30;
 
 // Fill np points into ctx->wZ (row-major, one nbf^2 block per point).
 void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int np, int mixed,
-                ChunkResult &res) {
+                ChunkResult &res, TableSet *keep_tabs = nullptr,
+                std::vector<FillParams> *keep_fp = nullptr) {
     hipStream_t st = ctx->st;
     const double ul = pts[0].u / sh->upm;
     for (int p = 1; p < np; ++p)
@@ -580,6 +582,8 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     }
     stage_end(2, st);
     sync(ctx);
+    if (keep_fp) *keep_fp = fp;
+    if (keep_tabs) *keep_tabs = std::move(tabs);
 }
 
 // Solve the np systems in ctx->wZ for the right-hand sides in ctx->wB (nbf x ncol
@@ -710,21 +714,14 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
 }
 
 // One chunk of the sheet GSM pipeline.
-void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfss_gsm *gsms,
-               int np, int precision, pssfss_point_info *infos) {
-    hipStream_t st = ctx->st;
-    const bool mag = sh->cls == 'M';
-    const int n1 = pts[0].nmodes[0], n2 = pts[0].nmodes[1];
-    for (int p = 1; p < np; ++p)
-        if (pts[p].nmodes[0] != n1 || pts[p].nmodes[1] != n2)
-            throw ArgError("batched points must have equal mode counts");
+// Fourier-transform jobs of the points' end-region modes with the reuse rule of
+// src/PSSFSS.jl:407-423; jobof[p][col] is the job of column col (region-1 modes
+// first, then region-N modes).
+void make_jobs(const pssfss_point *pts, int np, int n1, int n2, std::vector<BfftJob> &jobs,
+               std::vector<std::vector<int>> &jobof) {
     const int ncol = n1 + n2;
-    const int64_t n = sh->nbf;
-    ChunkResult cr;
-    fill_chunk(ctx, sh, pts, np, precision == PSSFSS_MIXED, cr);
-    // Fourier-transform jobs with the reuse rule of src/PSSFSS.jl:407-423.
-    std::vector<BfftJob> jobs;
-    std::vector<std::vector<int>> jobof(np, std::vector<int>(ncol));
+    jobs.clear();
+    jobof.assign(np, std::vector<int>(ncol));
     for (int p = 0; p < np; ++p) {
         const pssfss_point &pt = pts[p];
         const int nm[2] = {n1, n2};
@@ -749,6 +746,41 @@ void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfs
                 jobs.push_back(J);
             }
     }
+}
+
+// Source vector (7.39) of right-hand-side column col ('J' sheets: v c acf t;
+// 'M' sheets: i c Y (z x t)).
+void col_source(const pssfss_point &pt, int n1, int col, bool mag, cd &s0, cd &s1) {
+    int sr = col < n1 ? 0 : 1, qp = col < n1 ? col : col - n1;
+    cd vinc = lay(pt.vinc, qp + pt.nmax * sr);
+    cd c = lay(pt.c[sr], qp);
+    double tx = pt.tvec[sr][2 * qp], ty = pt.tvec[sr][2 * qp + 1];
+    if (!mag) {
+        cd f = (vinc * c) * pt.acf[sr];
+        s0 = f * tx;
+        s1 = f * ty;
+    } else {
+        cd f = (vinc * c) * lay(pt.Y[sr], qp);
+        s0 = f * (-ty);
+        s1 = f * tx;
+    }
+}
+
+void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfss_gsm *gsms,
+               int np, int precision, pssfss_point_info *infos) {
+    hipStream_t st = ctx->st;
+    const bool mag = sh->cls == 'M';
+    const int n1 = pts[0].nmodes[0], n2 = pts[0].nmodes[1];
+    for (int p = 1; p < np; ++p)
+        if (pts[p].nmodes[0] != n1 || pts[p].nmodes[1] != n2)
+            throw ArgError("batched points must have equal mode counts");
+    const int ncol = n1 + n2;
+    const int64_t n = sh->nbf;
+    ChunkResult cr;
+    fill_chunk(ctx, sh, pts, np, precision == PSSFSS_MIXED, cr);
+    std::vector<BfftJob> jobs;
+    std::vector<std::vector<int>> jobof;
+    make_jobs(pts, np, n1, n2, jobs, jobof);
     const int njobs = (int)jobs.size();
     DBuf<BfftJob> djobs(njobs);
     djobs.upload(jobs.data(), njobs, st);
@@ -761,20 +793,8 @@ void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfs
     for (int p = 0; p < np; ++p) {
         const pssfss_point &pt = pts[p];
         for (int col = 0; col < ncol; ++col) {
-            int sr = col < n1 ? 0 : 1, qp = col < n1 ? col : col - n1;
-            cd vinc = lay(pt.vinc, qp + pt.nmax * sr);
-            cd c = lay(pt.c[sr], qp);
-            double tx = pt.tvec[sr][2 * qp], ty = pt.tvec[sr][2 * qp + 1];
             cd s0, s1;
-            if (!mag) {
-                cd f = (vinc * c) * pt.acf[sr];
-                s0 = f * tx;
-                s1 = f * ty;
-            } else {
-                cd f = (vinc * c) * lay(pt.Y[sr], qp);
-                s0 = f * (-ty);
-                s1 = f * tx;
-            }
+            col_source(pt, n1, col, mag, s0, s1);
             coljob[p * ncol + col] = jobof[p][col];
             colsrc[2 * (p * ncol + col)] = s0;
             colsrc[2 * (p * ncol + col) + 1] = s1;
@@ -839,6 +859,185 @@ void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfs
     }
 }
 
+// Sensitivities of the sheet GSM along nparam geometry directions, FP64, 'J'
+// sheets.  With W the weight of each GSM entry (ocol = observed mode, col = source
+// mode), S_scat[q,qp] = -kappa_q g_q^T I_qp (eq. 6.18), Z^T lambda_q = g_q and
+//   sum W dS = sum W (-kappa_q) (dg_q^T I_qp + lambda_q^T dV_qp)
+//            + sum_{m,s} dZ_ms sum_qp A[m,qp] I[s,qp],  A = Lambda (W kappa).
+// The first line uses the transforms' derivatives (host sums), the second the
+// class terms' derivatives contracted on the device (dZ is never formed).
+void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int np,
+                   int nparam, const double *vtan, int nobj, const double *wbar,
+                   double *grad) {
+    hipStream_t st = ctx->st;
+    const int n1 = pts[0].nmodes[0], n2 = pts[0].nmodes[1];
+    for (int p = 1; p < np; ++p)
+        if (pts[p].nmodes[0] != n1 || pts[p].nmodes[1] != n2)
+            throw ArgError("batched points must have equal mode counts");
+    const int ncol = n1 + n2;
+    const int64_t n = sh->nbf, nf = sh->nface;
+    const double ul = pts[0].u / sh->upm;
+    // forward: fill (keeping tables), transforms, RHS, FP64 LU + solves
+    ChunkResult cr;
+    TableSet tabs;
+    std::vector<FillParams> fp;
+    fill_chunk(ctx, sh, pts, np, 0, cr, &tabs, &fp);
+    std::vector<BfftJob> jobs;
+    std::vector<std::vector<int>> jobof;
+    make_jobs(pts, np, n1, n2, jobs, jobof);
+    const int njobs = (int)jobs.size();
+    DBuf<BfftJob> djobs(njobs);
+    djobs.upload(jobs.data(), njobs, st);
+    ctx->wft.ensure((size_t)njobs * n * 2);
+    launch_bfft(sh->dev(), sh->bf_face.p, sh->bf_loc.p, djobs.p, njobs, ctx->wft.p, st);
+    std::vector<int32_t> coljob(np * ncol);
+    std::vector<cd> colsrc(2 * np * ncol);
+    std::vector<double> colt(2 * np * ncol);
+    for (int p = 0; p < np; ++p)
+        for (int col = 0; col < ncol; ++col) {
+            col_source(pts[p], n1, col, false, colsrc[2 * (p * ncol + col)],
+                       colsrc[2 * (p * ncol + col) + 1]);
+            coljob[p * ncol + col] = jobof[p][col];
+            int orr = col < n1 ? 0 : 1, q = col < n1 ? col : col - n1;
+            colt[2 * (p * ncol + col)] = pts[p].tvec[orr][2 * q];
+            colt[2 * (p * ncol + col) + 1] = pts[p].tvec[orr][2 * q + 1];
+        }
+    DBuf<int32_t> dcj(coljob.size());
+    DBuf<cd> dcs(colsrc.size());
+    DBuf<double> dct(colt.size());
+    dcj.upload(coljob.data(), coljob.size(), st);
+    dcs.upload(colsrc.data(), colsrc.size(), st);
+    dct.upload(colt.data(), colt.size(), st);
+    ctx->wB.ensure((size_t)n * ncol * np);
+    launch_rhs(n, np * ncol, ctx->wft.p, dcj.p, dcs.p, ctx->wB.p, n, st);
+    solve_chunk(ctx, n, ncol, np, PSSFSS_FP64, nullptr);
+    // adjoint: observation vectors, transposed solves with the same factors
+    DBuf<cd> lam((size_t)n * ncol * np), tmp((size_t)n * ncol * np);
+    launch_obs(n, np * ncol, ctx->wft.p, dcj.p, dct.p, lam.p, n, st);
+    lu_solve_transposed(n, ncol, np, ctx->wZ.p, n * n, ctx->wperm.p, n, lam.p, n * ncol, tmp.p,
+                        st);
+    std::vector<cd> hX((size_t)n * ncol * np), hL((size_t)n * ncol * np);
+    d2h(ctx, hX.data(), ctx->wB.p, hX.size());
+    d2h(ctx, hL.data(), lam.p, hL.size());
+    sync(ctx);
+    // weights W kappa and A = Lambda (W kappa) per point and objective
+    const double area = sh->s1[0] * sh->s2[1] - sh->s1[1] * sh->s2[0];
+    const double area_m2 = fabs(area) / (sh->upm * sh->upm);
+    const int nm[2] = {n1, n2};
+    std::vector<cd> Wk((size_t)np * nobj * ncol * ncol), hA((size_t)np * nobj * n * ncol);
+    std::vector<cd> kap((size_t)np * ncol);
+    const int64_t wsz = (int64_t)ncol * ncol;
+    for (int p = 0; p < np; ++p) {
+        const pssfss_point &pt = pts[p];
+        for (int oc = 0; oc < ncol; ++oc) {
+            int orr = oc < n1 ? 0 : 1, q = oc < n1 ? oc : oc - n1;
+            cd tl = lay(pt.tlgf, q + pt.nmax * orr);
+            cd cq = lay(pt.c[orr], q);
+            kap[p * ncol + oc] = tl / ((cq * pt.acf[orr]) * area_m2);
+        }
+        for (int o = 0; o < nobj; ++o) {
+            const double *wb = wbar + 2 * ((int64_t)p * nobj + o) * wsz;
+            // blocks s11, s12, s21, s22, column-major, as pssfss_gsm
+            int64_t off = 0;
+            for (int orr = 0; orr < 2; ++orr)
+                for (int sr = 0; sr < 2; ++sr) {
+                    for (int qp = 0; qp < nm[sr]; ++qp)
+                        for (int q = 0; q < nm[orr]; ++q) {
+                            int oc = orr == 0 ? q : n1 + q, c = sr == 0 ? qp : n1 + qp;
+                            cd w = lay(wb, (int)(off + q + (int64_t)nm[orr] * qp));
+                            Wk[((size_t)p * nobj + o) * wsz + oc * ncol + c] =
+                                w * kap[p * ncol + oc];
+                        }
+                    off += (int64_t)nm[orr] * nm[sr];
+                }
+            cd *Ao = &hA[((size_t)p * nobj + o) * n * ncol];
+            const cd *L = &hL[(size_t)p * n * ncol];
+            const cd *Wo = &Wk[((size_t)p * nobj + o) * wsz];
+            for (int c = 0; c < ncol; ++c)
+                for (int64_t m = 0; m < n; ++m) {
+                    cd a = {0, 0};
+                    for (int oc = 0; oc < ncol; ++oc) a += L[m + n * oc] * Wo[oc * ncol + c];
+                    Ao[m + n * c] = a;
+                }
+        }
+    }
+    DBuf<cd> dA(hA.size());
+    dA.upload(hA.data(), hA.size(), st);
+    DBuf<FillParams> dfp(np);
+    dfp.upload(fp.data(), np, st);
+    // per direction: tangent spatial sums / singular terms, class contraction,
+    // transform derivatives
+    const int64_t nu = sh->nufp;
+    DBuf<double> fts(6 * nf), ftm(6 * nf);
+    DBuf<cdd> jkd((size_t)6 * nu);
+    DBuf<dd> rrd((size_t)2 * nu), rid((size_t)nu);
+    DBuf<cd> part((size_t)((nu + 255) / 256) * nobj), zout((size_t)nobj * np);
+    DBuf<cd> dft((size_t)njobs * n * 2);
+    std::vector<double> hts(6 * nf), htm(6 * nf);
+    std::vector<cd> hz((size_t)nobj * np), hdft((size_t)njobs * n * 2);
+    const double us1[2] = {ul * sh->s1[0], ul * sh->s1[1]};
+    const double us2[2] = {ul * sh->s2[0], ul * sh->s2[1]};
+    for (int k = 0; k < nparam; ++k) {
+        const double *vt = vtan + (size_t)k * 2 * sh->nnode;
+        for (int64_t f = 0; f < nf; ++f)
+            for (int a = 0; a < 3; ++a) {
+                const int v = sh->hfv[3 * f + a];
+                hts[6 * f + 2 * a] = vt[2 * v];
+                hts[6 * f + 2 * a + 1] = vt[2 * v + 1];
+                htm[6 * f + 2 * a] = vt[2 * v] / sh->upm;
+                htm[6 * f + 2 * a + 1] = vt[2 * v + 1] / sh->upm;
+            }
+        fts.upload(hts.data(), hts.size(), st);
+        ftm.upload(htm.data(), htm.size(), st);
+        launch_singular_tan(sh->dev(), ul, fts.p, rrd.p, rid.p, st);
+        double kpsi1 = NAN, kpsi2 = NAN;
+        for (int p = 0; p < np; ++p) {
+            if (!(pts[p].psi1 == kpsi1 && pts[p].psi2 == kpsi2)) {
+                kpsi1 = pts[p].psi1;
+                kpsi2 = pts[p].psi2;
+                launch_jk_tan(sh->dev(), ul, us1, us2, kpsi1, kpsi2, fts.p, jkd.p, st);
+            }
+            launch_class_tan(sh->dev(), dfp.p + p, jkd.p, rrd.p, rid.p, ftm.p,
+                             dA.p + (size_t)p * nobj * n * ncol, n * ncol,
+                             ctx->wB.p + (size_t)p * n * ncol, ncol, nobj, part.p,
+                             zout.p + (size_t)p * nobj, st);
+        }
+        launch_bfft_tan(sh->dev(), sh->bf_face.p, sh->bf_loc.p, djobs.p, njobs, ftm.p, dft.p, st);
+        d2h(ctx, hz.data(), zout.p, hz.size());
+        d2h(ctx, hdft.data(), dft.p, hdft.size());
+        sync(ctx);
+        for (int p = 0; p < np; ++p) {
+            const cd *X = &hX[(size_t)p * n * ncol];
+            const cd *L = &hL[(size_t)p * n * ncol];
+            // explicit terms per (observed, source) pair
+            std::vector<cd> ex((size_t)wsz);
+            for (int oc = 0; oc < ncol; ++oc) {
+                const cd *dg = &hdft[(size_t)jobof[p][oc] * n * 2];
+                const double tx = colt[2 * (p * ncol + oc)], ty = colt[2 * (p * ncol + oc) + 1];
+                for (int c = 0; c < ncol; ++c) {
+                    const cd *dv = &hdft[(size_t)jobof[p][c] * n * 2];
+                    const cd s0 = colsrc[2 * (p * ncol + c)], s1 = colsrc[2 * (p * ncol + c) + 1];
+                    cd t = {0, 0};
+                    for (int64_t m = 0; m < n; ++m) {
+                        cd g = tx * dg[2 * m] + ty * dg[2 * m + 1];
+                        cd v = conj(dv[2 * m]) * s0 + conj(dv[2 * m + 1]) * s1;
+                        t += g * X[m + n * c] + L[m + n * oc] * v;
+                    }
+                    ex[oc * ncol + c] = t;
+                }
+            }
+            for (int o = 0; o < nobj; ++o) {
+                const cd *Wo = &Wk[((size_t)p * nobj + o) * wsz];
+                cd gsum = hz[(size_t)p * nobj + o];
+                for (int64_t e = 0; e < wsz; ++e) gsum = gsum - Wo[e] * ex[e];
+                double *gp = grad + 2 * (((size_t)p * nobj + o) * nparam + k);
+                gp[0] = gsum.x;
+                gp[1] = gsum.y;
+            }
+        }
+    }
+}
+
 int chunk_size(pssfss_ctx *ctx, const pssfss_sheet *sh, int npts, int precision) {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
@@ -873,10 +1072,12 @@ void upload_sheet(pssfss_ctx *ctx, const pssfss_sheet_desc *d, pssfss_sheet *sh)
     }
     std::vector<double> fxs(6 * nf), fxm(6 * nf), fr(nf, 0.0);
     std::vector<int32_t> fbf(3 * nf), fsg(3 * nf), feci(3 * nf), fedge(3 * nf);
+    sh->hfv.assign(3 * nf, 0);
     for (int64_t f = 0; f < nf; ++f) {
         for (int i = 0; i < 3; ++i) {
             int64_t v = d->fv[3 * f + i] - 1;
             if (v < 0 || v >= nn) throw ArgError("sheet: fv index out of range");
+            sh->hfv[3 * f + i] = (int32_t)v;
             fxs[6 * f + 2 * i] = d->rho[2 * v];
             fxs[6 * f + 2 * i + 1] = d->rho[2 * v + 1];
             fxm[6 * f + 2 * i] = d->rho[2 * v] / sh->upm;
@@ -1039,6 +1240,29 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
         for (int p0 = 0; p0 < npts; p0 += c) {
             int np = std::min(c, npts - p0);
             gsm_chunk(ctx, sheet, pts + p0, gsms + p0, np, precision, infos ? infos + p0 : nullptr);
+        }
+    });
+}
+
+int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
+                         const pssfss_point *pts, int nparam, const double *vtan, int nobj,
+                         const double *wbar, double *grad) {
+    return guarded(ctx, [&] {
+        if (!sheet || !pts || !vtan || !wbar || !grad || npts < 0 || nparam < 0)
+            throw ArgError("null argument");
+        if (nobj < 1 || nobj > 4) throw ArgError("adjoint: 1..4 objectives");
+        if (sheet->cls != 'J') throw ArgError("adjoint: electric-current ('J') sheets only");
+        if (sheet->has_fr) throw ArgError("adjoint: resistive faces are not supported");
+        HIPCHECK(hipSetDevice(ctx->device));
+        for (int p = 0; p < npts; ++p) check_point(sheet, &pts[p], true);
+        if (npts == 0 || nparam == 0) return;
+        int c = chunk_size(ctx, sheet, npts, PSSFSS_FP64);
+        const int ncol = pts[0].nmodes[0] + pts[0].nmodes[1];
+        for (int p0 = 0; p0 < npts; p0 += c) {
+            int np = std::min(c, npts - p0);
+            adjoint_chunk(ctx, sheet, pts + p0, np, nparam, vtan, nobj,
+                          wbar + 2 * (size_t)p0 * nobj * ncol * ncol,
+                          grad + 2 * (size_t)p0 * nobj * nparam);
         }
     });
 }

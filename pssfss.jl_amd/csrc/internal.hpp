@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "cplx.hpp"
+#include "dual.hpp"
 
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -266,3 +267,26 @@ inline void stage_end(int s, hipStream_t st, double flops = 0.0) {
 inline void stage_flops(int s, double flops) {
     if (g_stage_rec) g_stage_rec->flops[s] += flops;
 }
+
+// ---- adjoint.hip (forward-mode directions through the fill; transposed solves)
+// ftan_s / ftan_m: 6 per face, derivative of the face's vertex coordinates along
+// the direction (sheet units / metres).
+void launch_jk_tan(const SheetDev &sh, double ul, const double *us1, const double *us2,
+                   double psi1, double psi2, const double *ftan_s, cdd *out, hipStream_t st);
+void launch_singular_tan(const SheetDev &sh, double ul, const double *ftan_s, dd *rho_r,
+                         dd *rinv, hipStream_t st);
+// out[o] = sum_{m,s} dZ_ms sum_c A[o*sAo + m + nbf*c] X[s + nbf*c]; part: one slot per
+// (256-class block, objective).
+void launch_class_tan(const SheetDev &sh, const FillParams *d_fp, const cdd *jkd,
+                      const dd *rho_r, const dd *rinv, const double *ftan_m, const cd *A,
+                      int64_t sAo, const cd *X, int ncol, int nobj, cd *part, cd *out,
+                      hipStream_t st);
+void launch_bfft_tan(const SheetDev &sh, const int32_t *bf_face, const int32_t *bf_loc,
+                     const BfftJob *d_jobs, int njobs, const double *ftan_m, cd *dft,
+                     hipStream_t st);
+void launch_obs(int64_t nbf, int ncol, const cd *ft, const int32_t *d_col_job,
+                const double *d_col_t, cd *g, int64_t ld, hipStream_t st);
+// Z^T X = B with the factors of lu_factor_batched (perm from lu_make_perm), in place.
+void lu_solve_transposed(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
+                         const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
+                         hipStream_t st);

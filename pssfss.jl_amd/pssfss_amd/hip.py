@@ -90,6 +90,8 @@ def lib():
                                        C.POINTER(PointInfo)]
         L.pssfss_sheet_gsm_batch.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.POINTER(Gsm),
                                              C.c_int, C.POINTER(PointInfo)]
+        L.pssfss_sheet_adjoint.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.c_int, _dp,
+                                           C.c_int, _dp, _dp]
         L.pssfss_fill.argtypes = [vp, vp, C.POINTER(Point), _dp, C.POINTER(PointInfo)]
         L.pssfss_filljk.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.pssfss_modal_tables.argtypes = [vp, C.c_int, C.POINTER(Point), _dp, _dp, _dp,
@@ -115,7 +117,7 @@ EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssf
            "pssfss_sheet_upload", "pssfss_sheet_free", "pssfss_sheet_gsm",
            "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk", "pssfss_modal_tables",
            "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
-           "pssfss_stats_enable", "pssfss_stats_read")
+           "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint")
 
 
 def _d(a):
