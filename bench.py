@@ -11,7 +11,9 @@ device; meshing, setup_rwg and mode selection are outside the timed region.
 value = points/s of the mixed path (FP32 LU and smooth spatial sums, FP64
 refinement, singular terms and tables); the plain FP64 path is timed beside it
 and max|dS| between them is reported, as is max|dS| of both against the oracle
-on the CPU-baseline sample point.
+on the CPU-baseline sample point.  detail.two_layer_patch holds the
+differentiation workload: forward points/s in FP64 and mixed, and adjoint
+gradient vectors per second.
 """
 import argparse
 import json
@@ -44,6 +46,8 @@ def main():
     ap.add_argument("--workload", default="ring", choices=["ring", "patch"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--no-two-layer", action="store_true",
+                    help="skip the two-layer patch forward / adjoint measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -136,6 +140,9 @@ def main():
     extra["dominant_stage"] = max(stage_ms, key=stage_ms.get)
     extra["lu_tflops_per_s_whole_step"] = round(lu_flops / step_s / 1e12, 3)
 
+    if not args.no_two_layer:
+        extra["two_layer_patch"] = two_layer_patch(pk, hip, ctx)
+
     cpu = None
     if not args.no_cpu:
         cpu, errs = cpu_baseline(pk, strata, freqs, steer, results)
@@ -152,6 +159,47 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "detail": extra,
     }
     print(json.dumps(line), flush=True)
+
+
+def two_layer_patch(pk, hip, ctx):
+    """Corner-parameterised 8 mm patch (39 x 39 bilinear grid: 3042 faces, 4485
+    unknowns, every face pair its own class) in a 12 mm cell on eps_r 3.0 (1.0 mm)
+    + eps_r 1.1 foam (3.0 mm); 12 frequencies 8..12 GHz at normal incidence.
+    Forward points/s of the FP64 and mixed paths, and gradient vectors per second of
+    |S21|^2 co-polar TE and TM with respect to the 8 corner coordinates (one
+    gradient vector = 8 derivatives of one objective at one point)."""
+    import numpy as np
+    from pssfss_amd import adjoint as A
+    corners = [[2.0, 2.0], [10.0, 2.0], [10.0, 10.0], [2.0, 10.0]]
+    sh, vt = A.bilinear_patch(corners, 39, 39, 12.0, 12.0)
+    L = pk.Layer.make
+    strata = [L(), sh, L(width=1.0, units="mm", epsr=3.0), L(width=3.0, units="mm", epsr=1.1),
+              L()]
+    freqs = np.linspace(8.0, 12.0, 12)
+    steer = {"theta": [0.0], "phi": [0.0]}
+    setup = pk.analyze.prepare(strata, freqs, steer)
+    out = {"faces": sh.nface, "unknowns": int(setup.rwgdat[0].nbf), "points": len(freqs)}
+    res = {}
+    for p in ("fp64", "mixed"):
+        s = hip.SheetSolver(ctx, setup, p)
+        s.run()                                   # warm-up (J/K integrals, allocations)
+        t0 = time.perf_counter()
+        res[p], _ = s.run()
+        dt = time.perf_counter() - t0
+        out[f"points_per_s_{p}"] = round(len(freqs) / dt, 3)
+        s.close()
+        print(f"[bench] two-layer patch {p}: {dt:.3f} s for {len(freqs)} points", flush=True)
+    out["speedup_mixed_vs_fp64"] = round(out["points_per_s_mixed"] / out["points_per_s_fp64"], 3)
+    out["max_dS_mixed_vs_fp64"] = max_dS(res["mixed"], res["fp64"])
+    obj = [(2, 1, 1, 1), (2, 1, 2, 2)]
+    t0 = time.perf_counter()
+    grad, _ = A.gradients(strata, freqs, steer, vt, obj, ctx=ctx)
+    dt = time.perf_counter() - t0
+    out["adjoint_grads_per_s"] = round(grad.shape[0] * grad.shape[1] / dt, 3)
+    out["adjoint_s"] = round(dt, 3)
+    print(f"[bench] two-layer patch adjoint: {dt:.3f} s for {grad.shape[0]} points x "
+          f"{grad.shape[1]} objectives x {grad.shape[2]} parameters", flush=True)
+    return out
 
 
 def cpu_baseline(pk, strata, freqs, steer, gpu_results):
