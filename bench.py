@@ -26,6 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 for _sub in ("pssfss.jl_amd", "oracle"):
     sys.path.insert(0, os.path.join(ROOT, _sub))
 
+# HBM bytes per launch of the mixed spatial-sum kernel on this workload, from the
+# FETCH_SIZE / WRITE_SIZE passes in profiles/ring_on_slab_traffic_pmc.md (counters
+# calibrated with a 256 MiB copy kernel: FETCH x2.00, WRITE x1.00).
+JK_MFMA_TRAFFIC_BYTES = 2923.0 * 1024 * 2.0 + 43239678.1 * 1024 * 1.0
 PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 (vector = matrix) / FP32 matrix
 
 
@@ -126,7 +130,9 @@ def main():
     roof = {"bound": "mfma", "kernel": jk_kernel, "achieved": round(achieved, 3),
             "peak": PEAK[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK[head], 4),
             "avg_launch_ms": round(jk["ms"] / max(1, jk["launches"]), 3),
-            "flops_per_launch": jk["flops"] / max(1, jk["launches"]), "traffic": None}
+            "flops_per_launch": jk["flops"] / max(1, jk["launches"]),
+            "traffic": (JK_MFMA_TRAFFIC_BYTES if head == "mixed" and args.workload == "ring"
+                        else None)}
     g = st["lu_gemm"]
     g_ach = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
     extra["roofline_lu_gemm"] = {"kernel": "k_gemm_mfma (LU trailing update)",
