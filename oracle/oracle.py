@@ -242,12 +242,13 @@ def _approx(a, b):
 
 
 # ------------------------------------------------------------------- analyze
-def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False):
+def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False, dbmin=None):
     """The `_analyze` loop (src/PSSFSS.jl:199-334) with the oracle sheet solver.
 
     steering: dict with keys 'theta','phi' (degrees) or 'psi1','psi2' (degrees);
     the first key is the outer loop.  Returns a list of result dicts."""
-    setup = pk.analyze.prepare(strata, freqs, steering, mode_override=mode_override)
+    kw = {} if dbmin is None else {"dbmin": dbmin}
+    setup = pk.analyze.prepare(strata, freqs, steering, mode_override=mode_override, **kw)
     caches = [SheetCache() for _ in setup.sheets]
     results = []
     for point in setup.points():

@@ -185,7 +185,7 @@ def _unique_indices(v):
     return ui
 
 
-def prepare(strata, flist, steering, mode_override=None) -> Setup:
+def prepare(strata, flist, steering, mode_override=None, dbmin: float = DBMIN) -> Setup:
     """One-time setup of `_analyze` (src/PSSFSS.jl:114-142, 213-226)."""
     import copy
     layers = [copy.deepcopy(s) for s in strata if isinstance(s, Layer)]
@@ -208,7 +208,7 @@ def prepare(strata, flist, steering, mode_override=None) -> Setup:
     _check_inputs(layers)
     k0min, k0max = [TWOPI * 1e9 / C0 * f for f in (min(freqs), max(freqs))]
     gbls = G.choose_gblocks(layers, sheets, junc, k0min)
-    Md.choose_layer_modes(layers, sheets, junc, gbls, k0max, DBMIN, mode_override=mode_override)
+    Md.choose_layer_modes(layers, sheets, junc, gbls, k0max, dbmin, mode_override=mode_override)
     gbldup = _get_gbldup(gbls, layers, sheets, junc)
     usi = _unique_indices(sheets)
     rwgdat = [setup_rwg(sh) if sh.style != "NULL" else None for sh in sheets]
@@ -232,7 +232,7 @@ def _check_inputs(layers):
 
 
 def analyze(strata, flist, steering, precision: str = "fp64", mode_override=None,
-            ctx=None, return_stats: bool = False):
+            ctx=None, return_stats: bool = False, dbmin: float = DBMIN):
     """Analyze the structure over all steering values and frequencies on the GPU.
 
     Returns a list of result dicts {gsm, FGHz, steer, beta00} in the reference's
@@ -240,7 +240,7 @@ def analyze(strata, flist, steering, precision: str = "fp64", mode_override=None
     smooth sums, FP64 refinement)."""
     from .hip import SheetSolver, get_context
     ctx = ctx or get_context()
-    setup = prepare(strata, flist, steering, mode_override=mode_override)
+    setup = prepare(strata, flist, steering, mode_override=mode_override, dbmin=dbmin)
     solver = SheetSolver(ctx, setup, precision)
     results, stats = solver.run()
     return (results, stats) if return_stats else results

@@ -169,8 +169,20 @@ def unique_face_pairs(sheet: RWGSheet):
     d3 = cm - r3[:, None, :]
     data = np.concatenate([d1, d2, d3], axis=2).reshape(nface * nface, 6)
     keys = np.rint(data / q).astype(np.int64)
-    _, first_idx, inverse = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    # Classes of equal 6-component keys: a 64-bit mix of the key sorts fast; every
+    # class is then checked member by member against its first key, and the exact
+    # row-wise grouping is used if a collision ever merged different keys.
+    mult = np.array([0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F, 0x165667B19E3779F9,
+                     0xD6E8FEB86659FD93, 0xFF51AFD7ED558CCD, 0xC4CEB9FE1A85EC53], np.uint64)
+    with np.errstate(over="ignore"):
+        h = (keys.astype(np.uint64) * mult).sum(axis=1, dtype=np.uint64)
+        h ^= h >> np.uint64(31)
+    _, first_idx, inverse = np.unique(h, return_index=True, return_inverse=True)
     inverse = inverse.reshape(-1)
+    if not np.array_equal(keys, keys[first_idx[inverse]]):
+        _, first_idx, inverse = np.unique(keys, axis=0, return_index=True,
+                                          return_inverse=True)
+        inverse = inverse.reshape(-1)
     # Order classes by smallest member.
     order = np.argsort(first_idx, kind="stable")
     rank = np.empty_like(order)

@@ -120,6 +120,127 @@ def recttri(rhobl, rhotr, nx: int, ny: int) -> RWGSheet:
     return sh
 
 
+def combine(sh1: RWGSheet, sh2: RWGSheet, dup_coor: str, dup_coor_value: float) -> RWGSheet:
+    """Join two triangulations, merging the nodes and edges of sh2 that coincide
+    with sh1 on the line x (or y) = dup_coor_value (src/Sheets.jl:234-387).  sh2's
+    duplicate nodes/edges map onto sh1's; its other nodes/edges follow sh1's in
+    their original order."""
+    n1, n2 = sh1.rho.shape[0], sh2.rho.shape[0]
+    node_map = np.full(n2, -1, np.int64)
+    dup_nodes = []
+    if dup_coor != " ":
+        e0 = sh1.rho[sh1.e1[0]] - sh1.rho[sh1.e2[0]]
+        tol = 0.5e-4 * math.hypot(e0[0], e0[1])
+        axis = {"x": 0, "y": 1}[dup_coor]
+        for i in range(n2):
+            if abs(sh2.rho[i, axis] - dup_coor_value) < tol:
+                d = np.hypot(*(sh1.rho - sh2.rho[i]).T)
+                hit = np.nonzero(d < tol)[0]
+                if hit.size:
+                    node_map[i] = hit[0]
+                    dup_nodes.append(i)
+    dup_set = set(dup_nodes)
+    nxt = n1
+    for i in range(n2):
+        if i not in dup_set:
+            node_map[i] = nxt
+            nxt += 1
+    # edges of sh2 lying on the join line, matched with sh1's
+    edge_map = np.full(len(sh2.e1), -1, np.int64)
+    sh1_edges = {}
+    for j in range(len(sh1.e1)):
+        sh1_edges.setdefault((int(sh1.e1[j]), int(sh1.e2[j])), j)
+        sh1_edges.setdefault((int(sh1.e2[j]), int(sh1.e1[j])), j)
+    for i in range(len(sh2.e1)):
+        a, b = int(sh2.e1[i]), int(sh2.e2[i])
+        if a in dup_set and b in dup_set:
+            key = (int(node_map[a]), int(node_map[b]))
+            if key not in sh1_edges:
+                raise ValueError("Unable to find matching duplicate edge in combine")
+            edge_map[i] = sh1_edges[key]
+    ne = len(sh1.e1)
+    for i in range(len(sh2.e1)):
+        if edge_map[i] < 0:
+            edge_map[i] = ne
+            ne += 1
+    keep = np.nonzero(edge_map >= len(sh1.e1))[0]
+    sh = RWGSheet()
+    sh.units = sh1.units
+    keep_nodes = [i for i in range(n2) if i not in dup_set]
+    sh.rho = np.vstack([sh1.rho, sh2.rho[keep_nodes]])
+    sh.e1 = np.concatenate([sh1.e1, node_map[sh2.e1[keep]]])
+    sh.e2 = np.concatenate([sh1.e2, node_map[sh2.e2[keep]]])
+    sh.fv = np.hstack([sh1.fv, node_map[sh2.fv]])
+    sh.fe = np.hstack([sh1.fe, edge_map[sh2.fe]])
+    return sh
+
+
+def meander(*, a, b, h, w1, w2, ntri: int, units, orient=0.0, sheet_class="J", dx=0.0,
+            dy=0.0, rot=0.0, Rsheet=0.0, fufp=True) -> RWGSheet:
+    """Meanderline in an a x b cell built from ten structured rectangles
+    (src/Elements.jl:617-729)."""
+    for name, v in (("a", a), ("b", b), ("h", h), ("w1", w1), ("w2", w2), ("ntri", ntri)):
+        if not v > 0:
+            raise ValueError(f"{name} must be positive!")
+    if orient != 0 and a != b:
+        raise ValueError("Nonzero orient only allowed for square unit cell")
+    morient = orient % 360
+    if morient not in (0, 90, 180, 270):
+        raise ValueError("orient must be a multiple of 90")
+
+    def rnd(x):          # Julia round(Int, x): ties to even
+        return int(np.round(x))
+    t1 = w2 * (a + 2 * w1) / (2 * (h - 2 * w2) ** 2)
+    t2 = w1 / (h - 2 * w2)
+    fny2 = math.sqrt(ntri / (4 * (t1 + t2)))
+    ny2 = max(rnd(fny2), 1)
+    ny1 = max(rnd(fny2 * w2 / (h - 2 * w2)), 1)
+    nx1 = max(rnd(fny2 * (a - 2 * w1) / (4 * (h - 2 * w2))), 1)
+    nx2 = max(rnd(fny2 * w1 / (h - 2 * w2)), 1)
+    yoff = (b - h) / 2
+    Lx = (a / 2 - w1) / 2
+    Ly = h - 2 * w2
+    R = lambda bl, size, nx, ny: recttri(bl, np.asarray(bl) + np.asarray(size), nx, ny)
+    sh1 = R([0.0, yoff], [Lx, w2], nx1, ny1)                       # section 1
+    sh2 = R([Lx, yoff], [w1, w2], nx2, ny1)                        # section 3
+    sh3 = combine(sh1, sh2, "x", Lx)
+    bl = [Lx, yoff + w2]                                           # section 5
+    sh2 = combine(sh3, R(bl, [w1, Ly], nx2, ny2), "y", bl[1])
+    bl = [Lx, yoff + w2 + Ly]                                      # section 7
+    sh3 = combine(sh2, R(bl, [w1, w2], nx2, ny1), "y", bl[1])
+    bl = [Lx + w1, yoff + w2 + Ly]                                 # sections 9, 10
+    tr = [bl[0] + 2 * Lx, bl[1] + w2]
+    sh2 = combine(sh3, R(bl, [2 * Lx, w2], 2 * nx1, ny1), "x", bl[0])
+    bl = [tr[0], bl[1]]                                            # section 8
+    tr = [bl[0] + w1, bl[1] + w2]
+    sh3 = combine(sh2, R(bl, [w1, w2], nx2, ny1), "x", bl[0])
+    bl = [bl[0], yoff + w2]                                        # section 6
+    tr = [bl[0] + w1, bl[1] + Ly]
+    sh2 = combine(sh3, R(bl, [w1, Ly], nx2, ny2), "y", tr[1])
+    bl = [bl[0], yoff]                                             # section 4
+    tr = [bl[0] + w1, bl[1] + w2]
+    sh3 = combine(sh2, R(bl, [w1, w2], nx2, ny1), "y", tr[1])
+    bl = [tr[0], yoff]                                             # section 2
+    sh = combine(sh3, R(bl, [Lx, w2], nx1, ny1), "x", bl[0])
+    if morient != 0:
+        x, y = sh.rho[:, 0].copy(), sh.rho[:, 1].copy()
+        sh.rho = {90: np.stack([b - y, x], 1), 180: np.stack([a - x, b - y], 1),
+                  270: np.stack([y, a - x], 1)}[morient]
+    sh.fr = np.full(sh.nface, float(Rsheet))
+    sh.fufp = bool(fufp)
+    sh.sheet_class = sheet_class
+    sh.style = "meander"
+    sh.units = units
+    sh.s1 = np.array([float(a), 0.0])
+    sh.s2 = np.array([0.0, float(b)])
+    sh.beta1, sh.beta2 = s1s2_to_beta(sh.s1, sh.s2)
+    _rotate(sh, rot)
+    if dx != 0 or dy != 0:
+        sh.rho = sh.rho + np.array([dx, dy], float)
+    sh.xieta_check = True
+    return sh
+
+
 def _rotate(sh: RWGSheet, rot: float) -> None:
     if rot == 0:
         return

@@ -8,11 +8,15 @@ ring_on_slab: PEC annulus a = 3.0 mm, b = 3.6 mm in a 9 mm cell on a 0.787 mm
     51 frequencies 5..25 GHz.  Structured polar mesh (nr x nphi quads, 2 faces each).
 patch_two_layer: 12 mm cell, square patch on eps_r = 3.0 (1.0 mm) + eps_r = 1.1
     foam (3.0 mm); 12 frequencies 8..12 GHz at normal incidence.
+meander_polarizer: four PEC meanderlines (6 x 10 mm cell, 0.3 mm strips, meander
+    heights drawn from a seeded U(2, 3) mm), eps_r = 1.07 foam spacers 5 mm thick,
+    eps_r = 2.8 skins 0.25 mm thick; 101 frequencies 8..14 GHz at theta 0 and 40
+    deg, phi = 0.
 """
 import numpy as np
 
 from .layers import Layer
-from .sheets import rectstrip, annular_ring
+from .sheets import rectstrip, annular_ring, meander
 
 
 def patch_free_space(n: int = 14, nfreq: int = 21, sheet_class: str = "J"):
@@ -38,6 +42,21 @@ def patch_two_layer(n: int = 39, nfreq: int = 12, L: float = 8.0):
               Layer.make(width=3.0, units="mm", epsr=1.1), Layer.make()]
     freqs = np.linspace(8.0, 12.0, nfreq)
     steering = {"theta": [0.0], "phi": [0.0]}
+    return strata, freqs, steering
+
+
+def meander_polarizer(ntri: int = 8000, nfreq: int = 101, seed: int = 0, thetas=(0.0, 40.0)):
+    rng = np.random.default_rng(seed)
+    hs = rng.uniform(2.0, 3.0, 4)
+    sheets = [meander(a=6.0, b=10.0, h=float(h), w1=0.3, w2=0.3, ntri=ntri, units="mm")
+              for h in hs]
+    L = Layer.make
+    foam = lambda: L(width=5.0, units="mm", epsr=1.07)
+    skin = lambda: L(width=0.25, units="mm", epsr=2.8)
+    strata = [L(), skin(), sheets[0], foam(), sheets[1], foam(), sheets[2], foam(), sheets[3],
+              skin(), L()]
+    freqs = np.linspace(8.0, 14.0, nfreq)
+    steering = {"phi": [0.0], "theta": list(thetas)}
     return strata, freqs, steering
 
 

@@ -283,6 +283,27 @@ def test_analyze_ring_many_phasings(ctx, pk, orc):
     assert dm <= 1e-6
 
 
+@pytest.mark.parametrize("dbmin", [30.0, 60.0])
+def test_analyze_meander_polarizer(ctx, pk, orc, dbmin):
+    """Four cascaded meanderline sheets (translation classes, periodic basis
+    functions across the cell) on foam spacers and skins, theta 0 and 40 deg;
+    dbmin 30 (the reference's rule: 10 modes in the foam) and 60 (30 modes, 60
+    right-hand sides per interior sheet)."""
+    _say(f"analyze 4-sheet meander polarizer (ntri=120 per sheet), 2 freqs x theta 0/40, "
+         f"dbmin {dbmin}")
+    strata, freqs, steer = pk.workloads.meander_polarizer(ntri=120, nfreq=2)
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx, dbmin=dbmin)
+    ref = orc.analyze(pk, strata, freqs, steer, dbmin=dbmin)
+    assert len(res) == len(ref) == 4
+    d = _max_dS(res, ref)
+    print(f"  max |dS| fp64 = {d:.3e}")
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx, dbmin=dbmin)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
 def _grating(kP, nterms=30):
     """Collin's strip-grating closed form as used by test/full_test.jl:27-42."""
     sum1 = 1.3862943611198906

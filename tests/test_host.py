@@ -132,3 +132,57 @@ def test_struct_layout_matches_header(pk):
     assert ctypes.sizeof(hip.SheetDesc) == 192
     assert ctypes.sizeof(hip.PointInfo) == 32
     assert ctypes.sizeof(hip.Stats) == 192
+
+
+def test_recttri_and_combine_goldens(pk):
+    """test/RWGSheet_test.jl:5-80: exact nodes, edges and face maps (1-based there)."""
+    from pssfss_amd.sheets import recttri, combine
+    sh1 = recttri([0., 0.], [1., 1.], 1, 1)
+    assert sh1.rho.tolist() == [[0, 0], [1, 0], [0, 1], [1, 1]]
+    assert (sh1.e1 + 1).tolist() == [1, 3, 1, 2, 1] and (sh1.e2 + 1).tolist() == [2, 4, 3, 4, 4]
+    assert (sh1.fe + 1).T.tolist() == [[2, 3, 5], [4, 5, 1]]
+    assert (sh1.fv + 1).T.tolist() == [[1, 4, 3], [1, 2, 4]]
+    cases = [
+        ([1., 0.], [2., 1.], "x", 1.0, [[0, 0], [1, 0], [0, 1], [1, 1], [2, 0], [2, 1]],
+         [1, 3, 1, 2, 1, 2, 4, 5, 2], [2, 4, 3, 4, 4, 5, 6, 6, 6],
+         [[1, 4, 3], [1, 2, 4], [2, 6, 4], [2, 5, 6]], [[2, 3, 5], [4, 5, 1], [7, 4, 9], [8, 9, 6]]),
+        ([0., 1.], [1., 2.], "y", 1.0, [[0, 0], [1, 0], [0, 1], [1, 1], [0, 2], [1, 2]],
+         [1, 3, 1, 2, 1, 5, 3, 4, 3], [2, 4, 3, 4, 4, 6, 5, 6, 6],
+         [[1, 4, 3], [1, 2, 4], [3, 6, 5], [3, 4, 6]], [[2, 3, 5], [4, 5, 1], [6, 7, 9], [8, 9, 2]]),
+        ([1., 1.], [2., 2.], "y", 1.0, [[0, 0], [1, 0], [0, 1], [1, 1], [2, 1], [1, 2], [2, 2]],
+         [1, 3, 1, 2, 1, 4, 6, 4, 5, 4], [2, 4, 3, 4, 4, 5, 7, 6, 7, 7],
+         [[1, 4, 3], [1, 2, 4], [4, 7, 6], [4, 5, 7]],
+         [[2, 3, 5], [4, 5, 1], [7, 8, 10], [9, 10, 6]]),
+        ([2., 0.], [3., 1.], " ", float("nan"),
+         [[0, 0], [1, 0], [0, 1], [1, 1], [2, 0], [3, 0], [2, 1], [3, 1]],
+         [1, 3, 1, 2, 1, 5, 7, 5, 6, 5], [2, 4, 3, 4, 4, 6, 8, 7, 8, 8],
+         [[1, 4, 3], [1, 2, 4], [5, 8, 7], [5, 6, 8]],
+         [[2, 3, 5], [4, 5, 1], [7, 8, 10], [9, 10, 6]]),
+    ]
+    for bl, tr, dc, dv, rho, e1, e2, fv, fe in cases:
+        sh3 = combine(sh1, recttri(bl, tr, 1, 1), dc, dv)
+        assert sh3.rho.tolist() == rho
+        assert (sh3.e1 + 1).tolist() == e1 and (sh3.e2 + 1).tolist() == e2
+        assert (sh3.fv + 1).T.tolist() == fv and (sh3.fe + 1).T.tolist() == fe
+
+
+@pytest.mark.parametrize("ntri,h", [(200, 2.0), (800, 2.6), (8000, 3.0)])
+def test_meander_mesh(pk, ntri, h):
+    """Conforming single-piece mesh of exactly the meander's area
+    a w2 + 2 w1 (h - w2) (ten rectangles of src/Elements.jl:617-729), reaching
+    both x = 0 and x = a so its basis functions wrap across the cell."""
+    a, b, w1, w2 = 6.0, 10.0, 0.3, 0.3
+    sh = pk.sheets.meander(a=a, b=b, h=h, w1=w1, w2=w2, ntri=ntri, units="mm")
+    v = sh.rho[sh.fv]                       # (3, nface, 2)
+    area = 0.5 * np.abs((v[1, :, 0] - v[0, :, 0]) * (v[2, :, 1] - v[0, :, 1]) -
+                        (v[2, :, 0] - v[0, :, 0]) * (v[1, :, 1] - v[0, :, 1]))
+    assert abs(area.sum() - (a * w2 + 2 * w1 * (h - w2))) < 1e-9
+    assert np.all(area > 0)
+    uses = np.bincount(sh.fe.reshape(-1), minlength=sh.nedge)
+    assert set(uses.tolist()) <= {1, 2}
+    assert sh.rho.shape[0] - sh.nedge + sh.nface == 1          # one simply connected piece
+    assert 0.5 * ntri < sh.nface < 1.5 * ntri
+    assert abs(sh.rho[:, 0].min()) < 1e-12 and abs(sh.rho[:, 0].max() - a) < 1e-12
+    if ntri <= 800:
+        d = pk.setup_rwg(sh)
+        assert np.count_nonzero(d.eci > 0) > 0 and d.nbf > 0
