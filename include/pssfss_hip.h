@@ -108,6 +108,18 @@ int pssfss_device_info(pssfss_ctx *ctx, char *name, int name_len, int *num_cu,
 int pssfss_sheet_upload(pssfss_ctx *ctx, const pssfss_sheet_desc *desc, pssfss_sheet **out);
 int pssfss_sheet_free(pssfss_ctx *ctx, pssfss_sheet *sheet);
 
+/* Unique-face-pair classes of setup_rwg (src/RWG.jl:270-322) on the device, for
+ * the sheet upload's ufp_offsets / ufp_members.  Pair index m + nface*(n-1)
+ * (1-based faces m, n as the reference), stored 0-based: mn = m0 + nface*n0.
+ * rho: 2*nnode, fv: 3*nface (1-based, column-major as RWGSheet.fv).  On return
+ * *nufp classes; offsets (capacity nface^2 + 1) gets nufp + 1 CSR offsets,
+ * members (nface^2) the pairs of each class ascending, ufpm (nface^2, may be
+ * NULL) the 1-based class of each pair (RWGData.ufpm).  Classes are numbered by
+ * their smallest member, as the reference's loop numbers them. */
+int pssfss_face_pair_classes(pssfss_ctx *ctx, int64_t nnode, const double *rho,
+                             int64_t nface, const int64_t *fv, int64_t *nufp, int64_t *offsets,
+                             int64_t *members, int64_t *ufpm);
+
 /* ---- the hot path ------------------------------------------------------------ */
 /* Sheet GSM at one point: fill -> LU -> RHS -> solve -> extract.
  * Replaces calculate_jtype_gsm (src/PSSFSS.jl:362-460) and calculate_mtype_gsm

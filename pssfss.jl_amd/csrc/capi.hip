@@ -1267,6 +1267,39 @@ int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
     });
 }
 
+int pssfss_face_pair_classes(pssfss_ctx *ctx, int64_t nnode, const double *rho,
+                             int64_t nface, const int64_t *fv, int64_t *nufp, int64_t *offsets,
+                             int64_t *members, int64_t *ufpm) {
+    return guarded(ctx, [&] {
+        if (!rho || !fv || !nufp || !offsets || !members || nnode <= 0 || nface < 2)
+            throw ArgError("null argument or fewer than two faces");
+        HIPCHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->st;
+        for (int64_t i = 0; i < 3 * nface; ++i)
+            if (fv[i] < 1 || fv[i] > nnode) throw ArgError("fv index out of range");
+        // grouping radius of the reference: 1e-6 |c1 - c2| (first two centroids)
+        double c[2][2];
+        for (int f = 0; f < 2; ++f)
+            for (int d = 0; d < 2; ++d)
+                c[f][d] = ((rho[2 * (fv[3 * f] - 1) + d] + rho[2 * (fv[3 * f + 1] - 1) + d]) +
+                           rho[2 * (fv[3 * f + 2] - 1) + d]) / 3.0;
+        const double radius = 1e-6 * hypot(c[0][0] - c[1][0], c[0][1] - c[1][1]);
+        const int64_t npair = nface * nface;
+        DBuf<double> drho(2 * nnode);
+        DBuf<int64_t> dfv(3 * nface), dufpm(npair), dmem(npair);
+        DBuf<uint64_t> doff(npair + 1);
+        drho.upload(rho, 2 * nnode, st);
+        dfv.upload(fv, 3 * nface, st);
+        const int64_t n = face_pair_classes_dev(nnode, drho.p, nface, dfv.p, radius, dufpm.p,
+                                                dmem.p, doff.p, st);
+        *nufp = n;
+        HIPCHECK(hipMemcpyAsync(offsets, doff.p, 8 * (n + 1), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(members, dmem.p, 8 * npair, hipMemcpyDeviceToHost, st));
+        if (ufpm) HIPCHECK(hipMemcpyAsync(ufpm, dufpm.p, 8 * npair, hipMemcpyDeviceToHost, st));
+        sync(ctx);
+    });
+}
+
 int pssfss_sheet_gsm(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt,
                      pssfss_gsm *gsm, int precision, pssfss_point_info *info) {
     return pssfss_sheet_gsm_batch(ctx, sheet, 1, pt, gsm, precision, info);

@@ -290,3 +290,11 @@ void launch_obs(int64_t nbf, int ncol, const cd *ft, const int32_t *d_col_job,
 void lu_solve_transposed(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
                          const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
                          hipStream_t st);
+
+// ---- ufp.hip
+// Face-pair classes of a mesh (rho: 2*nnode, fv: 3*nface 1-based, device arrays).
+// Writes ufpm (nface^2, 1-based class of pair m + nface*n), members (nface^2,
+// ascending within classes) and offsets (nufp + 1, capacity nface^2 + 1); returns nufp.
+int64_t face_pair_classes_dev(int64_t nnode, const double *d_rho, int64_t nface,
+                              const int64_t *d_fv, double radius, int64_t *d_ufpm,
+                              int64_t *d_members, uint64_t *d_offsets_u64, hipStream_t st);

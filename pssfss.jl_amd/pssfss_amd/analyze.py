@@ -185,7 +185,8 @@ def _unique_indices(v):
     return ui
 
 
-def prepare(strata, flist, steering, mode_override=None, dbmin: float = DBMIN) -> Setup:
+def prepare(strata, flist, steering, mode_override=None, dbmin: float = DBMIN,
+            classify=None) -> Setup:
     """One-time setup of `_analyze` (src/PSSFSS.jl:114-142, 213-226)."""
     import copy
     layers = [copy.deepcopy(s) for s in strata if isinstance(s, Layer)]
@@ -211,7 +212,7 @@ def prepare(strata, flist, steering, mode_override=None, dbmin: float = DBMIN) -
     Md.choose_layer_modes(layers, sheets, junc, gbls, k0max, dbmin, mode_override=mode_override)
     gbldup = _get_gbldup(gbls, layers, sheets, junc)
     usi = _unique_indices(sheets)
-    rwgdat = [setup_rwg(sh) if sh.style != "NULL" else None for sh in sheets]
+    rwgdat = [setup_rwg(sh, classify=classify) if sh.style != "NULL" else None for sh in sheets]
     uvec = []
     for sh in sheets:
         if sh.style == "NULL":
@@ -235,12 +236,13 @@ def analyze(strata, flist, steering, precision: str = "fp64", mode_override=None
             ctx=None, return_stats: bool = False, dbmin: float = DBMIN):
     """Analyze the structure over all steering values and frequencies on the GPU.
 
-    Returns a list of result dicts {gsm, FGHz, steer, beta00} in the reference's
-    loop order.  precision: "fp64" (plain FP64 path) or "mixed" (FP32 LU and
-    smooth sums, FP64 refinement)."""
-    from .hip import SheetSolver, get_context
+    Returns a list of Result records (gsm, FGHz, steering, beta00, ...) in the
+    reference's loop order.  precision: "fp64" (plain FP64 path) or "mixed" (FP32 LU
+    and smooth sums, FP64 refinement).  Face-pair classes are formed on the device."""
+    from .hip import SheetSolver, get_context, face_pair_classes
     ctx = ctx or get_context()
-    setup = prepare(strata, flist, steering, mode_override=mode_override, dbmin=dbmin)
+    setup = prepare(strata, flist, steering, mode_override=mode_override, dbmin=dbmin,
+                    classify=lambda sh: face_pair_classes(ctx, sh))
     solver = SheetSolver(ctx, setup, precision)
     results, stats = solver.run()
     return (results, stats) if return_stats else results

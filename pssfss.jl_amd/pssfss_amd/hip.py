@@ -92,6 +92,8 @@ def lib():
                                              C.c_int, C.POINTER(PointInfo)]
         L.pssfss_sheet_adjoint.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.c_int, _dp,
                                            C.c_int, _dp, _dp]
+        L.pssfss_face_pair_classes.argtypes = [vp, C.c_int64, _dp, C.c_int64, _ip,
+                                               C.POINTER(C.c_int64), _ip, _ip, _ip]
         L.pssfss_fill.argtypes = [vp, vp, C.POINTER(Point), _dp, C.POINTER(PointInfo)]
         L.pssfss_filljk.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.pssfss_modal_tables.argtypes = [vp, C.c_int, C.POINTER(Point), _dp, _dp, _dp,
@@ -117,7 +119,8 @@ EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssf
            "pssfss_sheet_upload", "pssfss_sheet_free", "pssfss_sheet_gsm",
            "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk", "pssfss_modal_tables",
            "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
-           "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint")
+           "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint",
+           "pssfss_face_pair_classes")
 
 
 def _d(a):
@@ -217,6 +220,26 @@ def get_context() -> Context:
     if _CTX is None:
         _CTX = Context(0)
     return _CTX
+
+
+def face_pair_classes(ctx: "Context", sheet):
+    """Unique-face-pair classes of `sheet` on the device: (offsets, members, ufpm) as
+    rwg.unique_face_pairs returns them (ufpm[m, n] 0-based)."""
+    nface = sheet.nface
+    npair = nface * nface
+    rho = np.ascontiguousarray(sheet.rho, np.float64)
+    fv = np.ascontiguousarray((sheet.fv + 1).T, np.int64)        # 3 per face, 1-based
+    offsets = np.zeros(npair + 1, np.int64)
+    members = np.zeros(npair, np.int64)
+    ufpm1 = np.zeros(npair, np.int64)
+    n = C.c_int64()
+    ctx.check(lib().pssfss_face_pair_classes(ctx.h, rho.shape[0], _d(rho), nface,
+                                             fv.ctypes.data_as(_ip), C.byref(n),
+                                             offsets.ctypes.data_as(_ip),
+                                             members.ctypes.data_as(_ip),
+                                             ufpm1.ctypes.data_as(_ip)))
+    ufpm = (ufpm1 - 1).reshape(nface, nface).T.copy()
+    return offsets[:n.value + 1].copy(), members, ufpm
 
 
 class DeviceSheet:

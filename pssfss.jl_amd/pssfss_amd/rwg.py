@@ -56,7 +56,10 @@ def _edge_xieta(sheet: RWGSheet):
     return xi_i, xi_t, eta_i, eta_t
 
 
-def setup_rwg(sheet: RWGSheet, fufp: Optional[bool] = None) -> RWGData:
+def setup_rwg(sheet: RWGSheet, fufp: Optional[bool] = None, classify=None) -> RWGData:
+    """Basis-function maps and face-pair classes.  classify(sheet) -> (offsets,
+    members, ufpm) replaces the host classification (e.g. the device one of
+    hip.face_pair_classes)."""
     tol = 1.0e-5
     nface = sheet.nface
     nedge = sheet.nedge
@@ -147,7 +150,7 @@ def setup_rwg(sheet: RWGSheet, fufp: Optional[bool] = None) -> RWGData:
     use_fufp = sheet.fufp if fufp is None else fufp
     if not use_fufp:
         return RWGData(bfe, bff, ebf, eci, nface * nface, True)
-    offsets, members, ufpm = unique_face_pairs(sheet)
+    offsets, members, ufpm = (classify or unique_face_pairs)(sheet)
     return RWGData(bfe, bff, ebf, eci, len(offsets) - 1, False, offsets, members, ufpm)
 
 

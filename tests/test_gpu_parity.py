@@ -337,3 +337,24 @@ def test_grating_closed_form(ctx, pk, cls):
           "s21": np.array([[t, 0], [0, -r]]), "s12": np.array([[t, 0], [0, -r]])}
     for k, v in ex.items():
         assert np.abs(getattr(g, k) - v).max() < 0.01, k
+
+
+@pytest.mark.parametrize("mesh", ["rectstrip", "meander", "rectstrip_rot"])
+def test_face_pair_classes_device_vs_host(ctx, pk, mesh):
+    """Device classification (hash, radix sort, verification) equals the host's
+    exactly: classes, their numbering by smallest member, member order, ufpm."""
+    from pssfss_amd.rwg import unique_face_pairs
+    from pssfss_amd.hip import face_pair_classes
+    _say(f"face-pair classes on the device: {mesh}")
+    if mesh == "rectstrip":
+        sh = pk.rectstrip(Lx=5, Ly=5, Px=10, Py=10, Nx=14, Ny=14, units="mm")
+    elif mesh == "meander":
+        sh = pk.sheets.meander(a=6.0, b=10.0, h=2.4, w1=0.3, w2=0.3, ntri=800, units="mm")
+    else:
+        sh = pk.rectstrip(Lx=4, Ly=3, Px=10, Py=8, Nx=9, Ny=7, units="mm", rot=30.0)
+    o1, m1, u1 = unique_face_pairs(sh)
+    o2, m2, u2 = face_pair_classes(ctx, sh)
+    print(f"  nufp {len(o1) - 1} (host) {len(o2) - 1} (device)")
+    assert np.array_equal(o1, o2)
+    assert np.array_equal(m1, m2)
+    assert np.array_equal(u1, u2)
