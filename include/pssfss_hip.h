@@ -91,7 +91,8 @@ typedef struct {
     int32_t refine_iters;       /* iterative-refinement sweeps (PSSFSS_MIXED)         */
     int32_t fallback_fp64;      /* 1: refinement stalled, point re-solved in FP64     */
     int32_t pad;
-    double refine_resid;        /* final max relative residual of the solves          */
+    double refine_resid;        /* best normwise backward error of the mixed solves,
+                                   max_c |r_c| / (sqrt(n) |x_c| |A| eps); < 1 converged */
 } pssfss_point_info;
 
 /* ---- context ---------------------------------------------------------------- */

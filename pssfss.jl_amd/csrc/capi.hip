@@ -89,6 +89,7 @@ struct pssfss_ctx {
     DBuf<cf> wTf;
     DBuf<double> wnorm;
     DBuf<cd> wG;                 // fill class terms
+    DBuf<double> wanorm;         // infinity norms of the systems (refinement)
     const cd *dft_matrix(int M) {
         auto it = dft.find(M);
         if (it != dft.end()) return it->second.p;
@@ -588,8 +589,9 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
 
 // Solve the np systems in ctx->wZ for the right-hand sides in ctx->wB (nbf x ncol
 // column-major per point), leaving X in ctx->wB.
-// Refinement stops at this max-norm residual relative to |b| (or on stagnation).
-constexpr double IR_TOL = 1e-11;
+// Refinement sweeps before a system that has not converged is re-solved in FP64
+// (LAPACK's zcgesv allows 30; stagnation ends it earlier).
+constexpr int IR_MAXIT = 30;
 
 void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
                  pssfss_point_info *infos) {
@@ -624,7 +626,7 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wT.ensure((size_t)sB * np);
     lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
     double *rn = ctx->wnorm.p, *xn = ctx->wnorm.p + np * ncol;
-    std::vector<double> hr(np * ncol), hx(np * ncol), hb(np * ncol);
+    std::vector<double> hr(np * ncol), hx(np * ncol);
     // systems still refining; the residuals and solves run on these only
     std::vector<int32_t> act(np);
     for (int p = 0; p < np; ++p) act[p] = p;
@@ -649,28 +651,32 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
                 }
         }
     };
-    // |B| via the residual of X = 0
+    // ||A||_inf of each system for the stopping rule of LAPACK's zcgesv:
+    // max_c ||r_c|| < sqrt(n) ||x_c|| ||A|| eps  (backward error at FP64 level)
+    ctx->wanorm.ensure((size_t)np + 2);
+    matrix_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
+    std::vector<double> anorm(np + 1);
+    d2h(ctx, anorm.data(), ctx->wanorm.p, np);
     launch_zero(ctx->wX.p, sB * np, st);
-    residual(ctx->wX.p);
-    hb = hr;
     convert_z_to_f32(ctx->wB.p, ctx->wRf.p, sB * np, st);
     lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wperm.p, n, ctx->wRf.p, sB,
                          ctx->wTf.p, st);
     add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
     std::vector<double> prev(np, 1e300), best(np, 1e300);
-    std::vector<int> iters(np, 0), done(np, 0), diverged(np, 0);
-    const int maxit = 10;
-    for (int it = 0; it < maxit; ++it) {
+    std::vector<int> iters(np, 0), done(np, 0), conv(np, 0);
+    const double eps = 2.220446049250313e-16, sqn = std::sqrt((double)n);
+    for (int it = 0; it < IR_MAXIT; ++it) {
         residual(ctx->wX.p);
         for (int p : act) {
+            // normwise backward error of the worst column, in units of sqrt(n) eps
             double r = 0;
             for (int c = 0; c < ncol; ++c) {
-                double bb = hb[p * ncol + c] > 0 ? hb[p * ncol + c] : 1.0;
-                r = std::max(r, hr[p * ncol + c] / bb);
+                const double den = sqn * hx[p * ncol + c] * anorm[p] * eps;
+                r = std::max(r, den > 0 ? hr[p * ncol + c] / den : (hr[p * ncol + c] > 0 ? 1e300 : 0));
             }
             iters[p] = it + 1;
-            if (r <= IR_TOL || r >= 0.5 * prev[p]) done[p] = 1;
-            if (r > 2 * best[p] && it > 0) diverged[p] = 1;
+            if (r < 1.0) conv[p] = done[p] = 1;
+            else if (r >= 0.5 * prev[p]) done[p] = 1;       // stagnated: FP64 LU below
             best[p] = std::min(best[p], r);
             prev[p] = r;
         }
@@ -688,10 +694,10 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
         add_correction_sel(ctx->wX.p, ctx->wRf.p, sB, ctx->wsel.p, (int)act.size(), st);
     }
     stage_end(5, st);
-    // Points whose refinement did not reach an FP64-level residual: FP64 LU.
+    // Points whose refinement did not reach an FP64-level backward error: FP64 LU.
     std::vector<int> fb;
     for (int p = 0; p < np; ++p)
-        if (diverged[p] || best[p] > 1e-10) fb.push_back(p);
+        if (!conv[p]) fb.push_back(p);
     for (int p : fb) {
         lu_factor_batched(n, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n, st);
         lu_make_perm(n, 1, ctx->wpiv.p + p * n, n, ctx->wperm.p + p * n, st);

@@ -177,6 +177,9 @@ void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, c
                       const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
                       hipStream_t st, const int32_t *sel = nullptr, int nsel = 0);
 void residual_to_f32(const cd *R, cf *Rf, int64_t count, hipStream_t st);
+// out[b] = infinity norm of system b (out holds 2 * ceil(batch / 2) doubles).
+void matrix_inf_norms(int64_t n, int batch, const cd *A, int64_t sA, double *out,
+                      hipStream_t st);
 void add_correction(cd *X, const cf *D, int64_t count, hipStream_t st);
 // X[sys] += D[sys] for the nsel systems listed in sel, each of sB entries.
 void add_correction_sel(cd *X, const cf *D, int64_t sB, const int32_t *sel, int nsel,

@@ -12,6 +12,8 @@
 namespace {
 
 constexpr int NB = 64;  // panel width
+// Panels with more rows than this are factored by 8-column leaves (panel_recursive).
+constexpr int64_t RECURSIVE_PANEL_ROWS = 3072;
 
 // ------------------------------------------------------------------ GEMM (NN)
 // C = alpha*A*B + beta*C, column-major complex, batched by blockIdx.z, on the
@@ -393,13 +395,15 @@ __global__ __launch_bounds__(1024) void k_panel_reg(int64_t n, int64_t k, int kb
 // panel), in pivot order.
 template <class C>
 __global__ void k_rowswap(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
-                          const int32_t *ipiv0, int64_t sP) {
+                          const int32_t *ipiv0, int64_t sP, int64_t cbeg, int64_t cend,
+                          int64_t sbeg, int64_t send) {
+    // columns [cbeg, cend) without [sbeg, send), in pivot order of rows k..k+kb-1
     C *A = A0 + (int64_t)blockIdx.y * sA;
     const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.y * sP;
-    const int64_t c0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t other = n - kb;
-    if (c0 >= other) return;
-    const int64_t c = c0 < k ? c0 : c0 + kb;
+    const int64_t c0 = cbeg + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t skip = send - sbeg;
+    if (c0 >= cend - skip) return;
+    const int64_t c = c0 < sbeg ? c0 : c0 + skip;
     for (int j = 0; j < kb; ++j) {
         const int64_t r = k + j, p = ipiv[r];
         if (p != r) {
@@ -410,10 +414,11 @@ __global__ void k_rowswap(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
     }
 }
 
-// U12 = L11^{-1} A12 (unit lower L11), one thread per trailing column.
+// U12 = L11^{-1} A12 (unit lower L11) for columns [k + kb, cend), one thread per
+// column.
 template <class C>
 __global__ __launch_bounds__(64) void k_trsm_u12(int64_t n, int64_t k, int kb, C *A0,
-                                                 int64_t sA) {
+                                                 int64_t sA, int64_t cend) {
     C *A = A0 + (int64_t)blockIdx.y * sA;
     __shared__ C L[NB][NB];
     for (int t = threadIdx.x; t < kb * kb; t += blockDim.x) {
@@ -422,7 +427,7 @@ __global__ __launch_bounds__(64) void k_trsm_u12(int64_t n, int64_t k, int kb, C
     }
     __syncthreads();
     const int64_t c = k + kb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    if (c >= cend) return;
     C x[NB];
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
@@ -479,16 +484,19 @@ __device__ __forceinline__ C wave_sum(C v) {
 // cross-lane reduction per row and right-hand side), then the 64x64 diagonal
 // block is solved one wavefront per right-hand side.  XL: the solution is also
 // kept in LDS (n * NR entries of dynamic shared memory) for the dot products.
-// sel (optional) lists the systems to solve, one per workgroup.
+// Grid: x = system (sel, optional, lists them), y = group of NR right-hand sides.
 template <class C, int NR, int RW, int U, bool XL>
 __global__ __launch_bounds__(1024) void k_trsv_ll(int64_t n, int nrhs, const C *LU0, int64_t sA,
                                                   const int32_t *perm0, int64_t sP, C *B0,
                                                   int64_t sB, C *T0, const int32_t *sel) {
     const int64_t sys = sel ? sel[blockIdx.x] : blockIdx.x;
+    // right-hand-side group of this workgroup: columns [NR y, NR y + NR)
+    const int64_t cg = (int64_t)blockIdx.y * NR;
+    nrhs = nrhs - (int)cg < NR ? nrhs - (int)cg : NR;
     const C *LU = LU0 + sys * sA;
     const int32_t *perm = perm0 + sys * sP;
-    C *B = B0 + sys * sB;
-    C *T = T0 + sys * sB;
+    C *B = B0 + sys * sB + cg * n;
+    C *T = T0 + sys * sB + cg * n;
     __shared__ C D[NB][NB + 1];
     __shared__ C Y[NR][NB];
     extern __shared__ char xlds_raw[];
@@ -633,6 +641,21 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd 
     }
 }
 
+// Infinity norm (largest row sum of |a_ij|) of each row-major n x n system, as
+// a monotone bit pattern through atomicMax; one wavefront per row.
+__global__ __launch_bounds__(256) void k_rownorm(int64_t n, const cd *A0, int64_t sA,
+                                                 unsigned long long *out) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const cd *row = A0 + b * sA + i * n;
+    double s = 0;
+    for (int64_t j = lane; j < n; j += 64) s += cabs(row[j]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) atomicMax(out + b, (unsigned long long)__double_as_longlong(s));
+}
+
 __global__ void k_cd_to_cf(const cd *a, cf *b, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -699,38 +722,80 @@ void launch_panel(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv,
 }
 
 template <class C>
-void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                 hipStream_t st) {
+void rowswap_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, const int32_t *ipiv, int64_t sP,
+               int64_t cbeg, int64_t cend, int64_t sbeg, int64_t send, int batch,
+               hipStream_t st) {
+    const int64_t cols = (cend - cbeg) - (send - sbeg);
+    if (cols <= 0) return;
+    dim3 g((unsigned)((cols + 255) / 256), (unsigned)batch);
+    hipLaunchKernelGGL(k_rowswap<C>, g, dim3(256), 0, st, n, k, kb, A, sA, ipiv, sP, cbeg, cend,
+                       sbeg, send);
+    HIPCHECK(hipGetLastError());
+}
+
+template <class C>
+void trsm_u12_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, int64_t cend, int batch,
+                hipStream_t st) {
+    const int64_t cols = cend - (k + kb);
+    if (cols <= 0) return;
+    dim3 g((unsigned)((cols + 63) / 64), (unsigned)batch);
+    hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k, kb, A, sA, cend);
+    HIPCHECK(hipGetLastError());
+}
+
+// A[r0:n, c0:c1] -= A[r0:n, kk:kk+kw] A[kk:kk+kw, c0:c1] (row-major, r0 = c0 = kk + kw)
+// as a column-major GEMM on the transposed views.
+template <class C>
+void update_t(int64_t n, int64_t kk, int kw, int64_t c1, C *A, int64_t sA, int batch,
+              hipStream_t st) {
+    const int64_t r0 = kk + kw;
+    const int64_t ncols = c1 - r0, nrows = n - r0;
+    if (ncols <= 0 || nrows <= 0) return;
     C mone, one;
     mone.x = -1; mone.y = 0;
     one.x = 1; one.y = 0;
+    gemm_t<C>((int)ncols, (int)nrows, kw, mone, A + kk * n + r0, n, sA, A + r0 * n + kk, n, sA,
+              one, A + r0 * n + r0, n, sA, batch, st);
+}
+
+// Panels taller than the register kernel's sweet spot: 8-column leaves factored by
+// the register kernel, each followed by its interchanges on the panel's other
+// columns, the U rows it needs and a rank-8 GEMM update of the panel's remaining
+// columns (multi-workgroup launches, so a few large systems still fill the chip).
+template <class C>
+void panel_recursive(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv,
+                     int64_t sP, int batch, hipStream_t st) {
+    constexpr int LW = 8;
+    for (int j0 = 0; j0 < kb; j0 += LW) {
+        const int lw = std::min(LW, kb - j0);
+        const int64_t kk = k + j0;
+        launch_panel<C>(n, kk, lw, A, sA, ipiv, sP, batch, st);
+        rowswap_t<C>(n, kk, lw, A, sA, ipiv, sP, k, k + kb, kk, kk + lw, batch, st);
+        trsm_u12_t<C>(n, kk, lw, A, sA, k + kb, batch, st);
+        update_t<C>(n, kk, lw, k + kb, A, sA, batch, st);
+    }
+}
+
+template <class C>
+void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                 hipStream_t st) {
     for (int64_t k = 0; k < n; k += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k);
         stage_begin(3, st);
-        launch_panel<C>(n, k, kb, A, sA, ipiv, sP, batch, st);
-        const int64_t other = n - kb;
-        if (other > 0) {
-            dim3 g((unsigned)((other + 255) / 256), (unsigned)batch);
-            hipLaunchKernelGGL(k_rowswap<C>, g, dim3(256), 0, st, n, k, kb, A, sA, ipiv, sP);
-            HIPCHECK(hipGetLastError());
-        }
+        if (n - k > RECURSIVE_PANEL_ROWS)
+            panel_recursive<C>(n, k, kb, A, sA, ipiv, sP, batch, st);
+        else
+            launch_panel<C>(n, k, kb, A, sA, ipiv, sP, batch, st);
+        rowswap_t<C>(n, k, kb, A, sA, ipiv, sP, 0, n, k, k + kb, batch, st);
+        trsm_u12_t<C>(n, k, kb, A, sA, n, batch, st);
+        stage_end(3, st);
         const int64_t rest = n - k - kb;
         if (rest > 0) {
-            dim3 g((unsigned)((rest + 63) / 64), (unsigned)batch);
-            hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k, kb, A, sA);
-            HIPCHECK(hipGetLastError());
-            stage_end(3, st);
             // Row-major A22 -= L21 U12, as a column-major GEMM on the transposes:
             // A22^T -= U12^T L21^T, every view having leading dimension n.
-            C *u12 = A + k * n + (k + kb);
-            C *l21t = A + (k + kb) * n + k;
-            C *a22 = A + (k + kb) * n + (k + kb);
             stage_begin(4, st);
-            gemm_t<C>((int)rest, (int)rest, kb, mone, u12, n, sA, l21t, n, sA, one, a22, n, sA,
-                      batch, st);
+            update_t<C>(n, k, kb, n, A, sA, batch, st);
             stage_end(4, st, 8.0 * (double)rest * (double)rest * kb * batch);
-        } else {
-            stage_end(3, st);
         }
     }
     stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
@@ -748,29 +813,28 @@ template <class C>
 void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *perm,
                 int64_t sP, C *B, int64_t sB, C *scratch, hipStream_t st,
                 const int32_t *sel = nullptr) {
-    if (batch <= 0) return;
-    const size_t static_lds = sizeof(C) * (NB * (NB + 1) + 8 * NB);
-    for (int c0 = 0; c0 < nrhs; c0 += 8) {
-        int nr = std::min(8, nrhs - c0);
-        C *Bc = B + (int64_t)c0 * n;
-        C *Tc = scratch + (int64_t)c0 * n;
-        const int NRk = nr <= 4 ? 4 : 8;
-        const size_t xl = sizeof(C) * (size_t)n * NRk;
-        const bool useX = static_lds + xl <= max_lds_bytes();
-        dim3 g((unsigned)batch), b(1024);
-#define TL(NRR, RWW, UU, XLL) hipLaunchKernelGGL((k_trsv_ll<C, NRR, RWW, UU, XLL>), g, b,      \
-                                                 XLL ? xl : 0, st, n, nr, LU, sA, perm, sP,  \
-                                                 Bc, sB, Tc, sel)
-        if constexpr (sizeof(C) == sizeof(cf)) {
-            if (NRk == 4) { if (useX) TL(4, 2, 4, true); else TL(4, 2, 4, false); }
-            else { if (useX) TL(8, 2, 4, true); else TL(8, 2, 2, false); }
-        } else {
-            if (NRk == 4) { if (useX) TL(4, 2, 4, true); else TL(4, 2, 2, false); }
-            else { if (useX) TL(8, 1, 2, true); else TL(8, 1, 2, false); }
-        }
-#undef TL
-        HIPCHECK(hipGetLastError());
+    if (batch <= 0 || nrhs <= 0) return;
+    constexpr int NR = 4;
+    const size_t static_lds = sizeof(C) * (NB * (NB + 1) + NR * NB);
+    const size_t xl = sizeof(C) * (size_t)n * NR;
+    const bool useX = static_lds + xl <= max_lds_bytes();
+    dim3 g((unsigned)batch, (unsigned)((nrhs + NR - 1) / NR)), b(1024);
+    if constexpr (sizeof(C) == sizeof(cf)) {
+        if (useX)
+            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 4, true>), g, b, xl, st, n, nrhs, LU, sA,
+                               perm, sP, B, sB, scratch, sel);
+        else
+            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 4, false>), g, b, 0, st, n, nrhs, LU, sA,
+                               perm, sP, B, sB, scratch, sel);
+    } else {
+        if (useX)
+            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 4, true>), g, b, xl, st, n, nrhs, LU, sA,
+                               perm, sP, B, sB, scratch, sel);
+        else
+            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 2, false>), g, b, 0, st, n, nrhs, LU, sA,
+                               perm, sP, B, sB, scratch, sel);
     }
+    HIPCHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -849,5 +913,13 @@ void add_correction_sel(cd *X, const cf *D, int64_t sB, const int32_t *sel, int 
     int64_t bx = (sB + 255) / 256;
     dim3 g((unsigned)(bx > 64 ? 64 : bx), (unsigned)nsel);
     hipLaunchKernelGGL(k_add_corr_sel, g, dim3(256), 0, st, X, D, sB, sel);
+    HIPCHECK(hipGetLastError());
+}
+
+void matrix_inf_norms(int64_t n, int batch, const cd *A, int64_t sA, double *out,
+                      hipStream_t st) {
+    launch_zero((cd *)out, (batch + 1) / 2, st);
+    dim3 g((unsigned)((n + 3) / 4), (unsigned)batch);
+    hipLaunchKernelGGL(k_rownorm, g, dim3(256), 0, st, n, A, sA, (unsigned long long *)out);
     HIPCHECK(hipGetLastError());
 }

@@ -59,12 +59,13 @@ def test_lu_pivoting(ctx):
     assert _rel(X[0], np.linalg.solve(A, B)) < 1e-11
 
 
-@pytest.mark.parametrize("n", [1100, 2100, 3100])
+@pytest.mark.parametrize("n", [1100, 2100, 3100, 4500])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_lu_pivoting_large(ctx, n, prec):
     """Unscaled Gaussian matrices: pivots land anywhere in tall panels of 1-6 rows per
-    thread (register panels of R = 2, 3 and 6) and solutions come from both the
-    LDS-resident and the streaming triangular-solve variants."""
+    thread (register panels of R = 2, 3 and 6; panels above 3072 rows go through the
+    8-column leaves) and solutions come from both the LDS-resident and the
+    streaming triangular-solve variants."""
     _say(f"lu_solve random Gaussian n={n} {prec}")
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
