@@ -12,8 +12,6 @@
 namespace {
 
 constexpr int NB = 64;  // panel width
-// Panels with more rows than this are factored by 8-column leaves (panel_recursive).
-constexpr int64_t RECURSIVE_PANEL_ROWS = 3072;
 
 // ------------------------------------------------------------------ GEMM (NN)
 // C = alpha*A*B + beta*C, column-major complex, batched by blockIdx.z, on the
@@ -196,198 +194,192 @@ __global__ __launch_bounds__(1024) void k_panel(int64_t n, int64_t k, int kb, C 
     }
 }
 
-// Panel factorisation with the rows resident in registers, W columns at a time.
-// Interchanges inside the panel are logical: perm[i] (LDS) is the physical row
-// (relative to k) holding logical row i, so a pivot swap exchanges two perm
-// entries and two register rows and moves no memory.  After each W-column
-// sub-block is factored, its U rows for the remaining panel columns are solved
-// against the W x W unit-lower block, and the remaining panel columns of the
-// rows below take the rank-W update.  At the end the panel rows are written back
-// in logical order, so the matrix holds exactly what LAPACK's sequential
-// interchanges (recorded in ipiv) produce.  Rows per thread: R (m <= 1024 R).
-template <class C, int R, int W>
-__global__ __launch_bounds__(1024) void k_panel_reg(int64_t n, int64_t k, int kb, C *A0,
-                                                    int64_t sA, int32_t *ipiv0, int64_t sP) {
+// The 64-column panel (rows [k, n)) is factored in a column-major scratch copy
+// (m = n - k rows, leading dimension m, one per system), so the thread-per-row
+// accesses of the leaf kernel are coalesced.  dir 0 copies the panel in, dir 1
+// copies it back; 64-row tiles are transposed through LDS.
+template <class C>
+__global__ __launch_bounds__(256) void k_panel_copy(int64_t n, int64_t k, int kb, C *A0,
+                                                    int64_t sA, C *P0, int64_t sPt, int dir) {
+    __shared__ C T[64][65];
+    const int64_t m = n - k;
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    C *A = A0 + (int64_t)blockIdx.y * sA;
+    C *P = P0 + (int64_t)blockIdx.y * sPt;
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    if (dir == 0) {
+        for (int p = 0; p < 16; ++p) {
+            const int rr = q + 4 * p;
+            if (r0 + rr < m && lane < kb) T[rr][lane] = A[(k + r0 + rr) * n + k + lane];
+        }
+        __syncthreads();
+        for (int p = 0; p < 16; ++p) {
+            const int cc = q + 4 * p;
+            if (r0 + lane < m && cc < kb) P[cc * m + r0 + lane] = T[lane][cc];
+        }
+    } else {
+        for (int p = 0; p < 16; ++p) {
+            const int cc = q + 4 * p;
+            if (r0 + lane < m && cc < kb) T[lane][cc] = P[cc * m + r0 + lane];
+        }
+        __syncthreads();
+        for (int p = 0; p < 16; ++p) {
+            const int rr = q + 4 * p;
+            if (r0 + rr < m && lane < kb) A[(k + r0 + rr) * n + k + lane] = T[rr][lane];
+        }
+    }
+}
+
+// Factor leaf columns [c0, c0 + lw) of the scratch panel over rows [c0, m) with
+// partial pivoting (largest |re| + |im|, first index on ties: izamax), the rows
+// in registers, R per thread.  An interchange swaps two register rows, so the
+// rows go back to memory in the order LAPACK's sequential interchanges give.
+// ipiv receives absolute pivot rows (k is the panel's first row).
+template <class C, int R, int LW>
+__global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0, int64_t sPt,
+                                               int32_t *ipiv0, int64_t sP, int64_t k) {
     typedef typename CTraits<C>::R Rl;
     constexpr int NT = 1024;
-    C *A = A0 + (int64_t)blockIdx.x * sA;
+    C *P = P0 + (int64_t)blockIdx.x * sPt;
     int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
-    const int m = (int)(n - k);
-    __shared__ int perm[R * NT];
-    __shared__ C prow[W];
-    __shared__ C Ld[W][W];
-    __shared__ C Us[W][NB];
-    __shared__ C xch[2][W];
+    const int mr = (int)(m - c0);
+    __shared__ C prow[LW];
+    __shared__ C xch[2][LW];
     __shared__ Rl rv[16];
     __shared__ int ri[16];
     __shared__ int pivs;
     const int tid = threadIdx.x;
     const C zero = CTraits<C>::zero();
-    for (int i = tid; i < m; i += NT) perm[i] = i;
-    __syncthreads();
-    C x[R][W];
-    for (int c0 = 0; c0 < kb; c0 += W) {
-        const int wb = kb - c0 < W ? kb - c0 : W;
+    C x[R][LW];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = tid + NT * r;
-            const bool live = i >= c0 && i < m;
-            const C *row = A + (k + (live ? perm[i] : 0)) * n + k + c0;
+    for (int r = 0; r < R; ++r) {
+        const int i = tid + NT * r;
 #pragma unroll
-            for (int c = 0; c < W; ++c) x[r][c] = (live && c < wb) ? row[c] : zero;
-        }
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            if (j >= wb) break;
-            const int col = c0 + j;
-            Rl best = (Rl)-1;
-            int bi = 0x7fffffff;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = tid + NT * r;
-                if (i >= col && i < m) {
-                    Rl v = cabs1(x[r][j]);
-                    if (v > best) { best = v; bi = i; }
-                }
-            }
-            for (int off = 32; off > 0; off >>= 1) {
-                Rl ov = __shfl_xor(best, off, 64);
-                int oi = __shfl_xor(bi, off, 64);
-                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-            }
-            if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
-            __syncthreads();
-            if (tid == 0) {
-                Rl b = rv[0];
-                int bx = ri[0];
-                for (int w = 1; w < NT / 64; ++w)
-                    if (rv[w] > b || (rv[w] == b && ri[w] < bx)) { b = rv[w]; bx = ri[w]; }
-                pivs = bx;
-                ipiv[k + col] = (int32_t)(k + bx);
-            }
-            __syncthreads();
-            const int p = pivs;
-            const int oc = col % NT, sc = col / NT, op = p % NT, sp = p / NT;
-            if (p != col) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if (tid == op && r == sp)
-#pragma unroll
-                        for (int c = 0; c < W; ++c) xch[0][c] = x[r][c];
-                    if (tid == oc && r == sc)
-#pragma unroll
-                        for (int c = 0; c < W; ++c) xch[1][c] = x[r][c];
-                }
-                if (tid == 0) { int t = perm[col]; perm[col] = perm[p]; perm[p] = t; }
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if (tid == oc && r == sc)
-#pragma unroll
-                        for (int c = 0; c < W; ++c) x[r][c] = xch[0][c];
-                    if (tid == op && r == sp)
-#pragma unroll
-                        for (int c = 0; c < W; ++c) x[r][c] = xch[1][c];
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (tid == oc && r == sc)
-#pragma unroll
-                    for (int c = 0; c < W; ++c) prow[c] = x[r][c];
-            __syncthreads();
-            const C piv = prow[j];
-            const bool nz = !(piv.x == 0 && piv.y == 0);
-            const C rp = nz ? CTraits<C>::one() / piv : zero;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = tid + NT * r;
-                if (i > col && i < m) {
-                    C l = nz ? x[r][j] * rp : x[r][j];
-                    x[r][j] = l;
-                    C ml = -l;
-#pragma unroll
-                    for (int c = j + 1; c < W; ++c) cfma(x[r][c], ml, prow[c]);
-                }
-            }
-        }
-        // write the sub-block back; the diagonal block goes to LDS as well
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = tid + NT * r;
-            if (i >= c0 && i < m) {
-                C *row = A + (k + perm[i]) * n + k + c0;
-#pragma unroll
-                for (int c = 0; c < W; ++c)
-                    if (c < wb) row[c] = x[r][c];
-                if (i < c0 + wb)
-#pragma unroll
-                    for (int c = 0; c < W; ++c) Ld[i - c0][c] = x[r][c];
-            }
-        }
-        __syncthreads();
-        const int cr0 = c0 + wb;
-        if (cr0 < kb) {
-            // U rows of this sub-block for panel columns [cr0, kb): unit-lower solve
-            if (tid < kb - cr0) {
-                const int cc = cr0 + tid;
-                C u[W];
-#pragma unroll
-                for (int j = 0; j < W; ++j) {
-                    if (j < wb) {
-                        C v = A[(k + perm[c0 + j]) * n + k + cc];
-#pragma unroll
-                        for (int l = 0; l < j; ++l) cfma(v, -Ld[j][l], u[l]);
-                        u[j] = v;
-                        A[(k + perm[c0 + j]) * n + k + cc] = v;
-                        Us[j][cc] = v;
-                    }
-                }
-            }
-            __syncthreads();
-            // rank-wb update of the remaining panel columns of the rows below
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = tid + NT * r;
-                if (i >= cr0 && i < m) {
-                    C *row = A + (k + perm[i]) * n + k;
-                    for (int cc = cr0; cc < kb; ++cc) {
-                        C v = row[cc];
-#pragma unroll
-                        for (int j = 0; j < W; ++j)
-                            if (j < wb) cfma(v, -x[r][j], Us[j][cc]);
-                        row[cc] = v;
-                    }
-                }
-            }
-            __syncthreads();
+        for (int c = 0; c < LW; ++c) {
+            x[r][c] = zero;
+            if (i < mr && c < lw) x[r][c] = P[(int64_t)(c0 + c) * m + c0 + i];
         }
     }
-    // Panel rows into logical order, W columns at a time.
-    for (int c0 = 0; c0 < kb; c0 += W) {
-        const int wb = kb - c0 < W ? kb - c0 : W;
+#pragma unroll
+    for (int j = 0; j < LW; ++j) {
+        if (j >= lw) break;
+        Rl best = (Rl)-1;
+        int bi = 0x7fffffff;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tid + NT * r;
-            if (i < m) {
-                const C *row = A + (k + perm[i]) * n + k + c0;
-#pragma unroll
-                for (int c = 0; c < W; ++c)
-                    if (c < wb) x[r][c] = row[c];
+            if (i >= j && i < mr) {
+                const Rl v = cabs1(x[r][j]);
+                if (v > best) { best = v; bi = i; }
             }
         }
+        for (int off = 32; off > 0; off >>= 1) {
+            const Rl ov = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
         __syncthreads();
+        if (tid == 0) {
+            Rl b = rv[0];
+            int bx = ri[0];
+            for (int w = 1; w < NT / 64; ++w)
+                if (rv[w] > b || (rv[w] == b && ri[w] < bx)) { b = rv[w]; bx = ri[w]; }
+            if (bx >= mr) bx = j;   // no comparable entry (NaN column): keep the diagonal
+            pivs = bx;
+            ipiv[k + c0 + j] = (int32_t)(k + c0 + bx);
+        }
+        __syncthreads();
+        const int p = pivs;
+        const int op = p % NT, sp = p / NT;
+        if (p != j) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (tid == op && r == sp)
+#pragma unroll
+                    for (int c = 0; c < LW; ++c) xch[0][c] = x[r][c];
+            }
+            if (tid == j)
+#pragma unroll
+                for (int c = 0; c < LW; ++c) xch[1][c] = x[0][c];
+            __syncthreads();
+            if (tid == j)
+#pragma unroll
+                for (int c = 0; c < LW; ++c) x[0][c] = xch[0][c];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (tid == op && r == sp)
+#pragma unroll
+                    for (int c = 0; c < LW; ++c) x[r][c] = xch[1][c];
+            }
+        }
+        if (tid == j)
+#pragma unroll
+            for (int c = 0; c < LW; ++c) prow[c] = x[0][c];
+        __syncthreads();
+        const C piv = prow[j];
+        const bool nz = !(piv.x == 0 && piv.y == 0);
+        const C rp = nz ? CTraits<C>::one() / piv : zero;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tid + NT * r;
-            if (i < m) {
-                C *row = A + (k + i) * n + k + c0;
+            if (i > j && i < mr) {
+                const C l = nz ? x[r][j] * rp : x[r][j];
+                x[r][j] = l;
+                const C ml = -l;
 #pragma unroll
-                for (int c = 0; c < W; ++c)
-                    if (c < wb) row[c] = x[r][c];
+                for (int c = j + 1; c < LW; ++c) cfma(x[r][c], ml, prow[c]);
             }
         }
-        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = tid + NT * r;
+        if (i < mr)
+#pragma unroll
+            for (int c = 0; c < LW; ++c)
+                if (c < lw) P[(int64_t)(c0 + c) * m + c0 + i] = x[r][c];
+    }
+}
+
+// After a leaf: its interchanges on the panel's other columns (in pivot order),
+// and the leaf's U rows for the columns to its right (unit-lower solve with the
+// leaf's lw x lw block).  One thread per panel column.
+template <class C, int LW>
+__global__ __launch_bounds__(64) void k_leaf_apply(int64_t m, int kb, int c0, int lw, C *P0,
+                                                   int64_t sPt, const int32_t *ipiv0, int64_t sP,
+                                                   int64_t k) {
+    C *P = P0 + (int64_t)blockIdx.x * sPt;
+    const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
+    __shared__ C L[LW][LW];
+    const int c = threadIdx.x;
+    if (c < lw * lw) {
+        const int a = c / lw, b = c % lw;
+        L[a][b] = P[(int64_t)(c0 + b) * m + c0 + a];
+    }
+    __syncthreads();
+    if (c >= kb || (c >= c0 && c < c0 + lw)) return;
+    C *col = P + (int64_t)c * m;
+    for (int j = 0; j < lw; ++j) {
+        const int64_t r = c0 + j, p = (int64_t)ipiv[k + c0 + j] - k;
+        if (p != r) {
+            const C t = col[r];
+            col[r] = col[p];
+            col[p] = t;
+        }
+    }
+    if (c >= c0 + lw) {
+        C u[LW];
+#pragma unroll
+        for (int j = 0; j < LW; ++j) {
+            if (j < lw) {
+                C v = col[c0 + j];
+#pragma unroll
+                for (int l = 0; l < j; ++l) cfma(v, -L[j][l], u[l]);
+                u[j] = v;
+                col[c0 + j] = v;
+            }
+        }
     }
 }
 
@@ -468,129 +460,143 @@ __device__ __forceinline__ C shfl_c(C v, int src) {
     return r;
 }
 
+// T = P B (row i of T is row perm[i] of B) for nrhs columns, the systems of the
+// batch (or those listed in sel).
 template <class C>
-__device__ __forceinline__ C wave_sum(C v) {
-    for (int off = 32; off > 0; off >>= 1) {
-        v.x += __shfl_xor(v.x, off, 64);
-        v.y += __shfl_xor(v.y, off, 64);
+__global__ void k_perm_rows(int64_t n, int nrhs, const int32_t *perm0, int64_t sP, const C *B0,
+                            C *T0, int64_t sB, const int32_t *sel) {
+    const int64_t sys = sel ? sel[blockIdx.y] : blockIdx.y;
+    const int32_t *perm = perm0 + sys * sP;
+    const C *B = B0 + sys * sB;
+    C *T = T0 + sys * sB;
+    const int64_t tot = n * nrhs;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t % n, c = t / n;
+        T[t] = B[perm[i] + n * c];
     }
-    return v;
 }
 
-// P B, then L y = b (unit lower) and U x = y for nrhs <= NR columns of a
-// column-major B, one workgroup per system, left-looking: the rows of block b
-// first take the whole dot product with the already solved part (lanes stride
-// the contiguous row, RW rows per wavefront, U column chunks in flight, one
-// cross-lane reduction per row and right-hand side), then the 64x64 diagonal
-// block is solved one wavefront per right-hand side.  XL: the solution is also
-// kept in LDS (n * NR entries of dynamic shared memory) for the dot products.
-// Grid: x = system (sel, optional, lists them), y = group of NR right-hand sides.
-template <class C, int NR, int RW, int U, bool XL>
-__global__ __launch_bounds__(1024) void k_trsv_ll(int64_t n, int nrhs, const C *LU0, int64_t sA,
-                                                  const int32_t *perm0, int64_t sP, C *B0,
-                                                  int64_t sB, C *T0, const int32_t *sel) {
-    const int64_t sys = sel ? sel[blockIdx.x] : blockIdx.x;
-    // right-hand-side group of this workgroup: columns [NR y, NR y + NR)
-    const int64_t cg = (int64_t)blockIdx.y * NR;
-    nrhs = nrhs - (int)cg < NR ? nrhs - (int)cg : NR;
+template <class C>
+__global__ void k_copy_sys(int64_t cnt, const C *S0, C *D0, int64_t sB, const int32_t *sel) {
+    const int64_t sys = sel ? sel[blockIdx.y] : blockIdx.y;
+    const C *S = S0 + sys * sB;
+    C *D = D0 + sys * sB;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+         t += (int64_t)gridDim.x * blockDim.x)
+        D[t] = S[t];
+}
+
+// One step of the blocked triangular solves L y = b (unit lower, UP false) and
+// U x = y (UP true) on a column-major X (n x nrhs per system), in place.  The
+// rows [s0, s0 + ss) are solved already; every other workgroup subtracts their
+// contribution from a 64-row tile (forward: rows below, backward: rows above),
+// and workgroup 0 owns the diagonal block [d0, d0 + ds), which it then solves
+// (one wavefront per right-hand side, lane = row).  Grid: x = tile (0 = the
+// diagonal block), y = group of NRG right-hand sides, z = system (sel lists
+// them when given).  A pass is 2 + 2 ceil(n / 64) launches, each reading its
+// tiles of the factors once.
+template <class C, int NRG, bool UP>
+__global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C *LU0, int64_t sA,
+                                                   C *X0, int64_t sB, const int32_t *sel,
+                                                   int64_t s0, int ss, int64_t d0, int ds) {
+    constexpr int NS = NRG / 4;
+    const int64_t sys = sel ? sel[blockIdx.z] : blockIdx.z;
     const C *LU = LU0 + sys * sA;
-    const int32_t *perm = perm0 + sys * sP;
-    C *B = B0 + sys * sB + cg * n;
-    C *T = T0 + sys * sB + cg * n;
-    __shared__ C D[NB][NB + 1];
-    __shared__ C Y[NR][NB];
-    extern __shared__ char xlds_raw[];
-    C *XS = reinterpret_cast<C *>(xlds_raw);      // [NR][n] when XL
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    const int cg = (int)blockIdx.y * NRG;
+    const int nc = nrhs - cg < NRG ? nrhs - cg : NRG;
+    C *X = X0 + sys * sB + (int64_t)cg * n;
+    __shared__ C Tt[64][65];
+    __shared__ C XS[64][NRG];
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     const C zero = CTraits<C>::zero();
-    for (int64_t t = tid; t < n * nrhs; t += blockDim.x) {
-        int64_t i = t % n, c = t / n;
-        T[i + n * c] = B[perm[i] + n * c];
+    int64_t r0;
+    int rows;
+    if (blockIdx.x == 0) {
+        r0 = d0;
+        rows = ds;
+    } else if (!UP) {
+        r0 = d0 + ds + 64 * (int64_t)(blockIdx.x - 1);
+        rows = n - r0 < 64 ? (int)(n - r0) : 64;
+    } else {
+        r0 = d0 - 64 * (int64_t)blockIdx.x;
+        rows = 64;
+    }
+    C y[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = q + 4 * s;
+        y[s] = zero;
+        if (lane < rows && c < nc) y[s] = X[r0 + lane + n * c];
+    }
+    if (ss > 0) {
+        // factor tile rows [r0, r0 + rows) x columns [s0, s0 + ss), transposed
+        for (int p = 0; p < 16; ++p) {
+            const int rr = q + 4 * p;
+            C v = zero;
+            if (rr < rows && lane < ss) v = LU[(r0 + rr) * n + s0 + lane];
+            Tt[lane][rr] = v;
+        }
+        for (int t = tid; t < 64 * NRG; t += 256) {
+            const int l = t % 64, c = t / 64;
+            C v = zero;
+            if (l < ss && c < nc) v = X[s0 + l + n * c];
+            XS[l][c] = v;
+        }
+        __syncthreads();
+        C acc[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[s] = zero;
+#pragma unroll 8
+        for (int l = 0; l < 64; ++l) {
+            const C a = Tt[l][lane];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) cfma(acc[s], a, XS[l][q + 4 * s]);
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) y[s] = y[s] - acc[s];
+    }
+    if (blockIdx.x != 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int c = q + 4 * s;
+            if (lane < rows && c < nc) X[r0 + lane + n * c] = y[s];
+        }
+        return;
+    }
+    // the diagonal block, transposed: Tt[col][row]
+    __syncthreads();
+    for (int p = 0; p < 16; ++p) {
+        const int rr = q + 4 * p;
+        C v = zero;
+        if (rr < ds && lane < ds) v = LU[(d0 + rr) * n + d0 + lane];
+        Tt[lane][rr] = v;
     }
     __syncthreads();
-    for (int64_t t = tid; t < n * nrhs; t += blockDim.x) B[t] = T[t];
-    __syncthreads();
-    const C *X = XL ? XS : B;
-    const int64_t nblk = (n + NB - 1) / NB;
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool fwd = pass == 0;
-        for (int64_t q = 0; q < nblk; ++q) {
-            const int64_t bb = fwd ? q : nblk - 1 - q;
-            const int64_t b0 = bb * NB;
-            const int bs = (int)min((int64_t)NB, n - b0);
-            for (int t = tid; t < bs * bs; t += blockDim.x) {
-                int r = t / bs, l = t % bs;
-                D[r][l] = LU[(b0 + r) * n + b0 + l];
+    if (!UP) {
+        for (int r = 0; r < ds; ++r) {
+            const C dl = Tt[r][lane];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const C xr = shfl_c(y[s], r);
+                if (lane > r) cfma(y[s], -dl, xr);
             }
-            const int64_t j0 = fwd ? 0 : b0 + bs, j1 = fwd ? b0 : n;
-            for (int r0 = wave * RW; r0 < bs; r0 += nw * RW) {
-                C acc[RW][NR];
-#pragma unroll
-                for (int a = 0; a < RW; ++a)
-#pragma unroll
-                    for (int c = 0; c < NR; ++c) acc[a][c] = zero;
-                const C *rows[RW];
-#pragma unroll
-                for (int a = 0; a < RW; ++a)
-                    rows[a] = LU + (b0 + (r0 + a < bs ? r0 + a : r0)) * n;
-                int64_t j = j0 + lane;
-                for (; j + 64 * (U - 1) < j1; j += 64 * U) {
-                    C l[U][RW];
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-#pragma unroll
-                        for (int a = 0; a < RW; ++a) l[u][a] = rows[a][j + 64 * u];
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-#pragma unroll
-                        for (int c = 0; c < NR; ++c) {
-                            if (c < nrhs) {
-                                const C xv = X[j + 64 * u + n * c];
-#pragma unroll
-                                for (int a = 0; a < RW; ++a) cfma(acc[a][c], l[u][a], xv);
-                            }
-                        }
-                }
-                for (; j < j1; j += 64) {
-#pragma unroll
-                    for (int a = 0; a < RW; ++a) {
-                        const C lv = rows[a][j];
-#pragma unroll
-                        for (int c = 0; c < NR; ++c)
-                            if (c < nrhs) cfma(acc[a][c], lv, X[j + n * c]);
-                    }
-                }
-#pragma unroll
-                for (int a = 0; a < RW; ++a)
-#pragma unroll
-                    for (int c = 0; c < NR; ++c) {
-                        C v = wave_sum(acc[a][c]);
-                        if (lane == c && c < nrhs && r0 + a < bs)
-                            Y[c][r0 + a] = B[b0 + r0 + a + n * c] - v;
-                    }
-            }
-            __syncthreads();
-            if (wave < nrhs) {
-                C y = lane < bs ? Y[wave][lane] : zero;
-                if (fwd) {
-                    for (int r = 0; r < bs; ++r) {
-                        C xr = shfl_c(y, r);
-                        if (lane > r && lane < bs) cfma(y, -D[lane][r], xr);
-                    }
-                } else {
-                    for (int r = bs - 1; r >= 0; --r) {
-                        if (lane == r) y = y / D[r][r];
-                        C xr = shfl_c(y, r);
-                        if (lane < r) cfma(y, -D[lane][r], xr);
-                    }
-                }
-                if (lane < bs) {
-                    B[b0 + lane + n * wave] = y;
-                    if (XL) XS[b0 + lane + n * wave] = y;
-                }
-            }
-            __syncthreads();
         }
+    } else {
+        for (int r = ds - 1; r >= 0; --r) {
+            const C dl = Tt[r][lane];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (lane == r) y[s] = y[s] / dl;
+                const C xr = shfl_c(y[s], r);
+                if (lane < r) cfma(y[s], -dl, xr);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = q + 4 * s;
+        if (lane < ds && c < nc) X[d0 + lane + n * c] = y[s];
     }
 }
 
@@ -693,31 +699,48 @@ void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, c
     HIPCHECK(hipGetLastError());
 }
 
-// Register-resident panel for panels of up to 12288 rows, the streaming panel
-// beyond.  W is chosen so the R x W register block needs no scratch.
+// Rows held in registers by the leaf kernel: panels of up to this many rows are
+// factored in the column-major scratch, taller ones by the streaming k_panel.
+constexpr int64_t LEAF_MAX_ROWS = 12288;
+
+// Leaf width for a panel of m rows: as wide as R x LW complex registers allow
+// (R = ceil(m / 1024) rows per thread, at most 24 FP64 or 48 FP32 complex).
 template <class C>
-void launch_panel(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                  int batch, hipStream_t st) {
-    const int64_t m = n - k;
+int leaf_width(int64_t m) {
     constexpr bool F = sizeof(C) == sizeof(cf);
+    if (m <= 3072) return 8;
+    if (m <= 6144) return F ? 8 : 4;
+    return F ? 4 : 2;
+}
+
+template <class C>
+void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, int64_t sP,
+                 int64_t k, int batch, hipStream_t st) {
     dim3 g((unsigned)batch), b(1024);
-#define PR(RR, WW) hipLaunchKernelGGL((k_panel_reg<C, RR, WW>), g, b, 0, st, n, k, kb, A, sA, ipiv, sP)
-    if constexpr (F) {
-        if (m <= 1024) PR(1, 8);
-        else if (m <= 2048) PR(2, 8);
-        else if (m <= 3072) PR(3, 8);
-        else if (m <= 6144) PR(6, 4);
-        else if (m <= 12288) PR(12, 2);
-        else hipLaunchKernelGGL(k_panel<C>, g, b, 0, st, n, k, kb, A, sA, ipiv, sP);
+#define LF(RR, LL) hipLaunchKernelGGL((k_leaf<C, RR, LL>), g, b, 0, st, m, c0, lw, P, sPt, ipiv, sP, k)
+    constexpr bool F = sizeof(C) == sizeof(cf);
+    if (m <= 1024) LF(1, 8);
+    else if (m <= 2048) LF(2, 8);
+    else if (m <= 3072) LF(3, 8);
+    else if constexpr (F) {
+        if (m <= 6144) LF(6, 8);
+        else LF(12, 4);
     } else {
-        if (m <= 1024) PR(1, 8);
-        else if (m <= 2048) PR(2, 4);
-        else if (m <= 3072) PR(3, 4);
-        else if (m <= 6144) PR(6, 2);
-        else if (m <= 12288) PR(12, 1);
-        else hipLaunchKernelGGL(k_panel<C>, g, b, 0, st, n, k, kb, A, sA, ipiv, sP);
+        if (m <= 6144) LF(6, 4);
+        else LF(12, 2);
     }
-#undef PR
+#undef LF
+    HIPCHECK(hipGetLastError());
+}
+
+template <class C>
+void launch_leaf_apply(int64_t m, int kb, int c0, int lw, C *P, int64_t sPt, const int32_t *ipiv,
+                       int64_t sP, int64_t k, int batch, hipStream_t st) {
+    dim3 g((unsigned)batch), b(64);
+    if (lw > 4)
+        hipLaunchKernelGGL((k_leaf_apply<C, 8>), g, b, 0, st, m, kb, c0, lw, P, sPt, ipiv, sP, k);
+    else
+        hipLaunchKernelGGL((k_leaf_apply<C, 4>), g, b, 0, st, m, kb, c0, lw, P, sPt, ipiv, sP, k);
     HIPCHECK(hipGetLastError());
 }
 
@@ -758,34 +781,52 @@ void update_t(int64_t n, int64_t kk, int kw, int64_t c1, C *A, int64_t sA, int b
               one, A + r0 * n + r0, n, sA, batch, st);
 }
 
-// Panels taller than the register kernel's sweet spot: 8-column leaves factored by
-// the register kernel, each followed by its interchanges on the panel's other
-// columns, the U rows it needs and a rank-8 GEMM update of the panel's remaining
-// columns (multi-workgroup launches, so a few large systems still fill the chip).
+// Panel [k, k + kb) of rows [k, n): into the scratch, leaves of leaf_width columns
+// (each: factor, its interchanges and U rows on the other panel columns, rank-lw
+// GEMM update of the panel columns to its right), back into A.  Every launch
+// covers the whole batch.
 template <class C>
-void panel_recursive(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv,
-                     int64_t sP, int batch, hipStream_t st) {
-    constexpr int LW = 8;
-    for (int j0 = 0; j0 < kb; j0 += LW) {
-        const int lw = std::min(LW, kb - j0);
-        const int64_t kk = k + j0;
-        launch_panel<C>(n, kk, lw, A, sA, ipiv, sP, batch, st);
-        rowswap_t<C>(n, kk, lw, A, sA, ipiv, sP, k, k + kb, kk, kk + lw, batch, st);
-        trsm_u12_t<C>(n, kk, lw, A, sA, k + kb, batch, st);
-        update_t<C>(n, kk, lw, k + kb, A, sA, batch, st);
+void panel_leaves(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                  C *Pt, int64_t sPt, int batch, hipStream_t st) {
+    const int64_t m = n - k;
+    const dim3 gc((unsigned)((m + 63) / 64), (unsigned)batch);
+    hipLaunchKernelGGL(k_panel_copy<C>, gc, dim3(256), 0, st, n, k, kb, A, sA, Pt, sPt, 0);
+    HIPCHECK(hipGetLastError());
+    const int LW = leaf_width<C>(m);
+    C mone, one;
+    mone.x = -1; mone.y = 0;
+    one.x = 1; one.y = 0;
+    for (int c0 = 0; c0 < kb; c0 += LW) {
+        const int lw = std::min(LW, kb - c0);
+        launch_leaf<C>(m, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
+        if (kb > lw) launch_leaf_apply<C>(m, kb, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
+        const int64_t rr = m - c0 - lw;
+        const int rc = kb - c0 - lw;
+        if (rr > 0 && rc > 0)
+            gemm_t<C>((int)rr, rc, lw, mone, Pt + (int64_t)c0 * m + c0 + lw, m, sPt,
+                      Pt + (int64_t)(c0 + lw) * m + c0, m, sPt, one,
+                      Pt + (int64_t)(c0 + lw) * m + c0 + lw, m, sPt, batch, st);
     }
+    hipLaunchKernelGGL(k_panel_copy<C>, gc, dim3(256), 0, st, n, k, kb, A, sA, Pt, sPt, 1);
+    HIPCHECK(hipGetLastError());
 }
 
 template <class C>
 void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
                  hipStream_t st) {
+    // column-major panel scratch, n x NB per system
+    const int64_t sPt = n * NB;
+    DBuf<C> Pt((size_t)sPt * batch);
     for (int64_t k = 0; k < n; k += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k);
         stage_begin(3, st);
-        if (n - k > RECURSIVE_PANEL_ROWS)
-            panel_recursive<C>(n, k, kb, A, sA, ipiv, sP, batch, st);
-        else
-            launch_panel<C>(n, k, kb, A, sA, ipiv, sP, batch, st);
+        if (n - k <= LEAF_MAX_ROWS) {
+            panel_leaves<C>(n, k, kb, A, sA, ipiv, sP, Pt.p, sPt, batch, st);
+        } else {
+            hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(1024), 0, st, n, k, kb, A,
+                               sA, ipiv, sP);
+            HIPCHECK(hipGetLastError());
+        }
         rowswap_t<C>(n, k, kb, A, sA, ipiv, sP, 0, n, k, k + kb, batch, st);
         trsm_u12_t<C>(n, k, kb, A, sA, n, batch, st);
         stage_end(3, st);
@@ -799,41 +840,62 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
         }
     }
     stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
+    // the scratch is released once the factorisation has run
+    HIPCHECK(hipStreamSynchronize(st));
 }
 
-// Largest shared memory per workgroup on the current device (bytes).
-size_t max_lds_bytes() {
-    int dev = 0, v = 0;
-    HIPCHECK(hipGetDevice(&dev));
-    HIPCHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    return (size_t)v;
-}
-
+// P B, then L y = b and U x = y: 2 + 2 ceil(n / 64) launches of k_trsv_step on the
+// permuted copy in scratch, which is copied back into B.
 template <class C>
 void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *perm,
                 int64_t sP, C *B, int64_t sB, C *scratch, hipStream_t st,
                 const int32_t *sel = nullptr) {
     if (batch <= 0 || nrhs <= 0) return;
-    constexpr int NR = 4;
-    const size_t static_lds = sizeof(C) * (NB * (NB + 1) + NR * NB);
-    const size_t xl = sizeof(C) * (size_t)n * NR;
-    const bool useX = static_lds + xl <= max_lds_bytes();
-    dim3 g((unsigned)batch, (unsigned)((nrhs + NR - 1) / NR)), b(1024);
-    if constexpr (sizeof(C) == sizeof(cf)) {
-        if (useX)
-            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 4, true>), g, b, xl, st, n, nrhs, LU, sA,
-                               perm, sP, B, sB, scratch, sel);
-        else
-            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 4, false>), g, b, 0, st, n, nrhs, LU, sA,
-                               perm, sP, B, sB, scratch, sel);
-    } else {
-        if (useX)
-            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 4, true>), g, b, xl, st, n, nrhs, LU, sA,
-                               perm, sP, B, sB, scratch, sel);
-        else
-            hipLaunchKernelGGL((k_trsv_ll<C, NR, 2, 2, false>), g, b, 0, st, n, nrhs, LU, sA,
-                               perm, sP, B, sB, scratch, sel);
+    const int64_t cnt = n * nrhs;
+    const dim3 gp((unsigned)std::min<int64_t>((cnt + 255) / 256, 1024), (unsigned)batch);
+    hipLaunchKernelGGL(k_perm_rows<C>, gp, dim3(256), 0, st, n, nrhs, perm, sP, B, scratch, sB,
+                       sel);
+    HIPCHECK(hipGetLastError());
+    const int nrg = nrhs <= 4 ? 4 : 8;
+    const unsigned gy = (unsigned)((nrhs + nrg - 1) / nrg);
+    const int64_t nblk = (n + 63) / 64;
+    auto step = [&](bool up, int64_t s0, int ss, int64_t d0, int ds, int64_t ntile) {
+        const dim3 g((unsigned)(1 + ntile), gy, (unsigned)batch);
+        if (nrg == 4) {
+            if (up)
+                hipLaunchKernelGGL((k_trsv_step<C, 4, true>), g, dim3(256), 0, st, n, nrhs, LU,
+                                   sA, scratch, sB, sel, s0, ss, d0, ds);
+            else
+                hipLaunchKernelGGL((k_trsv_step<C, 4, false>), g, dim3(256), 0, st, n, nrhs, LU,
+                                   sA, scratch, sB, sel, s0, ss, d0, ds);
+        } else {
+            if (up)
+                hipLaunchKernelGGL((k_trsv_step<C, 8, true>), g, dim3(256), 0, st, n, nrhs, LU,
+                                   sA, scratch, sB, sel, s0, ss, d0, ds);
+            else
+                hipLaunchKernelGGL((k_trsv_step<C, 8, false>), g, dim3(256), 0, st, n, nrhs, LU,
+                                   sA, scratch, sB, sel, s0, ss, d0, ds);
+        }
+        HIPCHECK(hipGetLastError());
+    };
+    // forward: block 0, then each solved block b updates the rows below and
+    // workgroup 0 solves block b + 1
+    step(false, 0, 0, 0, (int)std::min<int64_t>(64, n), 0);
+    for (int64_t b = 0; b + 1 < nblk; ++b) {
+        const int64_t d0 = 64 * (b + 1);
+        const int ds = (int)std::min<int64_t>(64, n - d0);
+        step(false, 64 * b, 64, d0, ds, (n - d0 - ds + 63) / 64);
     }
+    // backward: the last block, then each solved block b updates the rows above
+    // and workgroup 0 solves block b - 1
+    const int64_t dl = 64 * (nblk - 1);
+    step(true, 0, 0, dl, (int)(n - dl), 0);
+    for (int64_t b = nblk - 1; b >= 1; --b) {
+        const int64_t s0 = 64 * b;
+        const int64_t d0 = 64 * (b - 1);
+        step(true, s0, (int)std::min<int64_t>(64, n - s0), d0, 64, d0 / 64);
+    }
+    hipLaunchKernelGGL(k_copy_sys<C>, gp, dim3(256), 0, st, cnt, scratch, B, sB, sel);
     HIPCHECK(hipGetLastError());
 }
 
