@@ -156,6 +156,9 @@ void launch_fill_gather(const SheetDev &sh, const FillParams *d_fp, int npts,
 void zgemm_batched(int m, int n, int k, cd alpha, const cd *A, int64_t lda, int64_t sA,
                    const cd *B, int64_t ldb, int64_t sB, cd beta, cd *C, int64_t ldc,
                    int64_t sC, int batch, hipStream_t st);
+void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int64_t sA,
+                   const cf *B, int64_t ldb, int64_t sB, cf beta, cf *C, int64_t ldc,
+                   int64_t sC, int batch, hipStream_t st);
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
                        hipStream_t st);
 // perm from lu_make_perm; scratch holds batch * sB entries.

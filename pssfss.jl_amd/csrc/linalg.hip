@@ -11,16 +11,23 @@
 
 namespace {
 
-constexpr int NB = 64;  // panel width
+constexpr int NB = 128;  // panel width (trailing updates are rank-128)
+constexpr int SB = 64;   // sub-panel width: triangular blocks held in LDS
 
 // ------------------------------------------------------------------ GEMM (NN)
 // C = alpha*A*B + beta*C, column-major complex, batched by blockIdx.z, on the
 // matrix cores.  A complex product is a real product with K doubled: with
 // (re, im) interleaved along k, Re C = [Ar, -Ai] [Br; Bi] and
 // Im C = [Ai, Ar] [Br; Bi], so B feeds the 16x16x4 MFMA in its memory layout
-// and each 4-deep real step covers two complex k.  64x64 tile per workgroup,
-// 32x32 per wavefront, BK = 16 complex k per LDS stage, double-buffered.
-template <class C>
+// and each 4-deep real step covers two complex k.  The matrix cores compute the
+// transposed tile (B^T A^T), so the 16 lanes that share an output row of the
+// instruction hold 16 consecutive rows of the column-major C: the epilogue's
+// loads and stores are coalesced.  Four wavefronts in 2 x 2, each owning
+// (16 WT) x (16 WT) of the (32 WT) x (32 WT) workgroup tile; BK = 16 complex k
+// per LDS stage, double-buffered.  WT = 2 (64 x 64) is what the library
+// launches: the 128 x 128 tile (WT = 4) runs one wavefront per SIMD and measured
+// slower on MI355X.
+template <class C, int WT>
 __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                                                    const C *__restrict__ A, int64_t lda,
                                                    int64_t sA, const C *__restrict__ B,
@@ -29,9 +36,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     typedef typename CTraits<C>::R R;
     typedef MfmaT<R> MT;
     typedef typename MT::V V;
-    constexpr int BM = 64, BN = 64, BK = 16;
+    constexpr int BM = 32 * WT, BN = 32 * WT, BK = 16, WS = 16 * WT;
+    constexpr int NL = BM * BK / 256;   // elements of each operand per thread and stage
     __shared__ C As[2][BK][BM];
-    __shared__ C Bs[2][BK][BN];
+    __shared__ C Bs[2][BK][BN + 1];   // padded: the k-fastest stores spread over banks
     const int64_t bz = blockIdx.z;
     A += bz * sA;
     B += bz * sB;
@@ -40,16 +48,16 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
     const C zero = CTraits<C>::zero();
-    V acc[2][2][2];
+    V acc[WT][WT][2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WT; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < WT; ++b)
 #pragma unroll
             for (int c = 0; c < 2; ++c) acc[a][b][c] = V{0, 0, 0, 0};
-    C ra[4], rb[4];
+    C ra[NL], rb[NL];
 #define GEMM_LOAD(K0)                                                                          \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                           \
+    _Pragma("unroll") for (int q = 0; q < NL; ++q) {                                          \
         const int t = tid + 256 * q;                                                           \
         const int i = t % BM, kk = t / BM;                                                     \
         ra[q] = zero;                                                                          \
@@ -59,7 +67,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
         if ((K0) + kb < k && n0 + j < n) rb[q] = B[((K0) + kb) + ldb * (n0 + j)];              \
     }
 #define GEMM_STORE(BUF)                                                                        \
-    _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                           \
+    _Pragma("unroll") for (int q = 0; q < NL; ++q) {                                          \
         const int t = tid + 256 * q;                                                           \
         As[BUF][t / BM][t % BM] = ra[q];                                                       \
         Bs[BUF][t % BK][t / BK] = rb[q];                                                       \
@@ -70,30 +78,32 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     __syncthreads();
     const int c = (lane >> 4) & 1;        // re / im half of the doubled k
     const int kh = lane >> 5;             // which of the two complex k of a step
+    // operands of two steps in flight: fewer registers, more waves per SIMD
+    constexpr int UNR = 2;
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < nk) { GEMM_LOAD((kt + 1) * BK) }
-#pragma unroll
+#pragma unroll UNR
         for (int s2 = 0; s2 < BK / 2; ++s2) {
             const int kc = 2 * s2 + kh;
-            R are[2], aim[2], bop[2];
+            R are[WT], aim[WT], bop[WT];
 #pragma unroll
-            for (int ti = 0; ti < 2; ++ti) {
-                const C a = As[buf][kc][32 * wm + 16 * ti + (lane & 15)];
+            for (int ti = 0; ti < WT; ++ti) {
+                const C a = As[buf][kc][WS * wm + 16 * ti + (lane & 15)];
                 are[ti] = c == 0 ? a.x : -a.y;
                 aim[ti] = c == 0 ? a.y : a.x;
             }
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj) {
-                const C b = Bs[buf][kc][32 * wn + 16 * tj + (lane & 15)];
+            for (int tj = 0; tj < WT; ++tj) {
+                const C b = Bs[buf][kc][WS * wn + 16 * tj + (lane & 15)];
                 bop[tj] = c == 0 ? b.x : b.y;
             }
 #pragma unroll
-            for (int ti = 0; ti < 2; ++ti)
+            for (int ti = 0; ti < WT; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < 2; ++tj) {
-                    acc[ti][tj][0] = MT::mma(are[ti], bop[tj], acc[ti][tj][0]);
-                    acc[ti][tj][1] = MT::mma(aim[ti], bop[tj], acc[ti][tj][1]);
+                for (int tj = 0; tj < WT; ++tj) {
+                    acc[ti][tj][0] = MT::mma(bop[tj], are[ti], acc[ti][tj][0]);
+                    acc[ti][tj][1] = MT::mma(bop[tj], aim[ti], acc[ti][tj][1]);
                 }
         }
         if (kt + 1 < nk) { GEMM_STORE(buf ^ 1) }
@@ -103,13 +113,13 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
 #undef GEMM_STORE
     const bool use_beta = !(beta.x == 0 && beta.y == 0);
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+    for (int ti = 0; ti < WT; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
+        for (int tj = 0; tj < WT; ++tj)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t i = m0 + 32 * wm + 16 * ti + MT::row(lane, e);
-                const int64_t j = n0 + 32 * wn + 16 * tj + (lane & 15);
+                const int64_t i = m0 + WS * wm + 16 * ti + (lane & 15);
+                const int64_t j = n0 + WS * wn + 16 * tj + MT::row(lane, e);
                 if (i < m && j < n) {
                     C v;
                     v.x = acc[ti][tj][0][e];
@@ -207,26 +217,29 @@ __global__ __launch_bounds__(256) void k_panel_copy(int64_t n, int64_t k, int kb
     C *A = A0 + (int64_t)blockIdx.y * sA;
     C *P = P0 + (int64_t)blockIdx.y * sPt;
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
-    if (dir == 0) {
-        for (int p = 0; p < 16; ++p) {
-            const int rr = q + 4 * p;
-            if (r0 + rr < m && lane < kb) T[rr][lane] = A[(k + r0 + rr) * n + k + lane];
+    for (int cb = 0; cb < kb; cb += 64) {   // 64-column blocks of the panel
+        if (dir == 0) {
+            for (int p = 0; p < 16; ++p) {
+                const int rr = q + 4 * p;
+                if (r0 + rr < m && cb + lane < kb) T[rr][lane] = A[(k + r0 + rr) * n + k + cb + lane];
+            }
+            __syncthreads();
+            for (int p = 0; p < 16; ++p) {
+                const int cc = q + 4 * p;
+                if (r0 + lane < m && cb + cc < kb) P[(cb + cc) * m + r0 + lane] = T[lane][cc];
+            }
+        } else {
+            for (int p = 0; p < 16; ++p) {
+                const int cc = q + 4 * p;
+                if (r0 + lane < m && cb + cc < kb) T[lane][cc] = P[(cb + cc) * m + r0 + lane];
+            }
+            __syncthreads();
+            for (int p = 0; p < 16; ++p) {
+                const int rr = q + 4 * p;
+                if (r0 + rr < m && cb + lane < kb) A[(k + r0 + rr) * n + k + cb + lane] = T[rr][lane];
+            }
         }
         __syncthreads();
-        for (int p = 0; p < 16; ++p) {
-            const int cc = q + 4 * p;
-            if (r0 + lane < m && cc < kb) P[cc * m + r0 + lane] = T[lane][cc];
-        }
-    } else {
-        for (int p = 0; p < 16; ++p) {
-            const int cc = q + 4 * p;
-            if (r0 + lane < m && cc < kb) T[lane][cc] = P[cc * m + r0 + lane];
-        }
-        __syncthreads();
-        for (int p = 0; p < 16; ++p) {
-            const int rr = q + 4 * p;
-            if (r0 + rr < m && lane < kb) A[(k + r0 + rr) * n + k + lane] = T[rr][lane];
-        }
     }
 }
 
@@ -343,12 +356,13 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
 }
 
 // After a leaf: its interchanges on the panel's other columns (in pivot order),
-// and the leaf's U rows for the columns to its right (unit-lower solve with the
-// leaf's lw x lw block).  One thread per panel column.
+// and the leaf's U rows for the columns to its right up to ce, the end of its
+// sub-panel (unit-lower solve with the leaf's lw x lw block).  One thread per
+// panel column.
 template <class C, int LW>
-__global__ __launch_bounds__(64) void k_leaf_apply(int64_t m, int kb, int c0, int lw, C *P0,
-                                                   int64_t sPt, const int32_t *ipiv0, int64_t sP,
-                                                   int64_t k) {
+__global__ __launch_bounds__(NB) void k_leaf_apply(int64_t m, int kb, int ce, int c0, int lw,
+                                                   C *P0, int64_t sPt, const int32_t *ipiv0,
+                                                   int64_t sP, int64_t k) {
     C *P = P0 + (int64_t)blockIdx.x * sPt;
     const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
     __shared__ C L[LW][LW];
@@ -368,7 +382,7 @@ __global__ __launch_bounds__(64) void k_leaf_apply(int64_t m, int kb, int c0, in
             col[p] = t;
         }
     }
-    if (c >= c0 + lw) {
+    if (c >= c0 + lw && c < ce) {
         C u[LW];
 #pragma unroll
         for (int j = 0; j < LW; ++j) {
@@ -380,6 +394,31 @@ __global__ __launch_bounds__(64) void k_leaf_apply(int64_t m, int kb, int c0, in
                 col[c0 + j] = v;
             }
         }
+    }
+}
+
+// U rows [0, SB) of the scratch panel's second sub-panel: L00^{-1} times columns
+// [SB, kb) (L00 the unit-lower SB x SB block), one thread per column.
+template <class C>
+__global__ __launch_bounds__(64) void k_pt_trsm(int64_t m, int kb, C *P0, int64_t sPt) {
+    C *P = P0 + (int64_t)blockIdx.x * sPt;
+    __shared__ C L[SB][SB];
+    for (int t = threadIdx.x; t < SB * SB; t += 64) {
+        const int r = t % SB, l = t / SB;
+        L[r][l] = P[(int64_t)l * m + r];
+    }
+    __syncthreads();
+    const int c = SB + threadIdx.x;
+    if (c >= kb) return;
+    C *col = P + (int64_t)c * m;
+    C x[SB];
+#pragma unroll
+    for (int r = 0; r < SB; ++r) {
+        C v = col[r];
+#pragma unroll
+        for (int l = 0; l < r; ++l) cfma(v, -L[r][l], x[l]);
+        x[r] = v;
+        col[r] = v;
     }
 }
 
@@ -406,23 +445,23 @@ __global__ void k_rowswap(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
     }
 }
 
-// U12 = L11^{-1} A12 (unit lower L11) for columns [k + kb, cend), one thread per
-// column.
+// U12 = L11^{-1} A12 (unit lower L11 = A[k, k + kb) x [k, k + kb), kb <= 64) for
+// the rows [k, k + kb) and columns [cbeg, cend), one thread per column.
 template <class C>
 __global__ __launch_bounds__(64) void k_trsm_u12(int64_t n, int64_t k, int kb, C *A0,
-                                                 int64_t sA, int64_t cend) {
+                                                 int64_t sA, int64_t cbeg, int64_t cend) {
     C *A = A0 + (int64_t)blockIdx.y * sA;
-    __shared__ C L[NB][NB];
+    __shared__ C L[SB][SB];
     for (int t = threadIdx.x; t < kb * kb; t += blockDim.x) {
         int r = t / kb, l = t % kb;
         L[r][l] = A[(k + r) * n + k + l];
     }
     __syncthreads();
-    const int64_t c = k + kb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = cbeg + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cend) return;
-    C x[NB];
+    C x[SB];
 #pragma unroll
-    for (int r = 0; r < NB; ++r) {
+    for (int r = 0; r < SB; ++r) {
         if (r < kb) {
             C v = A[(k + r) * n + c];
 #pragma unroll
@@ -694,8 +733,8 @@ void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, c
             hipStream_t st) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
     dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
-    hipLaunchKernelGGL(k_gemm_mfma<C>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA, B, ldb,
-                       sB, beta, Cm, ldc, sC);
+    hipLaunchKernelGGL((k_gemm_mfma<C, 2>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA, B,
+                       ldb, sB, beta, Cm, ldc, sC);
     HIPCHECK(hipGetLastError());
 }
 
@@ -734,13 +773,15 @@ void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, in
 }
 
 template <class C>
-void launch_leaf_apply(int64_t m, int kb, int c0, int lw, C *P, int64_t sPt, const int32_t *ipiv,
-                       int64_t sP, int64_t k, int batch, hipStream_t st) {
-    dim3 g((unsigned)batch), b(64);
+void launch_leaf_apply(int64_t m, int kb, int ce, int c0, int lw, C *P, int64_t sPt,
+                       const int32_t *ipiv, int64_t sP, int64_t k, int batch, hipStream_t st) {
+    dim3 g((unsigned)batch), b(NB);
     if (lw > 4)
-        hipLaunchKernelGGL((k_leaf_apply<C, 8>), g, b, 0, st, m, kb, c0, lw, P, sPt, ipiv, sP, k);
+        hipLaunchKernelGGL((k_leaf_apply<C, 8>), g, b, 0, st, m, kb, ce, c0, lw, P, sPt, ipiv, sP,
+                           k);
     else
-        hipLaunchKernelGGL((k_leaf_apply<C, 4>), g, b, 0, st, m, kb, c0, lw, P, sPt, ipiv, sP, k);
+        hipLaunchKernelGGL((k_leaf_apply<C, 4>), g, b, 0, st, m, kb, ce, c0, lw, P, sPt, ipiv, sP,
+                           k);
     HIPCHECK(hipGetLastError());
 }
 
@@ -756,14 +797,25 @@ void rowswap_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, const int32_t *ip
     HIPCHECK(hipGetLastError());
 }
 
+// U12 of panel [k, k + kb) for the columns [k + kb, n): the first SB rows against
+// their unit-lower triangle, the rank-SB update of the remaining panel rows (a
+// GEMM on the transposed row-major views), then their own triangle.
 template <class C>
-void trsm_u12_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, int64_t cend, int batch,
-                hipStream_t st) {
-    const int64_t cols = cend - (k + kb);
+void trsm_u12_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, int batch, hipStream_t st) {
+    const int64_t c0 = k + kb, cols = n - c0;
     if (cols <= 0) return;
     dim3 g((unsigned)((cols + 63) / 64), (unsigned)batch);
-    hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k, kb, A, sA, cend);
+    hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k, std::min(kb, SB), A, sA, c0, n);
     HIPCHECK(hipGetLastError());
+    if (kb > SB) {
+        C mone, one;
+        mone.x = -1; mone.y = 0;
+        one.x = 1; one.y = 0;
+        gemm_t<C>((int)cols, kb - SB, SB, mone, A + k * n + c0, n, sA, A + (k + SB) * n + k, n, sA,
+                  one, A + (k + SB) * n + c0, n, sA, batch, st);
+        hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k + SB, kb - SB, A, sA, c0, n);
+        HIPCHECK(hipGetLastError());
+    }
 }
 
 // A[r0:n, c0:c1] -= A[r0:n, kk:kk+kw] A[kk:kk+kw, c0:c1] (row-major, r0 = c0 = kk + kw)
@@ -781,10 +833,11 @@ void update_t(int64_t n, int64_t kk, int kw, int64_t c1, C *A, int64_t sA, int b
               one, A + r0 * n + r0, n, sA, batch, st);
 }
 
-// Panel [k, k + kb) of rows [k, n): into the scratch, leaves of leaf_width columns
-// (each: factor, its interchanges and U rows on the other panel columns, rank-lw
-// GEMM update of the panel columns to its right), back into A.  Every launch
-// covers the whole batch.
+// Panel [k, k + kb) of rows [k, n): into the scratch, then per sub-panel of SB
+// columns leaves of leaf_width columns (each: factor, its interchanges on the
+// other panel columns and U rows on the rest of its sub-panel, rank-lw GEMM
+// update of those columns); between the sub-panels the second one's U rows and a
+// rank-SB GEMM update; back into A.  Every launch covers the whole batch.
 template <class C>
 void panel_leaves(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
                   C *Pt, int64_t sPt, int batch, hipStream_t st) {
@@ -796,16 +849,27 @@ void panel_leaves(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv,
     C mone, one;
     mone.x = -1; mone.y = 0;
     one.x = 1; one.y = 0;
-    for (int c0 = 0; c0 < kb; c0 += LW) {
-        const int lw = std::min(LW, kb - c0);
-        launch_leaf<C>(m, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
-        if (kb > lw) launch_leaf_apply<C>(m, kb, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
-        const int64_t rr = m - c0 - lw;
-        const int rc = kb - c0 - lw;
-        if (rr > 0 && rc > 0)
-            gemm_t<C>((int)rr, rc, lw, mone, Pt + (int64_t)c0 * m + c0 + lw, m, sPt,
-                      Pt + (int64_t)(c0 + lw) * m + c0, m, sPt, one,
-                      Pt + (int64_t)(c0 + lw) * m + c0 + lw, m, sPt, batch, st);
+    for (int sb0 = 0; sb0 < kb; sb0 += SB) {
+        const int ce = std::min(kb, sb0 + SB);
+        if (sb0 > 0) {
+            hipLaunchKernelGGL(k_pt_trsm<C>, dim3((unsigned)batch), dim3(64), 0, st, m, kb, Pt,
+                               sPt);
+            HIPCHECK(hipGetLastError());
+            gemm_t<C>((int)(m - SB), kb - SB, SB, mone, Pt + SB, m, sPt, Pt + (int64_t)SB * m, m,
+                      sPt, one, Pt + (int64_t)SB * m + SB, m, sPt, batch, st);
+        }
+        for (int c0 = sb0; c0 < ce; c0 += LW) {
+            const int lw = std::min(LW, ce - c0);
+            launch_leaf<C>(m, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
+            if (kb > lw)
+                launch_leaf_apply<C>(m, kb, ce, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
+            const int64_t rr = m - c0 - lw;
+            const int rc = ce - c0 - lw;
+            if (rr > 0 && rc > 0)
+                gemm_t<C>((int)rr, rc, lw, mone, Pt + (int64_t)c0 * m + c0 + lw, m, sPt,
+                          Pt + (int64_t)(c0 + lw) * m + c0, m, sPt, one,
+                          Pt + (int64_t)(c0 + lw) * m + c0 + lw, m, sPt, batch, st);
+        }
     }
     hipLaunchKernelGGL(k_panel_copy<C>, gc, dim3(256), 0, st, n, k, kb, A, sA, Pt, sPt, 1);
     HIPCHECK(hipGetLastError());
@@ -828,7 +892,7 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
             HIPCHECK(hipGetLastError());
         }
         rowswap_t<C>(n, k, kb, A, sA, ipiv, sP, 0, n, k, k + kb, batch, st);
-        trsm_u12_t<C>(n, k, kb, A, sA, n, batch, st);
+        trsm_u12_t<C>(n, k, kb, A, sA, batch, st);
         stage_end(3, st);
         const int64_t rest = n - k - kb;
         if (rest > 0) {
@@ -905,6 +969,12 @@ void zgemm_batched(int m, int n, int k, cd alpha, const cd *A, int64_t lda, int6
                    const cd *B, int64_t ldb, int64_t sB, cd beta, cd *C, int64_t ldc,
                    int64_t sC, int batch, hipStream_t st) {
     gemm_t<cd>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st);
+}
+
+void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int64_t sA,
+                   const cf *B, int64_t ldb, int64_t sB, cf beta, cf *C, int64_t ldc,
+                   int64_t sC, int batch, hipStream_t st) {
+    gemm_t<cf>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st);
 }
 
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
