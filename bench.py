@@ -13,7 +13,9 @@ refinement, singular terms and tables); the plain FP64 path is timed beside it
 and max|dS| between them is reported, as is max|dS| of both against the oracle
 on the CPU-baseline sample point.  detail.two_layer_patch holds the
 differentiation workload: forward points/s in FP64 and mixed, and adjoint
-gradient vectors per second.
+gradient vectors per second; detail.meander_polarizer a sample of the cascaded
+4-sheet meanderline polarizer at full mesh size (about 12 k unknowns per sheet);
+detail.cross_batch a sample of the batched Jerusalem / loaded cross problems.
 """
 import argparse
 import json
@@ -52,6 +54,10 @@ def main():
     ap.add_argument("--precisions", default="fp64,mixed")
     ap.add_argument("--no-two-layer", action="store_true",
                     help="skip the two-layer patch forward / adjoint measurements")
+    ap.add_argument("--no-meander", action="store_true",
+                    help="skip the 4-sheet meanderline polarizer sample")
+    ap.add_argument("--no-cross", action="store_true",
+                    help="skip the batched cross-problem sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -148,6 +154,10 @@ def main():
 
     if not args.no_two_layer:
         extra["two_layer_patch"] = two_layer_patch(pk, hip, ctx)
+    if not args.no_meander:
+        extra["meander_polarizer"] = meander_polarizer(pk, hip, ctx)
+    if not args.no_cross:
+        extra["cross_batch"] = cross_batch(pk, hip, ctx)
 
     cpu = None
     if not args.no_cpu:
@@ -205,6 +215,74 @@ def two_layer_patch(pk, hip, ctx):
     out["adjoint_s"] = round(dt, 3)
     print(f"[bench] two-layer patch adjoint: {dt:.3f} s for {grad.shape[0]} points x "
           f"{grad.shape[1]} objectives x {grad.shape[2]} parameters", flush=True)
+    return out
+
+
+def _sweep(hip, ctx, setup, label):
+    """FP64 and mixed runs of a prepared sweep (one untimed warm-up each); points
+    are those analysed (the sweep skips cut-off points as the reference does)."""
+    out, res = {}, {}
+    for p in ("fp64", "mixed"):
+        s = hip.SheetSolver(ctx, setup, p)
+        s.run()
+        t0 = time.perf_counter()
+        res[p], info = s.run()
+        dt = time.perf_counter() - t0
+        s.close()
+        npts = len(res[p])
+        out["points"] = npts
+        out[f"points_per_s_{p}"] = round(npts / dt, 4)
+        if p == "mixed":
+            it = [i.refine_iters for i in info["infos"]]
+            out["mixed_refinement_sweeps_min_max"] = [min(it), max(it)]
+            out["mixed_fp64_fallbacks"] = int(sum(i.fallback_fp64 for i in info["infos"]))
+        print(f"[bench] {label} {p}: {dt:.3f} s for {npts} points", flush=True)
+    out["speedup_mixed_vs_fp64"] = round(out["points_per_s_mixed"] / out["points_per_s_fp64"], 3)
+    out["max_dS_mixed_vs_fp64"] = max_dS(res["mixed"], res["fp64"])
+    return out
+
+
+def meander_polarizer(pk, hip, ctx):
+    """Four seeded meanderline sheets of about 8000 faces (11.3-12.2 k unknowns
+    each) on eps_r 1.07 foam with eps_r 2.8 skins, 30 modes per foam layer
+    (dbmin 60), 2 frequencies (8, 14 GHz) at theta 0 and 40 deg: 4 points, 16
+    sheet systems, 32-60 right-hand sides per system.  Face-pair classes come from
+    the device classifier."""
+    strata, freqs, steer = pk.workloads.meander_polarizer(ntri=8000, nfreq=2)
+    t0 = time.perf_counter()
+    setup = pk.analyze.prepare(strata, freqs, steer, dbmin=60.0,
+                               classify=lambda sh: hip.face_pair_classes(ctx, sh))
+    prep = time.perf_counter() - t0
+    out = {"unknowns": [int(r.nbf) for r in setup.rwgdat],
+           "face_pair_classes": [int(r.nufp) for r in setup.rwgdat],
+           "modes_per_layer": [len(l.P) for l in setup.layers],
+           "prepare_s": round(prep, 3)}
+    out.update(_sweep(hip, ctx, setup, "meander polarizer"))
+    return out
+
+
+def cross_batch(pk, hip, ctx):
+    """The first Jerusalem cross and the first loaded cross of the seeded 16-problem
+    batch (full meshes), each at 8 of its 64 frequencies (every 8th from 1 GHz) and
+    the 4 scan angles: 64 points of two uneven problem sizes, run one problem after
+    the other."""
+    probs = pk.workloads.cross_batch()[:2]
+    out = {"problems": [], "points": 0}
+    tot = {"fp64": 0.0, "mixed": 0.0}
+    for strata, freqs, steer, par in probs:
+        f = list(freqs[::8])
+        setup = pk.analyze.prepare(strata, f, steer)
+        r = _sweep(hip, ctx, setup, par["kind"])
+        npts = r["points"]
+        r.update({"kind": par["kind"], "period_mm": round(par["P"], 3),
+                  "faces": int(strata[1].nface), "unknowns": int(setup.rwgdat[0].nbf)})
+        out["problems"].append(r)
+        out["points"] += npts
+        for p in tot:
+            tot[p] += npts / r[f"points_per_s_{p}"]
+    for p in tot:
+        out[f"points_per_s_{p}"] = round(out["points"] / tot[p], 4)
+    out["speedup_mixed_vs_fp64"] = round(tot["fp64"] / tot["mixed"], 3)
     return out
 
 

@@ -633,23 +633,14 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wsel.ensure((size_t)np);
     HIPCHECK(hipMemcpyAsync(ctx->wsel.p, act.data(), np * sizeof(int32_t),
                             hipMemcpyHostToDevice, st));
+    // R = B - Z X and the column maxima of R and X for the listed systems
     auto residual = [&](const cd *X) {
         const int na = (int)act.size();
-        for (int c0 = 0; c0 < ncol; c0 += 8) {
-            int nr = std::min(8, ncol - c0);
-            // norms are per (system, rhs) over a strided sub-block
-            residual_batched(n, nr, np, ctx->wZ.p, nb2, X + c0 * n, ctx->wB.p + c0 * n, sB,
-                             ctx->wR.p + c0 * n, rn, xn, st, ctx->wsel.p, na);
-            std::vector<double> tr(np * nr), tx(np * nr);
-            d2h(ctx, tr.data(), rn, np * nr);
-            d2h(ctx, tx.data(), xn, np * nr);
-            sync(ctx);
-            for (int p : act)
-                for (int c = 0; c < nr; ++c) {
-                    hr[p * ncol + c0 + c] = tr[p * nr + c];
-                    hx[p * ncol + c0 + c] = tx[p * nr + c];
-                }
-        }
+        residual_batched(n, ncol, np, ctx->wZ.p, nb2, X, ctx->wB.p, sB, ctx->wR.p, rn, xn, st,
+                         ctx->wsel.p, na);
+        d2h(ctx, hr.data(), rn, np * ncol);
+        d2h(ctx, hx.data(), xn, np * ncol);
+        sync(ctx);
     };
     // ||A||_inf of each system for the stopping rule of LAPACK's zcgesv:
     // max_c ||r_c|| < sqrt(n) ||x_c|| ||A|| eps  (backward error at FP64 level)

@@ -174,7 +174,9 @@ void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t 
                           const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
                           hipStream_t st, const int32_t *sel = nullptr);
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st);
-// r = b - A x (FP64), batched; then per-(system, rhs) norms of r and of x.
+// r = b - A x (FP64; a row-per-wavefront kernel for up to 8 right-hand sides, else
+// one transposed-A GEMM per system), batched; then the largest
+// |re| or |im| of each column of r and of x, at rnorm/xnorm[system * nrhs + rhs].
 // With sel, only the nsel listed systems are computed (norms of the others are 0).
 void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
                       const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,

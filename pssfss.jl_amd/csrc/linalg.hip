@@ -26,21 +26,22 @@ constexpr int SB = 64;   // sub-panel width: triangular blocks held in LDS
 // (16 WT) x (16 WT) of the (32 WT) x (32 WT) workgroup tile; BK = 16 complex k
 // per LDS stage, double-buffered.  WT = 2 (64 x 64) is what the library
 // launches: the 128 x 128 tile (WT = 4) runs one wavefront per SIMD and measured
-// slower on MI355X.
-template <class C, int WT>
+// slower on MI355X.  TA: A is given transposed (C = A^T B, A stored k x m), which
+// is a row-major m x k operand such as Z.  sel (optional) lists the batch entries.
+template <class C, int WT, bool TA>
 __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                                                    const C *__restrict__ A, int64_t lda,
                                                    int64_t sA, const C *__restrict__ B,
                                                    int64_t ldb, int64_t sB, C beta, C *Cm,
-                                                   int64_t ldc, int64_t sC) {
+                                                   int64_t ldc, int64_t sC, const int32_t *sel) {
     typedef typename CTraits<C>::R R;
     typedef MfmaT<R> MT;
     typedef typename MT::V V;
     constexpr int BM = 32 * WT, BN = 32 * WT, BK = 16, WS = 16 * WT;
     constexpr int NL = BM * BK / 256;   // elements of each operand per thread and stage
-    __shared__ C As[2][BK][BM];
+    __shared__ C As[2][BK][BM + (TA ? 1 : 0)];
     __shared__ C Bs[2][BK][BN + 1];   // padded: the k-fastest stores spread over banks
-    const int64_t bz = blockIdx.z;
+    const int64_t bz = sel ? sel[blockIdx.z] : blockIdx.z;
     A += bz * sA;
     B += bz * sB;
     Cm += bz * sC;
@@ -59,9 +60,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
 #define GEMM_LOAD(K0)                                                                          \
     _Pragma("unroll") for (int q = 0; q < NL; ++q) {                                          \
         const int t = tid + 256 * q;                                                           \
-        const int i = t % BM, kk = t / BM;                                                     \
+        const int i = TA ? t / BK : t % BM, kk = TA ? t % BK : t / BM;                         \
         ra[q] = zero;                                                                          \
-        if (m0 + i < m && (K0) + kk < k) ra[q] = A[(m0 + i) + lda * ((K0) + kk)];              \
+        if (m0 + i < m && (K0) + kk < k)                                                       \
+            ra[q] = TA ? A[((K0) + kk) + lda * (m0 + i)] : A[(m0 + i) + lda * ((K0) + kk)];    \
         const int kb = t % BK, j = t / BK;                                                     \
         rb[q] = zero;                                                                          \
         if ((K0) + kb < k && n0 + j < n) rb[q] = B[((K0) + kb) + ldb * (n0 + j)];              \
@@ -69,7 +71,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
 #define GEMM_STORE(BUF)                                                                        \
     _Pragma("unroll") for (int q = 0; q < NL; ++q) {                                          \
         const int t = tid + 256 * q;                                                           \
-        As[BUF][t / BM][t % BM] = ra[q];                                                       \
+        if (TA) As[BUF][t % BK][t / BK] = ra[q];                                               \
+        else As[BUF][t / BM][t % BM] = ra[q];                                                  \
         Bs[BUF][t % BK][t / BK] = rb[q];                                                       \
     }
     const int nk = (k + BK - 1) / BK;
@@ -639,8 +642,10 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
     }
 }
 
-// R = B - A X for row-major A, column-major X/B/R (n x nrhs), one wavefront per row;
-// also max |R| and max |X| per (system, rhs), as monotone bit patterns.
+// R = B - A X for row-major A, column-major X/B/R (n x nrhs <= NR), one wavefront
+// per row, reading A once; also max |R| and max |X| per (system, rhs) as monotone
+// bit patterns.  Used for few right-hand sides, where the GEMM's 64-column tile
+// would be mostly idle.
 template <int NR>
 __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd *A0, int64_t sA,
                                                   const cd *X0, const cd *B0, int64_t sB,
@@ -659,7 +664,7 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd 
     for (int c = 0; c < NR; ++c) acc[c] = cd{0.0, 0.0};
     const cd *row = A + i * n;
     for (int64_t j = lane; j < n; j += 64) {
-        cd a = row[j];
+        const cd a = row[j];
 #pragma unroll
         for (int c = 0; c < NR; ++c)
             if (c < nrhs) cfma(acc[c], a, X[j + n * c]);
@@ -676,13 +681,39 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd 
 #pragma unroll
         for (int c = 0; c < NR; ++c)
             if (c == lane) a = acc[c];
-        cd r = Bv[i + n * lane] - a;
+        const cd r = Bv[i + n * lane] - a;
         R[i + n * lane] = r;
-        double rm = fmax(fabs(r.x), fabs(r.y));
-        cd xv = X[i + n * lane];
-        double xm = fmax(fabs(xv.x), fabs(xv.y));
+        const double rm = fmax(fabs(r.x), fabs(r.y));
+        const cd xv = X[i + n * lane];
+        const double xm = fmax(fabs(xv.x), fabs(xv.y));
         atomicMax(rmax + b * nrhs + lane, (unsigned long long)__double_as_longlong(rm));
         atomicMax(xmax + b * nrhs + lane, (unsigned long long)__double_as_longlong(xm));
+    }
+}
+
+// Largest |re| or |im| of each column of R and of X (n x nrhs per system), as
+// monotone bit patterns through atomicMax.  Grid: x = row chunk, y = rhs,
+// z = system (sel lists them when given).
+__global__ __launch_bounds__(256) void k_colmax(int64_t n, int nrhs, const cd *R0, const cd *X0,
+                                                int64_t sB, const int32_t *sel,
+                                                unsigned long long *rmax,
+                                                unsigned long long *xmax) {
+    const int64_t b = sel ? sel[blockIdx.z] : blockIdx.z;
+    const int c = blockIdx.y;
+    const cd *R = R0 + b * sB + (int64_t)c * n;
+    const cd *X = X0 + b * sB + (int64_t)c * n;
+    double rm = 0, xm = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        rm = fmax(rm, fmax(fabs(R[i].x), fabs(R[i].y)));
+        xm = fmax(xm, fmax(fabs(X[i].x), fabs(X[i].y)));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rm = fmax(rm, __shfl_xor(rm, off, 64));
+        xm = fmax(xm, __shfl_xor(xm, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(rmax + b * nrhs + c, (unsigned long long)__double_as_longlong(rm));
+        atomicMax(xmax + b * nrhs + c, (unsigned long long)__double_as_longlong(xm));
     }
 }
 
@@ -727,14 +758,14 @@ unsigned grid1(int64_t n) {
     return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
 }
 
-template <class C>
+template <class C, bool TA = false>
 void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, const C *B,
             int64_t ldb, int64_t sB, C beta, C *Cm, int64_t ldc, int64_t sC, int batch,
-            hipStream_t st) {
+            hipStream_t st, const int32_t *sel = nullptr) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
     dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
-    hipLaunchKernelGGL((k_gemm_mfma<C, 2>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA, B,
-                       ldb, sB, beta, Cm, ldc, sC);
+    hipLaunchKernelGGL((k_gemm_mfma<C, 2, TA>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA,
+                       B, ldb, sB, beta, Cm, ldc, sC, sel);
     HIPCHECK(hipGetLastError());
 }
 
@@ -920,26 +951,17 @@ void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const i
     hipLaunchKernelGGL(k_perm_rows<C>, gp, dim3(256), 0, st, n, nrhs, perm, sP, B, scratch, sB,
                        sel);
     HIPCHECK(hipGetLastError());
-    const int nrg = nrhs <= 4 ? 4 : 8;
+    // right-hand sides per workgroup: each group of them reads the factors once
+    const int nrg = nrhs <= 4 ? 4 : (nrhs <= 8 ? 8 : 16);
     const unsigned gy = (unsigned)((nrhs + nrg - 1) / nrg);
     const int64_t nblk = (n + 63) / 64;
     auto step = [&](bool up, int64_t s0, int ss, int64_t d0, int ds, int64_t ntile) {
         const dim3 g((unsigned)(1 + ntile), gy, (unsigned)batch);
-        if (nrg == 4) {
-            if (up)
-                hipLaunchKernelGGL((k_trsv_step<C, 4, true>), g, dim3(256), 0, st, n, nrhs, LU,
-                                   sA, scratch, sB, sel, s0, ss, d0, ds);
-            else
-                hipLaunchKernelGGL((k_trsv_step<C, 4, false>), g, dim3(256), 0, st, n, nrhs, LU,
-                                   sA, scratch, sB, sel, s0, ss, d0, ds);
-        } else {
-            if (up)
-                hipLaunchKernelGGL((k_trsv_step<C, 8, true>), g, dim3(256), 0, st, n, nrhs, LU,
-                                   sA, scratch, sB, sel, s0, ss, d0, ds);
-            else
-                hipLaunchKernelGGL((k_trsv_step<C, 8, false>), g, dim3(256), 0, st, n, nrhs, LU,
-                                   sA, scratch, sB, sel, s0, ss, d0, ds);
-        }
+#define TS(NRG, UP) hipLaunchKernelGGL((k_trsv_step<C, NRG, UP>), g, dim3(256), 0, st, n, nrhs, LU, sA, scratch, sB, sel, s0, ss, d0, ds)
+        if (nrg == 4) { if (up) TS(4, true); else TS(4, false); }
+        else if (nrg == 8) { if (up) TS(8, true); else TS(8, false); }
+        else { if (up) TS(16, true); else TS(16, false); }
+#undef TS
         HIPCHECK(hipGetLastError());
     };
     // forward: block 0, then each solved block b updates the rows below and
@@ -1015,18 +1037,31 @@ void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {
 void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
                       const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
                       hipStream_t st, const int32_t *sel, int nsel) {
-    if (nrhs > 8) throw ArgError("residual_batched: at most 8 right-hand sides per call");
     launch_zero((cd *)rnorm, (batch * nrhs + 1) / 2, st);
     launch_zero((cd *)xnorm, (batch * nrhs + 1) / 2, st);
     const int ns = sel ? nsel : batch;
-    if (ns <= 0) return;
-    dim3 g((unsigned)((n + 3) / 4), (unsigned)ns);
-    if (nrhs <= 4)
-        hipLaunchKernelGGL(k_residual<4>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
-                           (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
-    else
-        hipLaunchKernelGGL(k_residual<8>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
-                           (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
+    if (ns <= 0 || nrhs <= 0) return;
+    if (nrhs <= 8) {
+        dim3 g((unsigned)((n + 3) / 4), (unsigned)ns);
+        if (nrhs <= 4)
+            hipLaunchKernelGGL(k_residual<4>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
+                               (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
+        else
+            hipLaunchKernelGGL(k_residual<8>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
+                               (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
+        HIPCHECK(hipGetLastError());
+        return;
+    }
+    // R = B, then R -= Z X as one GEMM per system (Z row-major: the transposed operand)
+    const int64_t cnt = n * nrhs;
+    const dim3 gp((unsigned)std::min<int64_t>((cnt + 255) / 256, 1024), (unsigned)ns);
+    hipLaunchKernelGGL(k_copy_sys<cd>, gp, dim3(256), 0, st, cnt, Bv, R, sB, sel);
+    HIPCHECK(hipGetLastError());
+    gemm_t<cd, true>((int)n, nrhs, (int)n, cd{-1.0, 0.0}, A, n, sA, X, n, sB, cd{1.0, 0.0}, R, n,
+                     sB, ns, st, sel);
+    const dim3 gm((unsigned)std::min<int64_t>((n + 255) / 256, 16), (unsigned)nrhs, (unsigned)ns);
+    hipLaunchKernelGGL(k_colmax, gm, dim3(256), 0, st, n, nrhs, R, X, sB, sel,
+                       (unsigned long long *)rnorm, (unsigned long long *)xnorm);
     HIPCHECK(hipGetLastError());
 }
 

@@ -343,3 +343,146 @@ def annular_ring(*, a, b, P, nr, nphi, units, sheet_class="J", phi0=0.0, fufp=Fa
     sh.fufp = fufp
     sh.xieta_check = False
     return sh
+
+
+def _rect_union_mesh(solid, hollow, ntri: int):
+    """Structured triangulation of a rectilinear metal region: the union of the
+    axis-aligned rectangles `solid` minus the union of `hollow`, each given as
+    (x0, x1, y0, y1).  Every rectangle edge is a grid line, so the region is a
+    union of grid cells; intervals between edges are split evenly with a step
+    h = sqrt(2 area / ntri), and each metal cell becomes two counter-clockwise
+    triangles (the diagonal alternating like a checkerboard).  Returns (rho, fv)."""
+    xs = sorted({v for r in solid + hollow for v in r[:2]})
+    ys = sorted({v for r in solid + hollow for v in r[2:]})
+
+    def inside(x, y):
+        if not any(r[0] < x < r[1] and r[2] < y < r[3] for r in solid):
+            return False
+        return not any(r[0] < x < r[1] and r[2] < y < r[3] for r in hollow)
+
+    area = 0.0
+    for i in range(len(xs) - 1):
+        for j in range(len(ys) - 1):
+            if inside(0.5 * (xs[i] + xs[i + 1]), 0.5 * (ys[j] + ys[j + 1])):
+                area += (xs[i + 1] - xs[i]) * (ys[j + 1] - ys[j])
+    if area <= 0:
+        raise ValueError("empty metal region")
+    h = math.sqrt(2.0 * area / ntri)
+
+    def refine(v):
+        out = [v[0]]
+        for a, b in zip(v[:-1], v[1:]):
+            k = max(1, int(round((b - a) / h)))
+            out.extend(a + (b - a) * (q + 1) / k for q in range(k))
+            out[-1] = b
+        return np.asarray(out)
+
+    gx, gy = refine(xs), refine(ys)
+    nx, ny = len(gx), len(gy)
+    node = -np.ones((nx, ny), np.int64)
+    faces, rho = [], []
+
+    def nid(i, j):
+        if node[i, j] < 0:
+            node[i, j] = len(rho)
+            rho.append((gx[i], gy[j]))
+        return int(node[i, j])
+
+    for j in range(ny - 1):
+        for i in range(nx - 1):
+            if not inside(0.5 * (gx[i] + gx[i + 1]), 0.5 * (gy[j] + gy[j + 1])):
+                continue
+            p00, p10 = nid(i, j), nid(i + 1, j)
+            p01, p11 = nid(i, j + 1), nid(i + 1, j + 1)
+            if (i + j) % 2 == 0:
+                faces += [(p00, p10, p11), (p00, p11, p01)]
+            else:
+                faces += [(p00, p10, p01), (p10, p11, p01)]
+    return np.asarray(rho, float), np.asarray(faces, np.int64).T.copy()
+
+
+def _cross_rects(L1, L2):
+    a, b = 0.5 * L1, 0.5 * L2
+    return [(-a, a, -b, b), (-b, b, -a, a)]
+
+
+def _jcross_rects(L1, L2, A, B):
+    a, b, c = 0.5 * L1, 0.5 * L2, 0.5 * A
+    return _cross_rects(L1, L2) + [(a - B, a, -c, c), (-a, -a + B, -c, c),
+                                   (-c, c, a - B, a), (-c, c, -a, -a + B)]
+
+
+def _element_sheet(style, rho, fv, s1, s2, units, sheet_class, dx, dy, rot, Rsheet, fufp):
+    e1, e2, fe = _build_edges(fv)
+    sh = RWGSheet()
+    sh.style = style
+    sh.units = units
+    sh.rho = rho
+    sh.e1, sh.e2, sh.fv, sh.fe = e1, e2, fv, fe
+    sh.s1 = np.asarray(s1, float)
+    sh.s2 = np.asarray(s2, float)
+    sh.beta1, sh.beta2 = s1s2_to_beta(sh.s1, sh.s2)
+    sh.fr = np.full(sh.nface, float(Rsheet))
+    sh.sheet_class = sheet_class
+    sh.fufp = fufp
+    sh.xieta_check = False
+    _rotate(sh, rot)
+    if dx != 0 or dy != 0:
+        sh.rho = sh.rho + np.array([dx, dy], float)
+    return sh
+
+
+def loadedcross(*, s1, s2, L1, L2, w, ntri: int, units, orient=0.0, sheet_class="J", dx=0.0,
+                dy=0.0, rot=0.0, Rsheet=0.0, fufp=False) -> RWGSheet:
+    """Loaded cross (src/Elements.jl:794-903): a cross of tip-to-tip length L1 and
+    arm width L2 whose metal is the strip of width w along its outline (the cross
+    minus the cross of L1 - 2w, L2 - 2w); solid when w >= L2/2.  Centred at the
+    origin of the lattice cell.
+
+    The reference meshes the polygon with Triangle (src/Meshsub.jl:42-111), which
+    is absent here: this is our own structured mesher (_rect_union_mesh), so mesh
+    parity with the reference is unpinned.  orient rotates the element by a
+    multiple of 90 degrees (the shape is invariant under those)."""
+    for name, v in (("L1", L1), ("L2", L2), ("w", w), ("ntri", ntri)):
+        if not v > 0:
+            raise ValueError(f"{name} must be positive!")
+    if len(s1) != 2 or len(s2) != 2:
+        raise ValueError("s1 and s2 must be 2-vectors")
+    solid = _cross_rects(L1, L2)
+    hollow = [] if 2 * w >= L2 else _cross_rects(L1 - 2 * w, L2 - 2 * w)
+    rho, fv = _rect_union_mesh(solid, hollow, ntri)
+    if orient % 90 != 0:
+        raise ValueError("orient must be a multiple of 90 for the structured mesher")
+    return _element_sheet("loadedcross", rho, fv, s1, s2, units, sheet_class, dx, dy, rot,
+                          Rsheet, fufp)
+
+
+def jerusalemcross(*, P, L1, L2, A, B, w, ntri: int, units, sheet_class="J", dx=0.0, dy=0.0,
+                   rot=0.0, Rsheet=0.0, fufp=False) -> RWGSheet:
+    """Jerusalem cross (src/Elements.jl:963-1110) in a square cell of period P: arms
+    of width L2 spanning L1 between the outer edges of the end caps, caps of length
+    A and width B.  Where 2w < L2 (arms) or 2w < B (caps) the metal is the strip of
+    width w along the outline (the shape minus the one with every dimension less
+    2w), otherwise that part is solid.  Centred at the origin of the cell.
+
+    Our own structured mesher replaces the reference's Triangle mesh, so mesh
+    parity with the reference is unpinned."""
+    for name, v in (("P", P), ("L1", L1), ("L2", L2), ("A", A), ("B", B), ("w", w),
+                    ("ntri", ntri)):
+        if not v > 0:
+            raise ValueError(f"{name} must be positive!")
+    if L1 >= P or A >= P:
+        raise ValueError("the cross must fit inside the unit cell")
+    solid = _jcross_rects(L1, L2, A, B)
+    hollow = []
+    if 2 * w < L2 and 2 * w < B:
+        hollow = _jcross_rects(L1 - 2 * w, L2 - 2 * w, A - 2 * w, B - 2 * w)
+    elif 2 * w < L2:
+        hollow = _cross_rects(L1 - 2 * B, L2 - 2 * w)
+    elif 2 * w < B:
+        a, c = 0.5 * L1, 0.5 * A
+        hollow = [(a - B + w, a - w, -c + w, c - w), (-a + w, -a + B - w, -c + w, c - w),
+                  (-c + w, c - w, a - B + w, a - w), (-c + w, c - w, -a + w, -a + B - w)]
+    rho, fv = _rect_union_mesh(solid, hollow, ntri)
+    return _element_sheet("jerusalemcross", rho, fv, [float(P), 0.0], [0.0, float(P)], units,
+                          sheet_class, dx, dy, rot, Rsheet, fufp)

@@ -12,11 +12,17 @@ meander_polarizer: four PEC meanderlines (6 x 10 mm cell, 0.3 mm strips, meander
     heights drawn from a seeded U(2, 3) mm), eps_r = 1.07 foam spacers 5 mm thick,
     eps_r = 2.8 skins 0.25 mm thick; 101 frequencies 8..14 GHz at theta 0 and 40
     deg, phi = 0.
+cross_batch: independent single-sheet problems in free space, alternately a solid
+    Jerusalem cross and a loaded cross drawn from a seed: period U(8, 15) mm, arm
+    length (half the tip-to-tip span) U(0.3, 0.45) P, strip width U(0.03, 0.08) P,
+    800..4000 requested triangles; Jerusalem: arm and cap width = strip, cap
+    length = arm length; loaded cross: arm width 4 strips, metal strip = strip.
+    64 frequencies 1..30 GHz at theta 0, 15, 30, 45 deg, phi = 0.
 """
 import numpy as np
 
 from .layers import Layer
-from .sheets import rectstrip, annular_ring, meander
+from .sheets import rectstrip, annular_ring, meander, jerusalemcross, loadedcross
 
 
 def patch_free_space(n: int = 14, nfreq: int = 21, sheet_class: str = "J"):
@@ -58,6 +64,32 @@ def meander_polarizer(ntri: int = 8000, nfreq: int = 101, seed: int = 0, thetas=
     freqs = np.linspace(8.0, 14.0, nfreq)
     steering = {"phi": [0.0], "theta": list(thetas)}
     return strata, freqs, steering
+
+
+def cross_batch(nprob: int = 16, nfreq: int = 64, seed: int = 0, thetas=(0.0, 15.0, 30.0, 45.0),
+                ntri_range=(800, 4000)):
+    """List of (strata, freqs, steering, params) for the batched cross problems."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nprob):
+        P = float(rng.uniform(8.0, 15.0))
+        arm = float(rng.uniform(0.3, 0.45)) * P
+        strip = float(rng.uniform(0.03, 0.08)) * P
+        ntri = int(rng.integers(ntri_range[0], ntri_range[1] + 1))
+        if i % 2 == 0:
+            sh = jerusalemcross(P=P, L1=2 * arm, L2=strip, A=arm, B=strip, w=strip, ntri=ntri,
+                                units="mm")
+            par = dict(kind="jerusalemcross", P=P, L1=2 * arm, L2=strip, A=arm, B=strip, w=strip)
+        else:
+            sh = loadedcross(s1=[P, 0.0], s2=[0.0, P], L1=2 * arm, L2=4 * strip, w=strip,
+                             ntri=ntri, units="mm")
+            par = dict(kind="loadedcross", P=P, L1=2 * arm, L2=4 * strip, w=strip)
+        par["ntri"] = ntri
+        strata = [Layer.make(), sh, Layer.make()]
+        freqs = np.linspace(1.0, 30.0, nfreq)
+        steering = {"phi": [0.0], "theta": list(thetas)}
+        out.append((strata, freqs, steering, par))
+    return out
 
 
 def sheet_points(strata, freqs, steering):

@@ -305,6 +305,29 @@ def test_analyze_meander_polarizer(ctx, pk, orc, dbmin):
     assert dm <= 1e-6
 
 
+@pytest.mark.parametrize("which", [0, 1])
+def test_analyze_cross_batch(ctx, pk, orc, which):
+    """The batched cross problems: a solid Jerusalem cross (which 0) and a loaded
+    cross (which 1) from the seeded generator, meshed coarsely, at the
+    generator's four scan angles and two frequencies (one above the first
+    grating-lobe onset at 45 deg)."""
+    strata, freqs, steer, par = pk.workloads.cross_batch(nprob=2, nfreq=2, seed=3,
+                                                        ntri_range=(150, 150))[which]
+    freqs = [9.0, 21.0]
+    _say(f"analyze {par['kind']} P={par['P']:.2f} mm, {strata[1].nface} faces, "
+         f"2 freqs x theta 0/15/30/45")
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    assert len(res) == len(ref) == 8
+    d = _max_dS(res, ref)
+    print(f"  max |dS| fp64 = {d:.3e}")
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
 def _grating(kP, nterms=30):
     """Collin's strip-grating closed form as used by test/full_test.jl:27-42."""
     sum1 = 1.3862943611198906

@@ -186,3 +186,48 @@ def test_meander_mesh(pk, ntri, h):
     if ntri <= 800:
         d = pk.setup_rwg(sh)
         assert np.count_nonzero(d.eci > 0) > 0 and d.nbf > 0
+
+
+@pytest.mark.parametrize("kind", ["loaded", "loaded_solid", "jerusalem", "jerusalem_loaded"])
+def test_cross_meshes_area_and_conformity(pk, kind):
+    """Structured cross meshes: the triangle areas add up to the closed-form metal
+    area, every triangle is counter-clockwise, and every edge is shared by at most
+    two faces (conforming mesh)."""
+    S = pk.sheets
+    if kind == "loaded":
+        sh = S.loadedcross(s1=[10, 0], s2=[0, 10], L1=8, L2=2, w=0.5, ntri=600, units="mm")
+        area = (2 * 8 * 2 - 4) - (2 * 7 * 1 - 1)
+    elif kind == "loaded_solid":
+        sh = S.loadedcross(s1=[10, 0], s2=[0, 10], L1=8, L2=2, w=1.5, ntri=300, units="mm")
+        area = 2 * 8 * 2 - 4
+    elif kind == "jerusalem":
+        sh = S.jerusalemcross(P=12, L1=10, L2=0.6, A=5, B=0.6, w=0.6, ntri=800, units="mm")
+        area = (2 * 10 * 0.6 - 0.36) + 4 * 5 * 0.6 - 4 * 0.36
+    else:
+        sh = S.jerusalemcross(P=12, L1=10, L2=2, A=5, B=2, w=0.5, ntri=1500, units="mm")
+        area = ((2 * 10 * 2 - 4) + 4 * 10 - 16) - ((2 * 9 - 1) + 4 * 4 - 4)
+    p = sh.rho[sh.fv]
+    a = 0.5 * ((p[1, :, 0] - p[0, :, 0]) * (p[2, :, 1] - p[0, :, 1])
+               - (p[2, :, 0] - p[0, :, 0]) * (p[1, :, 1] - p[0, :, 1]))
+    assert (a > 0).all()
+    assert abs(a.sum() - area) < 1e-9 * area
+    cnt = np.bincount(sh.fe.reshape(-1), minlength=sh.nedge)
+    assert cnt.max() <= 2 and cnt.min() >= 1
+    rw = pk.rwg.setup_rwg(sh)
+    assert rw.nbf == int((cnt == 2).sum())
+
+
+def test_cross_batch_generator(pk):
+    """Seeded generator: reproducible, parameters inside the stated ranges, meshes
+    between about 800 and 4000 faces."""
+    a = pk.workloads.cross_batch()
+    b = pk.workloads.cross_batch()
+    assert len(a) == 16
+    for (sa, fa, st, pa), (sb, _, _, pb) in zip(a, b):
+        assert pa == pb and np.array_equal(sa[1].rho, sb[1].rho)
+        P = pa["P"]
+        assert 8 <= P <= 15
+        assert 0.6 * P <= pa["L1"] <= 0.9 * P
+        assert 700 <= sa[1].nface <= 4500
+        assert len(fa) == 64 and fa[0] == 1.0 and fa[-1] == 30.0
+        assert st["theta"] == [0.0, 15.0, 30.0, 45.0]
