@@ -208,13 +208,18 @@ def two_layer_patch(pk, hip, ctx):
     out["speedup_mixed_vs_fp64"] = round(out["points_per_s_mixed"] / out["points_per_s_fp64"], 3)
     out["max_dS_mixed_vs_fp64"] = max_dS(res["mixed"], res["fp64"])
     obj = [(2, 1, 1, 1), (2, 1, 2, 2)]
-    t0 = time.perf_counter()
-    grad, _ = A.gradients(strata, freqs, steer, vt, obj, ctx=ctx)
-    dt = time.perf_counter() - t0
-    out["adjoint_grads_per_s"] = round(grad.shape[0] * grad.shape[1] / dt, 3)
-    out["adjoint_s"] = round(dt, 3)
-    print(f"[bench] two-layer patch adjoint: {dt:.3f} s for {grad.shape[0]} points x "
-          f"{grad.shape[1]} objectives x {grad.shape[2]} parameters", flush=True)
+    grads = {}
+    for p in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        grads[p], _ = A.gradients(strata, freqs, steer, vt, obj, ctx=ctx, precision=p)
+        dt = time.perf_counter() - t0
+        g = grads[p]
+        out[f"adjoint_grads_per_s_{p}"] = round(g.shape[0] * g.shape[1] / dt, 3)
+        out[f"adjoint_s_{p}"] = round(dt, 3)
+        print(f"[bench] two-layer patch adjoint {p}: {dt:.3f} s for {g.shape[0]} points x "
+              f"{g.shape[1]} objectives x {g.shape[2]} parameters", flush=True)
+    d = np.abs(grads["mixed"] - grads["fp64"]) / np.abs(grads["fp64"]).max(axis=-1, keepdims=True)
+    out["adjoint_max_rel_mixed_vs_fp64"] = float(d.max())
     return out
 
 

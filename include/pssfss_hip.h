@@ -148,12 +148,14 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
  * mesh nodes move as rho(t) = rho + t_k vtan[k] (vtan: nparam blocks of nnode
  * (x, y) pairs, sheet units).  An objective f(S) enters through wbar = df/dS (holomorphic
  * derivative); Re(grad) is then the derivative of a real objective.  The forward
- * solve is repeated in FP64; adjoint systems use the same LU factors; the fill is
- * recomputed with derivatives and contracted, Z's derivative is never stored.
- * 'J' sheets without surface resistance.  grad: npts x nobj x nparam complex. */
+ * solve is repeated in `precision` (PSSFSS_FP64, or PSSFSS_MIXED: FP32 LU with FP64
+ * refinement); the adjoint systems Z^T lambda = g use the same factors (refined in
+ * FP64 against Z in the mixed form); the fill is recomputed with derivatives and
+ * contracted, Z's derivative is never stored.  'J' sheets without surface
+ * resistance.  grad: npts x nobj x nparam complex. */
 int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
                          const pssfss_point *pts, int nparam, const double *vtan, int nobj,
-                         const double *wbar, double *grad);
+                         const double *wbar, double *grad, int precision);
 
 /* ---- single stages, each a parity boundary ---------------------------------- */
 /* fillz (src/FillZY.jl:50-245) / filly (src/FillZY.jl:273-457): the nbf x nbf

@@ -426,91 +426,6 @@ __global__ void k_obs(int64_t nbf, const cd *ft, const int32_t *col_job, const d
     g[n + ld * col] = col_t[2 * col] * f[0] + col_t[2 * col + 1] * f[1];
 }
 
-// ------------------------------------------------- transposed LU solves
-// Z^T x = b with P Z = L U (row-major packed L\U, perm[i] = original row at i):
-// U^T y = b (forward, lanes = the 64 rows of a block, columns of U read as the
-// contiguous rows of U), L^T z = y (backward, unit), then x[perm[i]] = z[i].
-// One workgroup per system, nrhs <= 4 columns of a column-major B (in place).
-__global__ __launch_bounds__(1024) void k_trsv_t(int64_t n, int nrhs, const cd *LU0,
-                                                 int64_t sA, const int32_t *perm0, int64_t sP,
-                                                 cd *B0, int64_t sB, cd *T0) {
-    constexpr int NB = 64, NR = 4;
-    const cd *LU = LU0 + (int64_t)blockIdx.x * sA;
-    const int32_t *perm = perm0 + (int64_t)blockIdx.x * sP;
-    cd *B = B0 + (int64_t)blockIdx.x * sB;
-    cd *T = T0 + (int64_t)blockIdx.x * sB;
-    __shared__ cd D[NB][NB + 1];
-    __shared__ cd part[16][NR][NB];
-    __shared__ cd Y[NR][NB];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    const int64_t nblk = (n + NB - 1) / NB;
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool fwd = pass == 0;          // fwd: U^T (lower); bwd: L^T (unit upper)
-        for (int64_t q = 0; q < nblk; ++q) {
-            const int64_t bb = fwd ? q : nblk - 1 - q;
-            const int64_t b0 = bb * NB;
-            const int bs = (int)min((int64_t)NB, n - b0);
-            // D[r][l] = (U^T or L^T)[b0 + r][b0 + l] = LU[b0 + l][b0 + r]
-            for (int t = tid; t < bs * bs; t += blockDim.x) {
-                int r = t % bs, l = t / bs;
-                D[r][l] = LU[(b0 + l) * n + b0 + r];
-            }
-            // off-block sums: rows i = b0 + lane, over solved j (wave-strided)
-            const int64_t j0 = fwd ? 0 : b0 + bs, j1 = fwd ? b0 : n;
-            cd acc[NR];
-#pragma unroll
-            for (int c = 0; c < NR; ++c) acc[c] = cd{0, 0};
-            if (lane < bs) {
-                for (int64_t j = j0 + wave; j < j1; j += nw) {
-                    const cd a = LU[j * n + b0 + lane];
-#pragma unroll
-                    for (int c = 0; c < NR; ++c)
-                        if (c < nrhs) cfma(acc[c], a, B[j + n * c]);
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NR; ++c) part[wave][c][lane] = acc[c];
-            __syncthreads();
-            if (tid < NR * NB) {
-                const int c = tid / NB, r = tid % NB;
-                if (c < nrhs && r < bs) {
-                    cd s = {0, 0};
-                    for (int w = 0; w < nw; ++w) s += part[w][c][r];
-                    Y[c][r] = B[b0 + r + n * c] - s;
-                }
-            }
-            __syncthreads();
-            if (wave < nrhs) {
-                cd y = lane < bs ? Y[wave][lane] : cd{0, 0};
-                if (fwd) {
-                    for (int r = 0; r < bs; ++r) {
-                        if (lane == r) y = y / D[r][r];
-                        cd xr;
-                        xr.x = __shfl(y.x, r, 64);
-                        xr.y = __shfl(y.y, r, 64);
-                        if (lane > r && lane < bs) cfma(y, -D[lane][r], xr);
-                    }
-                } else {
-                    for (int r = bs - 1; r >= 0; --r) {
-                        cd xr;
-                        xr.x = __shfl(y.x, r, 64);
-                        xr.y = __shfl(y.y, r, 64);
-                        if (lane < r) cfma(y, -D[lane][r], xr);
-                    }
-                }
-                if (lane < bs) B[b0 + lane + n * wave] = y;
-            }
-            __syncthreads();
-        }
-    }
-    for (int64_t t = tid; t < n * nrhs; t += blockDim.x) T[t] = B[t];
-    __syncthreads();
-    for (int64_t t = tid; t < n * nrhs; t += blockDim.x) {
-        int64_t i = t % n, c = t / n;
-        B[perm[i] + n * c] = T[t];
-    }
-}
-
 }  // namespace
 
 // ===================================================================== launchers
@@ -557,15 +472,4 @@ void launch_obs(int64_t nbf, int ncol, const cd *ft, const int32_t *d_col_job,
     dim3 grid((unsigned)((nbf + 255) / 256), (unsigned)ncol);
     hipLaunchKernelGGL(k_obs, grid, dim3(256), 0, st, nbf, ft, d_col_job, d_col_t, g, ld);
     HIPCHECK(hipGetLastError());
-}
-
-void lu_solve_transposed(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
-                         const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
-                         hipStream_t st) {
-    for (int c0 = 0; c0 < nrhs; c0 += 4) {
-        const int nr = std::min(4, nrhs - c0);
-        hipLaunchKernelGGL(k_trsv_t, dim3((unsigned)batch), dim3(1024), 0, st, n, nr, LU, sA,
-                           perm, sP, B + (int64_t)c0 * n, sB, scratch + (int64_t)c0 * n);
-        HIPCHECK(hipGetLastError());
-    }
 }

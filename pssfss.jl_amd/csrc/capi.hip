@@ -587,12 +587,95 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     if (keep_tabs) *keep_tabs = std::move(tabs);
 }
 
-// Solve the np systems in ctx->wZ for the right-hand sides in ctx->wB (nbf x ncol
-// column-major per point), leaving X in ctx->wB.
 // Refinement sweeps before a system that has not converged is re-solved in FP64
 // (LAPACK's zcgesv allows 30; stagnation ends it earlier).
 constexpr int IR_MAXIT = 30;
 
+// Mixed-precision refinement of Z X = B (tr false) or Z^T X = B (tr true) for the
+// listed systems, from the FP32 factors in ctx->wZf / ctx->wperm and the FP64 Z in
+// ctx->wZ.  B (device, n x ncol per system) is read; X goes to ctx->wX.  The
+// first sweep solves with the FP32 factors from X = 0; each sweep then forms the
+// FP64 residual and stops with LAPACK zcgesv's rule (max_c ||r_c|| < sqrt(n)
+// ||x_c|| ||op(Z)|| eps), at IR_MAXIT sweeps, or when a sweep does not halve the
+// backward error.  Systems not listed keep whatever ctx->wX holds.
+struct Refined {
+    std::vector<int> iters, conv;
+    std::vector<double> best;      // backward error in units of sqrt(n) eps
+};
+
+Refined refine_mixed(pssfss_ctx *ctx, int64_t n, int ncol, int np, bool tr, const cd *B,
+                     const std::vector<int> &systems) {
+    hipStream_t st = ctx->st;
+    const int64_t nb2 = n * n, sB = n * ncol;
+    ctx->wX.ensure((size_t)sB * np);
+    ctx->wR.ensure((size_t)sB * np);
+    ctx->wRf.ensure((size_t)sB * np);
+    ctx->wTf.ensure((size_t)sB * np);
+    ctx->wnorm.ensure((size_t)4 * np * ncol + 8);
+    ctx->wsel.ensure((size_t)np);
+    ctx->wanorm.ensure((size_t)np + 2);
+    Refined out;
+    out.iters.assign(np, 0);
+    out.conv.assign(np, 0);
+    out.best.assign(np, 1e300);
+    if (systems.empty()) return out;
+    double *rn = ctx->wnorm.p, *xn = ctx->wnorm.p + np * ncol;
+    std::vector<double> hr(np * ncol), hx(np * ncol);
+    std::vector<int32_t> act(systems.begin(), systems.end());
+    auto upload_act = [&] {
+        HIPCHECK(hipMemcpyAsync(ctx->wsel.p, act.data(), act.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, st));
+    };
+    upload_act();
+    // ||op(Z)||_inf of each system for the stopping rule
+    if (tr) matrix_one_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
+    else matrix_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
+    std::vector<double> anorm(np + 1);
+    d2h(ctx, anorm.data(), ctx->wanorm.p, np);
+    launch_zero(ctx->wX.p, sB * np, st);
+    convert_z_to_f32(B, ctx->wRf.p, sB * np, st);
+    lu_solve_batched_f32(n, ncol, (int)act.size(), ctx->wZf.p, nb2, ctx->wperm.p, n, ctx->wRf.p,
+                         sB, ctx->wTf.p, st, ctx->wsel.p, tr);
+    add_correction_sel(ctx->wX.p, ctx->wRf.p, sB, ctx->wsel.p, (int)act.size(), st);
+    std::vector<double> prev(np, 1e300);
+    std::vector<int> done(np, 0);
+    const double eps = 2.220446049250313e-16, sqn = std::sqrt((double)n);
+    for (int it = 0; it < IR_MAXIT; ++it) {
+        residual_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wX.p, B, sB, ctx->wR.p, rn, xn, st,
+                         ctx->wsel.p, (int)act.size(), tr);
+        d2h(ctx, hr.data(), rn, np * ncol);
+        d2h(ctx, hx.data(), xn, np * ncol);
+        sync(ctx);
+        for (int p : act) {
+            // normwise backward error of the worst column, in units of sqrt(n) eps
+            double r = 0;
+            for (int c = 0; c < ncol; ++c) {
+                const double den = sqn * hx[p * ncol + c] * anorm[p] * eps;
+                r = std::max(r, den > 0 ? hr[p * ncol + c] / den : (hr[p * ncol + c] > 0 ? 1e300 : 0));
+            }
+            out.iters[p] = it + 1;
+            if (r < 1.0) out.conv[p] = done[p] = 1;
+            else if (r >= 0.5 * prev[p]) done[p] = 1;       // stagnated
+            out.best[p] = std::min(out.best[p], r);
+            prev[p] = r;
+        }
+        std::vector<int32_t> next;
+        for (int p : act)
+            if (!done[p]) next.push_back(p);
+        if (next.empty()) break;
+        act.swap(next);
+        upload_act();
+        // only the listed systems are solved and corrected
+        residual_to_f32(ctx->wR.p, ctx->wRf.p, sB * np, st);
+        lu_solve_batched_f32(n, ncol, (int)act.size(), ctx->wZf.p, nb2, ctx->wperm.p, n,
+                             ctx->wRf.p, sB, ctx->wTf.p, st, ctx->wsel.p, tr);
+        add_correction_sel(ctx->wX.p, ctx->wRf.p, sB, ctx->wsel.p, (int)act.size(), st);
+    }
+    return out;
+}
+
+// Solve the np systems in ctx->wZ for the right-hand sides in ctx->wB (nbf x ncol
+// column-major per point), leaving X in ctx->wB.
 void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
                  pssfss_point_info *infos) {
     hipStream_t st = ctx->st;
@@ -613,82 +696,22 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     }
     // FP32 LU with FP64 iterative refinement.
     ctx->wZf.ensure((size_t)nb2 * np);
-    ctx->wX.ensure((size_t)sB * np);
-    ctx->wR.ensure((size_t)sB * np);
-    ctx->wRf.ensure((size_t)sB * np);
-    ctx->wnorm.ensure((size_t)4 * np * ncol + 8);
     stage_begin(3, st);
     convert_z_to_f32(ctx->wZ.p, ctx->wZf.p, nb2 * np, st);
     stage_end(3, st);
     lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st);
     stage_begin(5, st);
-    ctx->wTf.ensure((size_t)sB * np);
-    ctx->wT.ensure((size_t)sB * np);
     lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
-    double *rn = ctx->wnorm.p, *xn = ctx->wnorm.p + np * ncol;
-    std::vector<double> hr(np * ncol), hx(np * ncol);
-    // systems still refining; the residuals and solves run on these only
-    std::vector<int32_t> act(np);
-    for (int p = 0; p < np; ++p) act[p] = p;
-    ctx->wsel.ensure((size_t)np);
-    HIPCHECK(hipMemcpyAsync(ctx->wsel.p, act.data(), np * sizeof(int32_t),
-                            hipMemcpyHostToDevice, st));
-    // R = B - Z X and the column maxima of R and X for the listed systems
-    auto residual = [&](const cd *X) {
-        const int na = (int)act.size();
-        residual_batched(n, ncol, np, ctx->wZ.p, nb2, X, ctx->wB.p, sB, ctx->wR.p, rn, xn, st,
-                         ctx->wsel.p, na);
-        d2h(ctx, hr.data(), rn, np * ncol);
-        d2h(ctx, hx.data(), xn, np * ncol);
-        sync(ctx);
-    };
-    // ||A||_inf of each system for the stopping rule of LAPACK's zcgesv:
-    // max_c ||r_c|| < sqrt(n) ||x_c|| ||A|| eps  (backward error at FP64 level)
-    ctx->wanorm.ensure((size_t)np + 2);
-    matrix_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
-    std::vector<double> anorm(np + 1);
-    d2h(ctx, anorm.data(), ctx->wanorm.p, np);
-    launch_zero(ctx->wX.p, sB * np, st);
-    convert_z_to_f32(ctx->wB.p, ctx->wRf.p, sB * np, st);
-    lu_solve_batched_f32(n, ncol, np, ctx->wZf.p, nb2, ctx->wperm.p, n, ctx->wRf.p, sB,
-                         ctx->wTf.p, st);
-    add_correction(ctx->wX.p, ctx->wRf.p, sB * np, st);
-    std::vector<double> prev(np, 1e300), best(np, 1e300);
-    std::vector<int> iters(np, 0), done(np, 0), conv(np, 0);
-    const double eps = 2.220446049250313e-16, sqn = std::sqrt((double)n);
-    for (int it = 0; it < IR_MAXIT; ++it) {
-        residual(ctx->wX.p);
-        for (int p : act) {
-            // normwise backward error of the worst column, in units of sqrt(n) eps
-            double r = 0;
-            for (int c = 0; c < ncol; ++c) {
-                const double den = sqn * hx[p * ncol + c] * anorm[p] * eps;
-                r = std::max(r, den > 0 ? hr[p * ncol + c] / den : (hr[p * ncol + c] > 0 ? 1e300 : 0));
-            }
-            iters[p] = it + 1;
-            if (r < 1.0) conv[p] = done[p] = 1;
-            else if (r >= 0.5 * prev[p]) done[p] = 1;       // stagnated: FP64 LU below
-            best[p] = std::min(best[p], r);
-            prev[p] = r;
-        }
-        std::vector<int32_t> next;
-        for (int p : act)
-            if (!done[p]) next.push_back(p);
-        if (next.empty()) break;
-        act.swap(next);
-        HIPCHECK(hipMemcpyAsync(ctx->wsel.p, act.data(), act.size() * sizeof(int32_t),
-                                hipMemcpyHostToDevice, st));
-        // only the listed systems are solved and corrected
-        residual_to_f32(ctx->wR.p, ctx->wRf.p, sB * np, st);
-        lu_solve_batched_f32(n, ncol, (int)act.size(), ctx->wZf.p, nb2, ctx->wperm.p, n,
-                             ctx->wRf.p, sB, ctx->wTf.p, st, ctx->wsel.p);
-        add_correction_sel(ctx->wX.p, ctx->wRf.p, sB, ctx->wsel.p, (int)act.size(), st);
-    }
+    std::vector<int> all(np);
+    for (int p = 0; p < np; ++p) all[p] = p;
+    Refined rf = refine_mixed(ctx, n, ncol, np, false, ctx->wB.p, all);
     stage_end(5, st);
-    // Points whose refinement did not reach an FP64-level backward error: FP64 LU.
+    // Points whose refinement did not reach an FP64-level backward error: FP64 LU
+    // (in place: Z of those points is replaced by its factors).
     std::vector<int> fb;
     for (int p = 0; p < np; ++p)
-        if (!conv[p]) fb.push_back(p);
+        if (!rf.conv[p]) fb.push_back(p);
+    ctx->wT.ensure((size_t)sB * np);
     for (int p : fb) {
         lu_factor_batched(n, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n, st);
         lu_make_perm(n, 1, ctx->wpiv.p + p * n, n, ctx->wperm.p + p * n, st);
@@ -702,8 +725,8 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     sync(ctx);
     if (infos)
         for (int p = 0; p < np; ++p) {
-            infos[p].refine_iters = iters[p];
-            infos[p].refine_resid = best[p];
+            infos[p].refine_iters = rf.iters[p];
+            infos[p].refine_resid = rf.best[p];
             infos[p].fallback_fp64 = 0;
         }
     if (infos)
@@ -865,7 +888,7 @@ void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfs
 // class terms' derivatives contracted on the device (dZ is never formed).
 void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int np,
                    int nparam, const double *vtan, int nobj, const double *wbar,
-                   double *grad) {
+                   double *grad, int precision) {
     hipStream_t st = ctx->st;
     const int n1 = pts[0].nmodes[0], n2 = pts[0].nmodes[1];
     for (int p = 1; p < np; ++p)
@@ -874,7 +897,7 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
     const int ncol = n1 + n2;
     const int64_t n = sh->nbf, nf = sh->nface;
     const double ul = pts[0].u / sh->upm;
-    // forward: fill (keeping tables), transforms, RHS, FP64 LU + solves
+    // forward: fill (keeping tables), transforms, RHS, LU + solves in `precision`
     ChunkResult cr;
     TableSet tabs;
     std::vector<FillParams> fp;
@@ -907,12 +930,35 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
     dct.upload(colt.data(), colt.size(), st);
     ctx->wB.ensure((size_t)n * ncol * np);
     launch_rhs(n, np * ncol, ctx->wft.p, dcj.p, dcs.p, ctx->wB.p, n, st);
-    solve_chunk(ctx, n, ncol, np, PSSFSS_FP64, nullptr);
-    // adjoint: observation vectors, transposed solves with the same factors
-    DBuf<cd> lam((size_t)n * ncol * np), tmp((size_t)n * ncol * np);
+    std::vector<pssfss_point_info> sinfo(np);
+    solve_chunk(ctx, n, ncol, np, precision, sinfo.data());
+    // adjoint: observation vectors, Z^T lambda = g with the same factors
+    const int64_t sB = n * ncol, nb2 = n * n;
+    DBuf<cd> lam((size_t)sB * np), tmp((size_t)sB * np);
     launch_obs(n, np * ncol, ctx->wft.p, dcj.p, dct.p, lam.p, n, st);
-    lu_solve_transposed(n, ncol, np, ctx->wZ.p, n * n, ctx->wperm.p, n, lam.p, n * ncol, tmp.p,
-                        st);
+    if (precision == PSSFSS_FP64) {
+        lu_solve_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wperm.p, n, lam.p, sB, tmp.p, st,
+                         nullptr, true);
+    } else {
+        // systems refined in mixed precision still hold Z (FP64) and FP32 factors:
+        // transposed refinement; systems that fell back hold FP64 factors in wZ
+        std::vector<int> ref, fac;
+        for (int p = 0; p < np; ++p) (sinfo[p].fallback_fp64 ? fac : ref).push_back(p);
+        Refined rt = refine_mixed(ctx, n, ncol, np, true, lam.p, ref);
+        for (int p : ref) {
+            if (rt.conv[p]) {
+                HIPCHECK(hipMemcpyAsync(lam.p + p * sB, ctx->wX.p + p * sB, sB * sizeof(cd),
+                                        hipMemcpyDeviceToDevice, st));
+            } else {
+                lu_factor_batched(n, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n, st);
+                lu_make_perm(n, 1, ctx->wpiv.p + p * n, n, ctx->wperm.p + p * n, st);
+                fac.push_back(p);
+            }
+        }
+        for (int p : fac)
+            lu_solve_batched(n, ncol, 1, ctx->wZ.p + p * nb2, nb2, ctx->wperm.p + p * n, n,
+                             lam.p + p * sB, sB, tmp.p + p * sB, st, nullptr, true);
+    }
     std::vector<cd> hX((size_t)n * ncol * np), hL((size_t)n * ncol * np);
     d2h(ctx, hX.data(), ctx->wB.p, hX.size());
     d2h(ctx, hL.data(), lam.p, hL.size());
@@ -1243,23 +1289,25 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
 
 int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
                          const pssfss_point *pts, int nparam, const double *vtan, int nobj,
-                         const double *wbar, double *grad) {
+                         const double *wbar, double *grad, int precision) {
     return guarded(ctx, [&] {
         if (!sheet || !pts || !vtan || !wbar || !grad || npts < 0 || nparam < 0)
             throw ArgError("null argument");
+        if (precision != PSSFSS_FP64 && precision != PSSFSS_MIXED)
+            throw ArgError("precision must be PSSFSS_FP64 or PSSFSS_MIXED");
         if (nobj < 1 || nobj > 4) throw ArgError("adjoint: 1..4 objectives");
         if (sheet->cls != 'J') throw ArgError("adjoint: electric-current ('J') sheets only");
         if (sheet->has_fr) throw ArgError("adjoint: resistive faces are not supported");
         HIPCHECK(hipSetDevice(ctx->device));
         for (int p = 0; p < npts; ++p) check_point(sheet, &pts[p], true);
         if (npts == 0 || nparam == 0) return;
-        int c = chunk_size(ctx, sheet, npts, PSSFSS_FP64);
+        int c = chunk_size(ctx, sheet, npts, precision);
         const int ncol = pts[0].nmodes[0] + pts[0].nmodes[1];
         for (int p0 = 0; p0 < npts; p0 += c) {
             int np = std::min(c, npts - p0);
             adjoint_chunk(ctx, sheet, pts + p0, np, nparam, vtan, nobj,
                           wbar + 2 * (size_t)p0 * nobj * ncol * ncol,
-                          grad + 2 * (size_t)p0 * nobj * nparam);
+                          grad + 2 * (size_t)p0 * nobj * nparam, precision);
         }
     });
 }

@@ -161,27 +161,33 @@ void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int6
                    int64_t sC, int batch, hipStream_t st);
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
                        hipStream_t st);
-// perm from lu_make_perm; scratch holds batch * sB entries.
+// perm from lu_make_perm; scratch holds batch * sB entries.  sel (optional): the
+// batch lists system indices, one per entry.  tr: solve Z^T x = b instead.
 void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
                       const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
-                      hipStream_t st);
+                      hipStream_t st, const int32_t *sel = nullptr, bool tr = false);
 void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t *perm,
                   hipStream_t st);
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv,
                            int64_t sP, hipStream_t st);
-// sel (optional): the batch lists system indices, one per entry.
+// sel (optional): the batch lists system indices, one per entry.  tr: Z^T x = b.
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
                           const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
-                          hipStream_t st, const int32_t *sel = nullptr);
+                          hipStream_t st, const int32_t *sel = nullptr, bool tr = false);
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st);
 // r = b - A x (FP64; a row-per-wavefront kernel for up to 8 right-hand sides, else
 // one transposed-A GEMM per system), batched; then the largest
 // |re| or |im| of each column of r and of x, at rnorm/xnorm[system * nrhs + rhs].
 // With sel, only the nsel listed systems are computed (norms of the others are 0).
+// tr: r = b - A^T x (one GEMM per system).
 void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
                       const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
-                      hipStream_t st, const int32_t *sel = nullptr, int nsel = 0);
+                      hipStream_t st, const int32_t *sel = nullptr, int nsel = 0,
+                      bool tr = false);
 void residual_to_f32(const cd *R, cf *Rf, int64_t count, hipStream_t st);
+// out[b] = one norm of system b (out holds 2 * ceil(batch / 2) doubles).
+void matrix_one_norms(int64_t n, int batch, const cd *A, int64_t sA, double *out,
+                      hipStream_t st);
 // out[b] = infinity norm of system b (out holds 2 * ceil(batch / 2) doubles).
 void matrix_inf_norms(int64_t n, int batch, const cd *A, int64_t sA, double *out,
                       hipStream_t st);
@@ -294,10 +300,6 @@ void launch_bfft_tan(const SheetDev &sh, const int32_t *bf_face, const int32_t *
                      hipStream_t st);
 void launch_obs(int64_t nbf, int ncol, const cd *ft, const int32_t *d_col_job,
                 const double *d_col_t, cd *g, int64_t ld, hipStream_t st);
-// Z^T X = B with the factors of lu_factor_batched (perm from lu_make_perm), in place.
-void lu_solve_transposed(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
-                         const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
-                         hipStream_t st);
 
 // ---- ufp.hip
 // Face-pair classes of a mesh (rho: 2*nnode, fv: 3*nface 1-based, device arrays).

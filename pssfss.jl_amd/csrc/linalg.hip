@@ -529,8 +529,26 @@ __global__ void k_copy_sys(int64_t cnt, const C *S0, C *D0, int64_t sB, const in
         D[t] = S[t];
 }
 
+// B[perm[i]] = T[i] (the inverse of k_perm_rows), for the transposed solves.
+template <class C>
+__global__ void k_unperm_rows(int64_t n, int nrhs, const int32_t *perm0, int64_t sP, const C *T0,
+                              C *B0, int64_t sB, const int32_t *sel) {
+    const int64_t sys = sel ? sel[blockIdx.y] : blockIdx.y;
+    const int32_t *perm = perm0 + sys * sP;
+    const C *T = T0 + sys * sB;
+    C *B = B0 + sys * sB;
+    const int64_t tot = n * nrhs;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t % n, c = t / n;
+        B[perm[i] + n * c] = T[t];
+    }
+}
+
 // One step of the blocked triangular solves L y = b (unit lower, UP false) and
-// U x = y (UP true) on a column-major X (n x nrhs per system), in place.  The
+// U x = y (UP true) on a column-major X (n x nrhs per system), in place.  With TR
+// the factors are used transposed, for Z^T x = b: U^T y = b (lower, UP false) and
+// L^T z = y (unit upper, UP true); their tiles are rows of U and L read as stored.  The
 // rows [s0, s0 + ss) are solved already; every other workgroup subtracts their
 // contribution from a 64-row tile (forward: rows below, backward: rows above),
 // and workgroup 0 owns the diagonal block [d0, d0 + ds), which it then solves
@@ -538,7 +556,7 @@ __global__ void k_copy_sys(int64_t cnt, const C *S0, C *D0, int64_t sB, const in
 // diagonal block), y = group of NRG right-hand sides, z = system (sel lists
 // them when given).  A pass is 2 + 2 ceil(n / 64) launches, each reading its
 // tiles of the factors once.
-template <class C, int NRG, bool UP>
+template <class C, int NRG, bool UP, bool TR>
 __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C *LU0, int64_t sA,
                                                    C *X0, int64_t sB, const int32_t *sel,
                                                    int64_t s0, int ss, int64_t d0, int ds) {
@@ -572,12 +590,17 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
         if (lane < rows && c < nc) y[s] = X[r0 + lane + n * c];
     }
     if (ss > 0) {
-        // factor tile rows [r0, r0 + rows) x columns [s0, s0 + ss), transposed
+        // factor tile rows [r0, r0 + rows) x columns [s0, s0 + ss) as Tt[column][row]
         for (int p = 0; p < 16; ++p) {
             const int rr = q + 4 * p;
             C v = zero;
-            if (rr < rows && lane < ss) v = LU[(r0 + rr) * n + s0 + lane];
-            Tt[lane][rr] = v;
+            if (TR) {
+                if (rr < ss && lane < rows) v = LU[(s0 + rr) * n + r0 + lane];
+                Tt[rr][lane] = v;
+            } else {
+                if (rr < rows && lane < ss) v = LU[(r0 + rr) * n + s0 + lane];
+                Tt[lane][rr] = v;
+            }
         }
         for (int t = tid; t < 64 * NRG; t += 256) {
             const int l = t % 64, c = t / 64;
@@ -606,13 +629,15 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
         }
         return;
     }
-    // the diagonal block, transposed: Tt[col][row]
+    // the diagonal block as Tt[col][row]; non-unit diagonal for U and U^T
+    constexpr bool DIV = UP != TR;
     __syncthreads();
     for (int p = 0; p < 16; ++p) {
         const int rr = q + 4 * p;
         C v = zero;
         if (rr < ds && lane < ds) v = LU[(d0 + rr) * n + d0 + lane];
-        Tt[lane][rr] = v;
+        if (TR) Tt[rr][lane] = v;
+        else Tt[lane][rr] = v;
     }
     __syncthreads();
     if (!UP) {
@@ -620,6 +645,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
             const C dl = Tt[r][lane];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
+                if (DIV && lane == r) y[s] = y[s] / dl;
                 const C xr = shfl_c(y[s], r);
                 if (lane > r) cfma(y[s], -dl, xr);
             }
@@ -629,7 +655,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
             const C dl = Tt[r][lane];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                if (lane == r) y[s] = y[s] / dl;
+                if (DIV && lane == r) y[s] = y[s] / dl;
                 const C xr = shfl_c(y[s], r);
                 if (lane < r) cfma(y[s], -dl, xr);
             }
@@ -730,6 +756,19 @@ __global__ __launch_bounds__(256) void k_rownorm(int64_t n, const cd *A0, int64_
     for (int64_t j = lane; j < n; j += 64) s += cabs(row[j]);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) atomicMax(out + b, (unsigned long long)__double_as_longlong(s));
+}
+
+// One norm (largest column sum of |a_ij|) of each row-major n x n system, one
+// thread per column (consecutive threads read consecutive columns of a row).
+__global__ __launch_bounds__(256) void k_colnorm(int64_t n, const cd *A0, int64_t sA,
+                                                 unsigned long long *out) {
+    const int64_t b = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const cd *A = A0 + b * sA;
+    double s = 0;
+    for (int64_t i = 0; i < n; ++i) s += cabs(A[i * n + j]);
+    atomicMax(out + b, (unsigned long long)__double_as_longlong(s));
 }
 
 __global__ void k_cd_to_cf(const cd *a, cf *b, int64_t n) {
@@ -939,17 +978,21 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
     HIPCHECK(hipStreamSynchronize(st));
 }
 
-// P B, then L y = b and U x = y: 2 + 2 ceil(n / 64) launches of k_trsv_step on the
-// permuted copy in scratch, which is copied back into B.
-template <class C>
+// Z x = b: P B, then L y = b and U x = y: 2 + 2 ceil(n / 64) launches of
+// k_trsv_step on the permuted copy in scratch, which is copied back into B.
+// TR (Z^T x = b): U^T y = b and L^T z = y on a copy, then x = P^T z into B.
+template <class C, bool TR = false>
 void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *perm,
                 int64_t sP, C *B, int64_t sB, C *scratch, hipStream_t st,
                 const int32_t *sel = nullptr) {
     if (batch <= 0 || nrhs <= 0) return;
     const int64_t cnt = n * nrhs;
     const dim3 gp((unsigned)std::min<int64_t>((cnt + 255) / 256, 1024), (unsigned)batch);
-    hipLaunchKernelGGL(k_perm_rows<C>, gp, dim3(256), 0, st, n, nrhs, perm, sP, B, scratch, sB,
-                       sel);
+    if (TR)
+        hipLaunchKernelGGL(k_copy_sys<C>, gp, dim3(256), 0, st, cnt, B, scratch, sB, sel);
+    else
+        hipLaunchKernelGGL(k_perm_rows<C>, gp, dim3(256), 0, st, n, nrhs, perm, sP, B, scratch,
+                           sB, sel);
     HIPCHECK(hipGetLastError());
     // right-hand sides per workgroup: each group of them reads the factors once
     const int nrg = nrhs <= 4 ? 4 : (nrhs <= 8 ? 8 : 16);
@@ -957,7 +1000,7 @@ void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const i
     const int64_t nblk = (n + 63) / 64;
     auto step = [&](bool up, int64_t s0, int ss, int64_t d0, int ds, int64_t ntile) {
         const dim3 g((unsigned)(1 + ntile), gy, (unsigned)batch);
-#define TS(NRG, UP) hipLaunchKernelGGL((k_trsv_step<C, NRG, UP>), g, dim3(256), 0, st, n, nrhs, LU, sA, scratch, sB, sel, s0, ss, d0, ds)
+#define TS(NRG, UP) hipLaunchKernelGGL((k_trsv_step<C, NRG, UP, TR>), g, dim3(256), 0, st, n, nrhs, LU, sA, scratch, sB, sel, s0, ss, d0, ds)
         if (nrg == 4) { if (up) TS(4, true); else TS(4, false); }
         else if (nrg == 8) { if (up) TS(8, true); else TS(8, false); }
         else { if (up) TS(16, true); else TS(16, false); }
@@ -981,7 +1024,11 @@ void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const i
         const int64_t d0 = 64 * (b - 1);
         step(true, s0, (int)std::min<int64_t>(64, n - s0), d0, 64, d0 / 64);
     }
-    hipLaunchKernelGGL(k_copy_sys<C>, gp, dim3(256), 0, st, cnt, scratch, B, sB, sel);
+    if (TR)
+        hipLaunchKernelGGL(k_unperm_rows<C>, gp, dim3(256), 0, st, n, nrhs, perm, sP, scratch, B,
+                           sB, sel);
+    else
+        hipLaunchKernelGGL(k_copy_sys<C>, gp, dim3(256), 0, st, cnt, scratch, B, sB, sel);
     HIPCHECK(hipGetLastError());
 }
 
@@ -1006,8 +1053,9 @@ void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, i
 
 void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
                       const int32_t *perm, int64_t sP, cd *B, int64_t sB, cd *scratch,
-                      hipStream_t st) {
-    lu_solve_t<cd>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st);
+                      hipStream_t st, const int32_t *sel, bool tr) {
+    if (tr) lu_solve_t<cd, true>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
+    else lu_solve_t<cd, false>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
 }
 
 void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t *perm,
@@ -1025,8 +1073,9 @@ void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipi
 
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
                           const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
-                          hipStream_t st, const int32_t *sel) {
-    lu_solve_t<cf>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
+                          hipStream_t st, const int32_t *sel, bool tr) {
+    if (tr) lu_solve_t<cf, true>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
+    else lu_solve_t<cf, false>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
 }
 
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {
@@ -1036,11 +1085,26 @@ void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {
 
 void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, const cd *X,
                       const cd *Bv, int64_t sB, cd *R, double *rnorm, double *xnorm,
-                      hipStream_t st, const int32_t *sel, int nsel) {
+                      hipStream_t st, const int32_t *sel, int nsel, bool tr) {
     launch_zero((cd *)rnorm, (batch * nrhs + 1) / 2, st);
     launch_zero((cd *)xnorm, (batch * nrhs + 1) / 2, st);
     const int ns = sel ? nsel : batch;
     if (ns <= 0 || nrhs <= 0) return;
+    if (tr) {
+        // R = B - Z^T X: the row-major Z read as a column-major matrix is Z^T
+        const int64_t cnt = n * nrhs;
+        const dim3 gp((unsigned)std::min<int64_t>((cnt + 255) / 256, 1024), (unsigned)ns);
+        hipLaunchKernelGGL(k_copy_sys<cd>, gp, dim3(256), 0, st, cnt, Bv, R, sB, sel);
+        HIPCHECK(hipGetLastError());
+        gemm_t<cd, false>((int)n, nrhs, (int)n, cd{-1.0, 0.0}, A, n, sA, X, n, sB, cd{1.0, 0.0},
+                          R, n, sB, ns, st, sel);
+        const dim3 gm((unsigned)std::min<int64_t>((n + 255) / 256, 16), (unsigned)nrhs,
+                      (unsigned)ns);
+        hipLaunchKernelGGL(k_colmax, gm, dim3(256), 0, st, n, nrhs, R, X, sB, sel,
+                           (unsigned long long *)rnorm, (unsigned long long *)xnorm);
+        HIPCHECK(hipGetLastError());
+        return;
+    }
     if (nrhs <= 8) {
         dim3 g((unsigned)((n + 3) / 4), (unsigned)ns);
         if (nrhs <= 4)
@@ -1080,6 +1144,14 @@ void add_correction_sel(cd *X, const cf *D, int64_t sB, const int32_t *sel, int 
     int64_t bx = (sB + 255) / 256;
     dim3 g((unsigned)(bx > 64 ? 64 : bx), (unsigned)nsel);
     hipLaunchKernelGGL(k_add_corr_sel, g, dim3(256), 0, st, X, D, sB, sel);
+    HIPCHECK(hipGetLastError());
+}
+
+void matrix_one_norms(int64_t n, int batch, const cd *A, int64_t sA, double *out,
+                      hipStream_t st) {
+    launch_zero((cd *)out, (batch + 1) / 2, st);
+    dim3 g((unsigned)((n + 255) / 256), (unsigned)batch);
+    hipLaunchKernelGGL(k_colnorm, g, dim3(256), 0, st, n, A, sA, (unsigned long long *)out);
     HIPCHECK(hipGetLastError());
 }
 

@@ -111,9 +111,11 @@ def _blocks(g: G.GSM):
 
 # ------------------------------------------------------------------ driver
 def gradients(strata, freqs, steering, vtan, objectives: Sequence[Tuple[int, int, int, int]],
-              ctx=None):
+              ctx=None, precision: str = "fp64"):
     """Derivatives of f_o = |S_ij(m, n)|^2 of the total GSM (1-based i, j, m, n) with
     respect to the parameters whose node directions are vtan (nparam, nnode, 2).
+    precision "fp64", or "mixed" (FP32 LU with FP64 refinement for the forward and
+    the adjoint solves; the fill and its derivatives stay FP64).
 
     Returns (grad, values): grad (npoints, nobj, nparam), values (npoints, nobj) of f_o,
     for the points of the sweep in analyze order (cut-off points skipped)."""
@@ -147,7 +149,7 @@ def gradients(strata, freqs, steering, vtan, objectives: Sequence[Tuple[int, int
         points.append(pt)
     if not points:
         return np.zeros((0, len(objectives), nparam)), np.zeros((0, len(objectives)))
-    sheet_gsms, _ = hip.sheet_gsm_batch(ctx, ds, builders, "fp64")
+    sheet_gsms, _ = hip.sheet_gsm_batch(ctx, ds, builders, precision)
     nobj = len(objectives)
     wsets, values = [], []
     for pt, gs in zip(points, sheet_gsms):
@@ -175,6 +177,7 @@ def gradients(strata, freqs, steering, vtan, objectives: Sequence[Tuple[int, int
     pts = (hip.Point * npts)(*[b.point for b in builders])
     L = hip.lib()
     ctx.check(L.pssfss_sheet_adjoint(ctx.h, ds.h, npts, pts, nparam, hip._d(vt), nobj,
-                                     hip._d(wbar), hip._d(grad.view(np.float64))))
+                                     hip._d(wbar), hip._d(grad.view(np.float64)),
+                                     hip.PRECISION[precision]))
     ds.free()
     return grad.real, np.array(values)

@@ -91,7 +91,7 @@ def lib():
         L.pssfss_sheet_gsm_batch.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.POINTER(Gsm),
                                              C.c_int, C.POINTER(PointInfo)]
         L.pssfss_sheet_adjoint.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.c_int, _dp,
-                                           C.c_int, _dp, _dp]
+                                           C.c_int, _dp, _dp, C.c_int]
         L.pssfss_face_pair_classes.argtypes = [vp, C.c_int64, _dp, C.c_int64, _ip,
                                                C.POINTER(C.c_int64), _ip, _ip, _ip]
         L.pssfss_fill.argtypes = [vp, vp, C.POINTER(Point), _dp, C.POINTER(PointInfo)]
