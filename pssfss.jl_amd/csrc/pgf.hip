@@ -628,45 +628,67 @@ __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, dou
     }
 }
 
-// MFMA form of the many-phasing spatial sums.  The lattice-term tile (64 rows x 8
-// points) is the A operand and the phase tile (8 points x 32 phasings, cos and
-// sin columns) the B operand of 16x16x4 MFMAs (FP32 for T = float, FP64 for
-// T = double).  Wavefront w owns rows 16w..16w+15 and each lane owns 4 rows of
-// one phasing column in each of two 16-phasing groups, so it folds the ring sums
-// of its 8 (row, phasing) pairs of J and K into FP64 running sums and applies the
-// break test of src/PGF.jl:86-103 at every ring end.  Tiles are double-buffered
-// in LDS and the next phase tile is prefetched into registers, one barrier per
-// 8-point chunk.
-template <class T>
-__global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
-                                                 double us1x, double us1y, double us2x,
-                                                 double us2y, const T *phase, int ldp,
-                                                 int nf, cd *jk, int64_t jk_stride,
-                                                 unsigned long long *bcnt) {
-#pragma clang fp contract(off)
-    typedef MfmaT<T> MT;
-    typedef typename MT::V V;
-    constexpr int FT = 32;
-    // the ring tiles and the final per-row sums share one LDS region
+// Lattice points of rings 0..JK_RINGMAX in ring order: the offset u (m s1 + n s2)
+// in FP32 and (m, n), for the FP32 spatial sums.
+// Phasing halves (32 phasings each) per workgroup of the MFMA spatial sums: one.
+// Two (512 threads, 64 phasings, the term tile shared by twice the MFMA work) needs
+// about 195 VGPRs, runs two waves per SIMD and measured slower on MI355X.
+constexpr int JK_MFMA_WH = 1;
+
+struct LatPt {
+    float ox, oy;
+    int m, n;
+};
+
+__global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, LatPt *lat) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
+    if (e >= E) return;
+    const int r = ring_of(e);
+    int m, n;
+    ring_mn(r, (int)(e - ring_base(r)), m, n);
+    LatPt L;
+    L.ox = (float)((double)m * us1x + (double)n * us2x);
+    L.oy = (float)((double)m * us1y + (double)n * us2y);
+    L.m = m;
+    L.n = n;
+    lat[e] = L;
+}
+
+// MFMA form of the many-phasing spatial sums (mixed precision).  A workgroup of
+// 4 WH wavefronts owns 64 rows (9 classes x 7 quadrature points) and 32 WH
+// phasings.  Per chunk of 8 lattice points the threads form the FP32 lattice term
+// pairs (J: e^-a / a, K: e^-a) of the 64 x 8 tile and the phases of the 8 x 32 WH
+// tile; the term tile is the A operand and the phase tile (cos and sin columns)
+// the B operand of 16x16x4 FP32 MFMAs.  Wavefront (wr, wh) accumulates
+// rows 16 wr..16 wr + 15 against phasings 32 wh..32 wh + 31; each lane folds the
+// ring sums of its 8 (row, phasing) pairs into FP64 running sums at every ring
+// end and applies the break test of src/PGF.jl:86-103.  Lattice offsets come from
+// an FP32 table (k_lattice); phases of rings <= JK_RT are products of per-phasing
+// power tables in LDS.  Tiles are double-buffered, one barrier per chunk; their
+// rows are padded so the operand reads are free of bank conflicts.
+template <int WH>
+__global__ __launch_bounds__(256 * WH) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
+                                                      const LatPt *lat, const float *phase,
+                                                      int ldp, int nf, cd *jk, int64_t jk_stride,
+                                                      unsigned long long *bcnt) {
+    typedef MfmaT<float> MT;
+    typedef MT::V V;
+    constexpr int NT = 256 * WH, FT = 32 * WH, AP = 80, BP = 16 + FT;
     struct Tiles {
-        T Aj[2][8][64], Ak[2][8][64];
-        T Bre[2][8][FT], Bim[2][8][FT];
+        float Aj[2][8][AP], Ak[2][8][AP];
+        float Bre[2][8][BP], Bim[2][8][BP];
     };
-    constexpr size_t LBYTES = sizeof(Tiles) > sizeof(cd) * 64 * FT ? sizeof(Tiles)
-                                                                  : sizeof(cd) * 64 * FT;
+    constexpr size_t CSB = sizeof(cd) * 64 * 32;
+    constexpr size_t LBYTES = sizeof(Tiles) > CSB ? sizeof(Tiles) : CSB;
     __shared__ __align__(16) char lraw[LBYTES];
     __shared__ double urxs[64], urys[64];
-    __shared__ double offs[2][8][2];      // lattice offsets m s1 + n s2 of the chunk points
-    __shared__ int pmn[2][8][2];          // (m, n) of the chunk points
     constexpr int PW = 2 * JK_RT + 1;
-    __shared__ T P1[2][FT][PW], P2[2][FT][PW];   // [cos/sin][phasing][m + JK_RT]
+    __shared__ float P1[2][FT][PW], P2[2][FT][PW];   // [cos/sin][phasing][m + JK_RT]
     Tiles &TL = *reinterpret_cast<Tiles *>(lraw);
-    auto &Aj = TL.Aj;
-    auto &Ak = TL.Ak;
-    auto &Bre = TL.Bre;
-    auto &Bim = TL.Bim;
-    cd(*Cs)[FT] = reinterpret_cast<cd(*)[FT]>(lraw);
+    cd(*Cs)[32] = reinterpret_cast<cd(*)[32]>(lraw);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 3, wh = wave >> 2;
     const int64_t u0 = (int64_t)blockIdx.x * 9;
     const int f0 = blockIdx.y * FT;
     if (tid < 64) {
@@ -692,45 +714,39 @@ __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, doub
         urxs[r] = urx;
         urys[r] = ury;
     }
+    // exp(-i m psi1) and exp(-i n psi2) for |m|, |n| <= JK_RT, from the phase table's
+    // (m, 0) and (0, n) entries; phases of rings <= JK_RT are their products
+    for (int t = tid; t < 2 * FT * PW; t += NT) {
+        const int which = t / (FT * PW), f = (t / PW) % FT, j = t % PW - JK_RT;
+        const int64_t e = which == 0 ? ring_index(j, 0) : ring_index(0, j);
+        const float *pp = phase + e * 2 * (int64_t)ldp + 2 * (f0 + f);
+        if (which == 0) { P1[0][f][j + JK_RT] = pp[0]; P1[1][f][j + JK_RT] = pp[1]; }
+        else { P2[0][f][j + JK_RT] = pp[0]; P2[1][f][j + JK_RT] = pp[1]; }
+    }
     __syncthreads();
+    // this thread's term row (its lane) relative to the lattice offsets; its
+    // phase slot (point pkk, phasing pf)
+    const float urx = (float)urxs[lane], ury = (float)urys[lane];
+    const int pkk = tid / FT, pf = tid % FT;
     const int pc = lane & 15;
     cd js[2][4], ks[2][4];
     unsigned done = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const int r = 16 * wave + MT::row(lane, e);
+        const int r = 16 * wr + MT::row(lane, e);
         const bool live = r < 63 && u0 + r / 7 < nufp;
-        double arg = sqrt(urxs[r] * urxs[r] + urys[r] * urys[r]);
-        double rterm = exp(-arg);
-        double j0 = arg <= 1e-5 ? (-arg / 6 + 0.5) * arg - 1 : (rterm - 1) / arg;
+        const double arg = sqrt(urxs[r] * urxs[r] + urys[r] * urys[r]);
+        const double rterm = exp(-arg);
+        const double j0 = arg <= 1e-5 ? (-arg / 6 + 0.5) * arg - 1 : (rterm - 1) / arg;
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
             js[g][e] = cd{j0, 0.0};
             ks[g][e] = cd{rterm, 0.0};
-            if (!live || f0 + 16 * g + pc >= nf) done |= 1u << (4 * g + e);
+            if (!live || f0 + 32 * wh + 16 * g + pc >= nf) done |= 1u << (4 * g + e);
         }
     }
-    // this thread's slot of the phase tile
-    const int pkk = tid / FT, pf = tid % FT;
-    const T *pcol = phase + 2 * (f0 + pf);
+    const float *pcol = phase + 2 * (f0 + pf);
     const int64_t pstride = 2 * (int64_t)ldp;
-    // exp(-i m psi1) and exp(-i n psi2) for |m|, |n| <= JK_RT, from the phase table's
-    // (m, 0) and (0, n) entries; phases of rings <= JK_RT are their products
-    for (int t = tid; t < 2 * FT * PW; t += 256) {
-        const int which = t / (FT * PW), f = (t / PW) % FT, j = t % PW - JK_RT;
-        const int64_t e = which == 0 ? ring_index(j, 0) : ring_index(0, j);
-        const T *pp = phase + e * pstride + 2 * (f0 + f);
-        if (which == 0) { P1[0][f][j + JK_RT] = pp[0]; P1[1][f][j + JK_RT] = pp[1]; }
-        else { P2[0][f][j + JK_RT] = pp[0]; P2[1][f][j + JK_RT] = pp[1]; }
-    }
-    if (pf == 0) {                        // ring 1's first chunk
-        int m, n;
-        ring_mn(1, pkk, m, n);
-        offs[0][pkk][0] = (double)m * us1x + (double)n * us2x;
-        offs[0][pkk][1] = (double)m * us1y + (double)n * us2y;
-        pmn[0][pkk][0] = m;
-        pmn[0][pkk][1] = n;
-    }
     int buf = 0;
     const unsigned live0 = ~done & 0xffu;
     unsigned long long npts = 0;     // lattice points of the reference's sums
@@ -738,7 +754,7 @@ __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, doub
     for (int r = 1; r <= JK_RINGMAX; ++r) {
         if (!__syncthreads_or(done != 0xffu)) break;
         rlast = r;
-        V aJ[2][2], aK[2][2];          // [g][cos/sin]
+        V aJ[2][2], aK[2][2];        // [g][cos/sin]: this ring's sums
 #pragma unroll
         for (int g = 0; g < 2; ++g)
 #pragma unroll
@@ -746,66 +762,42 @@ __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, doub
                 aJ[g][c] = V{0, 0, 0, 0};
                 aK[g][c] = V{0, 0, 0, 0};
             }
+        const int64_t rb = ring_base(r);
         for (int c = 0; c < r; ++c) {
-            // terms of chunk c (offsets staged one chunk ahead) into buffer buf
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int t = tid + 256 * h;
-                const int row = t & 63, kk = t >> 6;
-                double dx = urxs[row] - offs[buf][kk][0];
-                double dy = urys[row] - offs[buf][kk][1];
-                T a, e, ea;
-                if constexpr (sizeof(T) == 4) {
-                    float fx = (float)dx, fy = (float)dy;
-                    a = __builtin_amdgcn_sqrtf(fx * fx + fy * fy);
-                    e = __expf(-a);
-                    ea = __fdividef(e, a);
-                } else {
-                    a = sqrt(dx * dx + dy * dy);
-                    e = exp(-a);
-                    ea = e * (1.0 / a);
-                }
-                Aj[buf][kk][row] = ea;
-                Ak[buf][kk][row] = e;
+            for (int h = 0; h < 2 / WH; ++h) {
+                const int kk = wave + 4 * WH * h;
+                const LatPt L = lat[rb + 8 * c + kk];
+                const float dx = urx - L.ox, dy = ury - L.oy;
+                const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+                const float ex = __expf(-a);
+                TL.Aj[buf][kk][lane] = __fdividef(ex, a);
+                TL.Ak[buf][kk][lane] = ex;
             }
-            {
-                T br, bi;
-                if (r <= JK_RT) {
-                    const int m = pmn[buf][pkk][0] + JK_RT, n = pmn[buf][pkk][1] + JK_RT;
-                    const T c1 = P1[0][pf][m], s1 = P1[1][pf][m];
-                    const T c2 = P2[0][pf][n], s2 = P2[1][pf][n];
-                    br = c1 * c2 - s1 * s2;
-                    bi = c1 * s2 + s1 * c2;
-                } else {
-                    const int64_t e = ring_base(r) + 8 * c + pkk;
-                    br = pcol[e * pstride];
-                    bi = pcol[e * pstride + 1];
-                }
-                Bre[buf][pkk][pf] = br;
-                Bim[buf][pkk][pf] = bi;
+            float br, bi;
+            if (r <= JK_RT) {
+                const LatPt L = lat[rb + 8 * c + pkk];
+                const float c1 = P1[0][pf][L.m + JK_RT], s1 = P1[1][pf][L.m + JK_RT];
+                const float c2 = P2[0][pf][L.n + JK_RT], s2 = P2[1][pf][L.n + JK_RT];
+                br = c1 * c2 - s1 * s2;
+                bi = c1 * s2 + s1 * c2;
+            } else {
+                const int64_t e = rb + 8 * c + pkk;
+                br = pcol[e * pstride];
+                bi = pcol[e * pstride + 1];
             }
-            // offsets of the next chunk (the next ring's first chunk at a ring end)
-            if (pf == 0) {
-                const bool same = c + 1 < r;
-                const int nr_ = same ? r : (r < JK_RINGMAX ? r + 1 : r);
-                const int nidx = same ? 8 * (c + 1) : 0;
-                int m, n;
-                ring_mn(nr_, nidx + pkk, m, n);
-                offs[buf ^ 1][pkk][0] = (double)m * us1x + (double)n * us2x;
-                offs[buf ^ 1][pkk][1] = (double)m * us1y + (double)n * us2y;
-                pmn[buf ^ 1][pkk][0] = m;
-                pmn[buf ^ 1][pkk][1] = n;
-            }
+            TL.Bre[buf][pkk][pf] = br;
+            TL.Bim[buf][pkk][pf] = bi;
             __syncthreads();
 #pragma unroll
             for (int k4 = 0; k4 < 2; ++k4) {
                 const int kr = 4 * k4 + (lane >> 4);
-                const T a_j = Aj[buf][kr][16 * wave + (lane & 15)];
-                const T a_k = Ak[buf][kr][16 * wave + (lane & 15)];
+                const float a_j = TL.Aj[buf][kr][16 * wr + pc];
+                const float a_k = TL.Ak[buf][kr][16 * wr + pc];
 #pragma unroll
                 for (int g = 0; g < 2; ++g) {
-                    const T bre = Bre[buf][kr][16 * g + (lane & 15)];
-                    const T bim = Bim[buf][kr][16 * g + (lane & 15)];
+                    const float bre = TL.Bre[buf][kr][32 * wh + 16 * g + pc];
+                    const float bim = TL.Bim[buf][kr][32 * wh + 16 * g + pc];
                     aJ[g][0] = MT::mma(a_j, bre, aJ[g][0]);
                     aJ[g][1] = MT::mma(a_j, bim, aJ[g][1]);
                     aK[g][0] = MT::mma(a_k, bre, aK[g][0]);
@@ -820,24 +812,17 @@ __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, doub
             for (int e = 0; e < 4; ++e) {
                 const unsigned bit = 1u << (4 * g + e);
                 if (!(done & bit)) {
-                    cd a = cd{(double)aJ[g][0][e], (double)aJ[g][1][e]};
-                    cd b = cd{(double)aK[g][0][e], (double)aK[g][1][e]};
+                    const cd a = cd{(double)aJ[g][0][e], (double)aJ[g][1][e]};
+                    const cd b = cd{(double)aK[g][0][e], (double)aK[g][1][e]};
                     js[g][e] += a;
                     ks[g][e] += b;
                     if (r > 5) {
-                        bool conv;
-                        if constexpr (sizeof(T) == 4) {
-                            // |ring/sum| < 1e-8 for J and K, as squared magnitudes
-                            const double aj = a.x * a.x + a.y * a.y;
-                            const double sj = js[g][e].x * js[g][e].x + js[g][e].y * js[g][e].y;
-                            const double ak = b.x * b.x + b.y * b.y;
-                            const double sk = ks[g][e].x * ks[g][e].x + ks[g][e].y * ks[g][e].y;
-                            conv = aj < 1e-16 * sj && ak < 1e-16 * sk;
-                        } else {
-                            double c1 = cabs(a / js[g][e]), c2 = cabs(b / ks[g][e]);
-                            conv = (c1 > c2 ? c1 : c2) < 1e-8;
-                        }
-                        if (conv) {
+                        // |ring/sum| < 1e-8 for J and K, as squared magnitudes
+                        const double aj = a.x * a.x + a.y * a.y;
+                        const double sj = js[g][e].x * js[g][e].x + js[g][e].y * js[g][e].y;
+                        const double ak = b.x * b.x + b.y * b.y;
+                        const double sk = ks[g][e].x * ks[g][e].x + ks[g][e].y * ks[g][e].y;
+                        if (aj < 1e-16 * sj && ak < 1e-16 * sk) {
                             done |= bit;
                             npts += lattice_points(r);
                         }
@@ -849,36 +834,40 @@ __global__ __launch_bounds__(256) void k_jk_mfma(SheetDev sh, int64_t nufp, doub
         npts += (unsigned long long)__popc(live0 & ~done) * lattice_points(rlast);
         block_count(npts, bcnt);
     }
-    // J then K: per-row sums through LDS, combined over the 7 quadrature points
-    // (src/Zint.jl:117-124)
-    for (int which = 0; which < 2; ++which) {
-        __syncthreads();
+    // J then K, each phasing half in turn: per-row sums through LDS, combined over
+    // the 7 quadrature points (src/Zint.jl:117-124)
+    for (int which = 0; which < 2; ++which)
+        for (int hh = 0; hh < WH; ++hh) {
+            __syncthreads();
+            if (wh == hh) {
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+                for (int g = 0; g < 2; ++g)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = 16 * wave + MT::row(lane, e);
-                Cs[r][16 * g + pc] = which == 0 ? js[g][e] : ks[g][e];
+                    for (int e = 0; e < 4; ++e) {
+                        const int r = 16 * wr + MT::row(lane, e);
+                        Cs[r][16 * g + pc] = which == 0 ? js[g][e] : ks[g][e];
+                    }
             }
-        __syncthreads();
-        for (int t = tid; t < 9 * FT; t += 256) {
-            const int u = t / FT, f = t % FT;
-            const int64_t iu = u0 + u;
-            if (iu >= nufp || f0 + f >= nf) continue;
-            cd S = {0, 0}, Sx = {0, 0}, Se = {0, 0};
+            __syncthreads();
+            for (int t = tid; t < 9 * 32; t += NT) {
+                const int u = t / 32, f = t % 32;
+                const int64_t iu = u0 + u;
+                const int fg = f0 + 32 * hh + f;
+                if (iu >= nufp || fg >= nf) continue;
+                cd S = {0, 0}, Sx = {0, 0}, Se = {0, 0};
 #pragma unroll
-            for (int q = 0; q < 7; ++q) {
-                cd w = Cs[7 * u + q][f] * QW[q];
-                Sx += w * QXI[q];
-                Se += w * QETA[q];
-                S += w;
+                for (int q = 0; q < 7; ++q) {
+                    const cd w = Cs[7 * u + q][f] * QW[q];
+                    Sx += w * QXI[q];
+                    Se += w * QETA[q];
+                    S += w;
+                }
+                cd *o = jk + (int64_t)fg * jk_stride + (int64_t)(3 * which) * nufp;
+                o[iu] = S;
+                o[nufp + iu] = Sx;
+                o[2 * nufp + iu] = Se;
             }
-            cd *o = jk + (int64_t)(f0 + f) * jk_stride + (int64_t)(3 * which) * nufp;
-            o[iu] = S;
-            o[nufp + iu] = Sx;
-            o[2 * nufp + iu] = Se;
         }
-    }
 }
 
 // Lattice phase table exp(-i (m psi1 + n psi2)) over rings 0..JK_RINGMAX in ring
@@ -1032,7 +1021,9 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
         }
         return;
     }
-    constexpr int FT = 32;                         // phasings per workgroup (16 * TN)
+    // phasings per workgroup: 32 JK_MFMA_WH for the MFMA kernel, 32 (16 * TN) for the
+    // tiled one
+    const int FT = mixed ? 32 * JK_MFMA_WH : 32;
     const int ldp = (nf + FT - 1) / FT * FT;
     const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
     const size_t elt = mixed ? sizeof(float) : sizeof(double);
@@ -1040,19 +1031,23 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
     dim3 grid((unsigned)((nufp + 8) / 9), (unsigned)(ldp / FT));
     const int64_t nblk = (int64_t)grid.x * grid.y;
     const size_t phase_bytes = (size_t)E * 2 * ldp * elt;
+    const size_t lat_bytes = mixed ? (size_t)E * sizeof(LatPt) : 0;
     const size_t cnt_bytes = total ? (size_t)nblk * sizeof(unsigned long long) : 0;
     void *ph = nullptr;
-    if (hipMallocAsync(&ph, phase_bytes + cnt_bytes, st) != hipSuccess) {
+    if (hipMallocAsync(&ph, phase_bytes + lat_bytes + cnt_bytes, st) != hipSuccess) {
         (void)hipGetLastError();
         throw HipError("hipMallocAsync of the lattice phase table failed");
     }
+    LatPt *lat = reinterpret_cast<LatPt *>((char *)ph + phase_bytes);
     unsigned long long *bcnt =
-        total ? reinterpret_cast<unsigned long long *>((char *)ph + phase_bytes) : nullptr;
+        total ? reinterpret_cast<unsigned long long *>((char *)ph + phase_bytes + lat_bytes)
+              : nullptr;
     if (mixed) {
         hipLaunchKernelGGL(k_phase_table<float>, pg, dim3(256), 0, st, d_psi, nf, ldp, (float *)ph);
-        hipLaunchKernelGGL(k_jk_mfma<float>, grid, dim3(256), 0, st, sh, nufp, ulocal, us1[0],
-                           us1[1], us2[0], us2[1], (const float *)ph, ldp, nf, jk, jk_stride,
-                           bcnt);
+        hipLaunchKernelGGL(k_lattice, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, us1[0],
+                           us1[1], us2[0], us2[1], lat);
+        hipLaunchKernelGGL(k_jk_mfma<JK_MFMA_WH>, grid, dim3(256 * JK_MFMA_WH), 0, st, sh, nufp,
+                           ulocal, lat, (const float *)ph, ldp, nf, jk, jk_stride, bcnt);
     } else {
         hipLaunchKernelGGL(k_phase_table<double>, pg, dim3(256), 0, st, d_psi, nf, ldp,
                            (double *)ph);
