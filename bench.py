@@ -31,7 +31,7 @@ for _sub in ("pssfss.jl_amd", "oracle"):
 # HBM bytes per launch of the mixed spatial-sum kernel on this workload, from the
 # FETCH_SIZE / WRITE_SIZE passes in profiles/ring_on_slab_traffic_pmc.md (counters
 # calibrated with a 256 MiB copy kernel: FETCH x2.00, WRITE x1.00).
-JK_MFMA_TRAFFIC_BYTES = 2923.0 * 1024 * 2.0 + 43239678.1 * 1024 * 1.0
+JK_MFMA_TRAFFIC_BYTES = 3195.75 * 1024 * 2.0 + 43239769.0 * 1024 * 1.0
 PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 (vector = matrix) / FP32 matrix
 
 
@@ -130,7 +130,7 @@ def main():
     # lattice point) that the reference's ring rule sums, counted on the device.
     st = stages[head]
     jk = st["spatial_sums"]
-    jk_kernel = ("k_jk_mfma<float> (spatial sums, FP32 MFMA)" if head == "mixed"
+    jk_kernel = ("k_jk_mfma<1> (spatial sums, FP32 MFMA)" if head == "mixed"
                  else "k_jk_tiled<double> (spatial sums, FP64 VALU)")
     achieved = jk["flops"] / (jk["ms"] * 1e-3) / 1e12 if jk["ms"] > 0 else 0.0
     roof = {"bound": "mfma", "kernel": jk_kernel, "achieved": round(achieved, 3),
