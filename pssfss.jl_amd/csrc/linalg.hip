@@ -250,6 +250,8 @@ __global__ __launch_bounds__(256) void k_panel_copy(int64_t n, int64_t k, int kb
 // partial pivoting (largest |re| + |im|, first index on ties: izamax), the rows
 // in registers, R per thread.  An interchange swaps two register rows, so the
 // rows go back to memory in the order LAPACK's sequential interchanges give.
+// Two barriers per column: after the wavefront winners of the pivot search, and
+// after the pivot row (and, on an interchange, row j) is staged in LDS.
 // ipiv receives absolute pivot rows (k is the panel's first row).
 template <class C, int R, int LW>
 __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0, int64_t sPt,
@@ -259,11 +261,9 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
     C *P = P0 + (int64_t)blockIdx.x * sPt;
     int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
     const int mr = (int)(m - c0);
-    __shared__ C prow[LW];
     __shared__ C xch[2][LW];
     __shared__ Rl rv[16];
     __shared__ int ri[16];
-    __shared__ int pivs;
     const int tid = threadIdx.x;
     const C zero = CTraits<C>::zero();
     C x[R][LW];
@@ -296,44 +296,40 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
         }
         if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
         __syncthreads();
-        if (tid == 0) {
+        // every thread reduces the 16 wavefront winners (no second barrier)
+        int p;
+        {
             Rl b = rv[0];
             int bx = ri[0];
             for (int w = 1; w < NT / 64; ++w)
                 if (rv[w] > b || (rv[w] == b && ri[w] < bx)) { b = rv[w]; bx = ri[w]; }
             if (bx >= mr) bx = j;   // no comparable entry (NaN column): keep the diagonal
-            pivs = bx;
-            ipiv[k + c0 + j] = (int32_t)(k + c0 + bx);
+            p = bx;
         }
-        __syncthreads();
-        const int p = pivs;
+        if (tid == 0) ipiv[k + c0 + j] = (int32_t)(k + c0 + p);
+        // the pivot row goes to xch[0] (it is also the row the update uses); on an
+        // interchange row j goes to xch[1] for the pivot row's owner
         const int op = p % NT, sp = p / NT;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (tid == op && r == sp)
+#pragma unroll
+                for (int c = 0; c < LW; ++c) xch[0][c] = x[r][c];
+        if (p != j && tid == j)
+#pragma unroll
+            for (int c = 0; c < LW; ++c) xch[1][c] = x[0][c];
+        __syncthreads();
         if (p != j) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (tid == op && r == sp)
-#pragma unroll
-                    for (int c = 0; c < LW; ++c) xch[0][c] = x[r][c];
-            }
-            if (tid == j)
-#pragma unroll
-                for (int c = 0; c < LW; ++c) xch[1][c] = x[0][c];
-            __syncthreads();
             if (tid == j)
 #pragma unroll
                 for (int c = 0; c < LW; ++c) x[0][c] = xch[0][c];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < R; ++r)
                 if (tid == op && r == sp)
 #pragma unroll
                     for (int c = 0; c < LW; ++c) x[r][c] = xch[1][c];
-            }
         }
-        if (tid == j)
-#pragma unroll
-            for (int c = 0; c < LW; ++c) prow[c] = x[0][c];
-        __syncthreads();
-        const C piv = prow[j];
+        const C piv = xch[0][j];
         const bool nz = !(piv.x == 0 && piv.y == 0);
         const C rp = nz ? CTraits<C>::one() / piv : zero;
 #pragma unroll
@@ -344,7 +340,7 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
                 x[r][j] = l;
                 const C ml = -l;
 #pragma unroll
-                for (int c = j + 1; c < LW; ++c) cfma(x[r][c], ml, prow[c]);
+                for (int c = j + 1; c < LW; ++c) cfma(x[r][c], ml, xch[0][c]);
             }
         }
     }
