@@ -87,6 +87,7 @@ struct pssfss_ctx {
     DBuf<int32_t> wpiv, wperm, wsel;
     DBuf<cd> wT;
     DBuf<cf> wTf;
+    DBuf<cf> wDinv;              // inverses of the FP32 factors' diagonal blocks
     DBuf<double> wnorm;
     DBuf<cd> wG;                 // fill class terms
     DBuf<double> wanorm;         // infinity norms of the systems (refinement)
@@ -594,7 +595,8 @@ constexpr int IR_MAXIT = 30;
 // Mixed-precision refinement of Z X = B (tr false) or Z^T X = B (tr true) for the
 // listed systems, from the FP32 factors in ctx->wZf / ctx->wperm and the FP64 Z in
 // ctx->wZ.  B (device, n x ncol per system) is read; X goes to ctx->wX.  The
-// first sweep solves with the FP32 factors from X = 0; each sweep then forms the
+// first sweep solves with the FP32 factors from X = 0 (diagonal blocks applied
+// through their inverses in ctx->wDinv, except for tr); each sweep then forms the
 // FP64 residual and stops with LAPACK zcgesv's rule (max_c ||r_c|| < sqrt(n)
 // ||x_c|| ||op(Z)|| eps), at IR_MAXIT sweeps, or when a sweep does not halve the
 // backward error.  Systems not listed keep whatever ctx->wX holds.
@@ -634,8 +636,9 @@ Refined refine_mixed(pssfss_ctx *ctx, int64_t n, int ncol, int np, bool tr, cons
     d2h(ctx, anorm.data(), ctx->wanorm.p, np);
     launch_zero(ctx->wX.p, sB * np, st);
     convert_z_to_f32(B, ctx->wRf.p, sB * np, st);
+    const cf *dinv = tr ? nullptr : ctx->wDinv.p;
     lu_solve_batched_f32(n, ncol, (int)act.size(), ctx->wZf.p, nb2, ctx->wperm.p, n, ctx->wRf.p,
-                         sB, ctx->wTf.p, st, ctx->wsel.p, tr);
+                         sB, ctx->wTf.p, st, ctx->wsel.p, tr, dinv);
     add_correction_sel(ctx->wX.p, ctx->wRf.p, sB, ctx->wsel.p, (int)act.size(), st);
     std::vector<double> prev(np, 1e300);
     std::vector<int> done(np, 0);
@@ -668,7 +671,7 @@ Refined refine_mixed(pssfss_ctx *ctx, int64_t n, int ncol, int np, bool tr, cons
         // only the listed systems are solved and corrected
         residual_to_f32(ctx->wR.p, ctx->wRf.p, sB * np, st);
         lu_solve_batched_f32(n, ncol, (int)act.size(), ctx->wZf.p, nb2, ctx->wperm.p, n,
-                             ctx->wRf.p, sB, ctx->wTf.p, st, ctx->wsel.p, tr);
+                             ctx->wRf.p, sB, ctx->wTf.p, st, ctx->wsel.p, tr, dinv);
         add_correction_sel(ctx->wX.p, ctx->wRf.p, sB, ctx->wsel.p, (int)act.size(), st);
     }
     return out;
@@ -702,6 +705,8 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st);
     stage_begin(5, st);
     lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
+    ctx->wDinv.ensure((size_t)np * 2 * ((n + 63) / 64) * 4096);
+    lu_diag_inverses_f32(n, np, ctx->wZf.p, nb2, ctx->wDinv.p, st);
     std::vector<int> all(np);
     for (int p = 0; p < np; ++p) all[p] = p;
     Refined rf = refine_mixed(ctx, n, ncol, np, false, ctx->wB.p, all);

@@ -171,9 +171,16 @@ void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv,
                            int64_t sP, hipStream_t st);
 // sel (optional): the batch lists system indices, one per entry.  tr: Z^T x = b.
+// dinv (optional, not with tr): the diagonal-block inverses of lu_diag_inverses_f32,
+// used in place of the sequential diagonal solves.
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
                           const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
-                          hipStream_t st, const int32_t *sel = nullptr, bool tr = false);
+                          hipStream_t st, const int32_t *sel = nullptr, bool tr = false,
+                          const cf *dinv = nullptr);
+// Inverses of the 64 x 64 diagonal blocks of L and U for each of the batch systems:
+// dinv holds batch * 2 * ceil(n / 64) * 4096 entries.
+void lu_diag_inverses_f32(int64_t n, int batch, const cf *LU, int64_t sA, cf *dinv,
+                          hipStream_t st);
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st);
 // r = b - A x (FP64; a row-per-wavefront kernel for up to 8 right-hand sides, else
 // one transposed-A GEMM per system), batched; then the largest

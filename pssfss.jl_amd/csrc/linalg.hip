@@ -541,6 +541,51 @@ __global__ void k_unperm_rows(int64_t n, int nrhs, const int32_t *perm0, int64_t
     }
 }
 
+// Inverses of the 64 x 64 diagonal blocks of L (unit lower, which 0) and U (upper,
+// which 1) of each factored system, row-major 64 x 64 each, zero beyond the block
+// size: Dinv[((system * 2 + which) * nblk + block) * 4096].  One thread per column
+// of the inverse, solving L x = e_c (or U x = e_c) with the block in LDS.
+template <class C>
+__global__ __launch_bounds__(64) void k_diag_inv(int64_t n, const C *LU0, int64_t sA, C *Dinv) {
+    const int64_t nblk = (n + 63) / 64;
+    const int64_t b = blockIdx.x, sys = blockIdx.y;
+    const int which = blockIdx.z;
+    const int64_t b0 = 64 * b;
+    const int bs = n - b0 < 64 ? (int)(n - b0) : 64;
+    const C *LU = LU0 + sys * sA;
+    __shared__ C D[64][65];
+    __shared__ C X[64][65];
+    const int c = threadIdx.x;
+    const C zero = CTraits<C>::zero(), one = CTraits<C>::one();
+    for (int r = 0; r < 64; ++r) {
+        C v = zero;
+        if (r < bs && c < bs) v = LU[(b0 + r) * n + b0 + c];
+        D[r][c] = v;
+        X[r][c] = zero;
+    }
+    __syncthreads();
+    if (c < bs) {
+        if (which == 0) {
+            X[c][c] = one;
+            for (int r = c + 1; r < bs; ++r) {
+                C v = zero;
+                for (int l = c; l < r; ++l) cfma(v, D[r][l], X[l][c]);
+                X[r][c] = -v;
+            }
+        } else {
+            X[c][c] = one / D[c][c];
+            for (int r = c - 1; r >= 0; --r) {
+                C v = zero;
+                for (int l = r + 1; l <= c; ++l) cfma(v, D[r][l], X[l][c]);
+                X[r][c] = -(v / D[r][r]);
+            }
+        }
+    }
+    __syncthreads();
+    C *o = Dinv + ((sys * 2 + which) * nblk + b) * 4096;
+    for (int r = 0; r < 64; ++r) o[r * 64 + c] = X[r][c];
+}
+
 // One step of the blocked triangular solves L y = b (unit lower, UP false) and
 // U x = y (UP true) on a column-major X (n x nrhs per system), in place.  With TR
 // the factors are used transposed, for Z^T x = b: U^T y = b (lower, UP false) and
@@ -555,7 +600,8 @@ __global__ void k_unperm_rows(int64_t n, int nrhs, const int32_t *perm0, int64_t
 template <class C, int NRG, bool UP, bool TR>
 __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C *LU0, int64_t sA,
                                                    C *X0, int64_t sB, const int32_t *sel,
-                                                   int64_t s0, int ss, int64_t d0, int ds) {
+                                                   int64_t s0, int ss, int64_t d0, int ds,
+                                                   const C *Dinv0) {
     constexpr int NS = NRG / 4;
     const int64_t sys = sel ? sel[blockIdx.z] : blockIdx.z;
     const C *LU = LU0 + sys * sA;
@@ -622,6 +668,33 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
         for (int s = 0; s < NS; ++s) {
             const int c = q + 4 * s;
             if (lane < rows && c < nc) X[r0 + lane + n * c] = y[s];
+        }
+        return;
+    }
+    if (Dinv0) {
+        // the diagonal block's inverse (k_diag_inv): y <- D^{-1} y, a 64 x 64
+        // product per right-hand side through LDS instead of the sequential solve
+        const int64_t nblk = (n + 63) / 64;
+        const C *Di = Dinv0 + ((sys * 2 + (UP ? 1 : 0)) * nblk + d0 / 64) * 4096;
+        __syncthreads();
+        for (int p = 0; p < 16; ++p) {
+            const int rr = q + 4 * p;
+            Tt[lane][rr] = Di[rr * 64 + lane];      // Tt[col][row] = D^{-1}[row][col]
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) XS[lane][q + 4 * s] = y[s];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) y[s] = zero;
+        for (int r = 0; r < ds; ++r) {
+            const C dl = Tt[r][lane];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) cfma(y[s], dl, XS[r][q + 4 * s]);
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int c = q + 4 * s;
+            if (lane < ds && c < nc) X[d0 + lane + n * c] = y[s];
         }
         return;
     }
@@ -980,7 +1053,7 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
 template <class C, bool TR = false>
 void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const int32_t *perm,
                 int64_t sP, C *B, int64_t sB, C *scratch, hipStream_t st,
-                const int32_t *sel = nullptr) {
+                const int32_t *sel = nullptr, const C *dinv = nullptr) {
     if (batch <= 0 || nrhs <= 0) return;
     const int64_t cnt = n * nrhs;
     const dim3 gp((unsigned)std::min<int64_t>((cnt + 255) / 256, 1024), (unsigned)batch);
@@ -996,7 +1069,7 @@ void lu_solve_t(int64_t n, int nrhs, int batch, const C *LU, int64_t sA, const i
     const int64_t nblk = (n + 63) / 64;
     auto step = [&](bool up, int64_t s0, int ss, int64_t d0, int ds, int64_t ntile) {
         const dim3 g((unsigned)(1 + ntile), gy, (unsigned)batch);
-#define TS(NRG, UP) hipLaunchKernelGGL((k_trsv_step<C, NRG, UP, TR>), g, dim3(256), 0, st, n, nrhs, LU, sA, scratch, sB, sel, s0, ss, d0, ds)
+#define TS(NRG, UP) hipLaunchKernelGGL((k_trsv_step<C, NRG, UP, TR>), g, dim3(256), 0, st, n, nrhs, LU, sA, scratch, sB, sel, s0, ss, d0, ds, TR ? nullptr : dinv)
         if (nrg == 4) { if (up) TS(4, true); else TS(4, false); }
         else if (nrg == 8) { if (up) TS(8, true); else TS(8, false); }
         else { if (up) TS(16, true); else TS(16, false); }
@@ -1069,9 +1142,17 @@ void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipi
 
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
                           const int32_t *perm, int64_t sP, cf *B, int64_t sB, cf *scratch,
-                          hipStream_t st, const int32_t *sel, bool tr) {
+                          hipStream_t st, const int32_t *sel, bool tr, const cf *dinv) {
     if (tr) lu_solve_t<cf, true>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
-    else lu_solve_t<cf, false>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel);
+    else lu_solve_t<cf, false>(n, nrhs, batch, LU, sA, perm, sP, B, sB, scratch, st, sel, dinv);
+}
+
+void lu_diag_inverses_f32(int64_t n, int batch, const cf *LU, int64_t sA, cf *dinv,
+                          hipStream_t st) {
+    const int64_t nblk = (n + 63) / 64;
+    hipLaunchKernelGGL(k_diag_inv<cf>, dim3((unsigned)nblk, (unsigned)batch, 2), dim3(64), 0, st,
+                       n, LU, sA, dinv);
+    HIPCHECK(hipGetLastError());
 }
 
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st) {
