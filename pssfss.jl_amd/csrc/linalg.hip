@@ -444,33 +444,6 @@ __global__ void k_rowswap(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
     }
 }
 
-// U12 = L11^{-1} A12 (unit lower L11 = A[k, k + kb) x [k, k + kb), kb <= 64) for
-// the rows [k, k + kb) and columns [cbeg, cend), one thread per column.
-template <class C>
-__global__ __launch_bounds__(64) void k_trsm_u12(int64_t n, int64_t k, int kb, C *A0,
-                                                 int64_t sA, int64_t cbeg, int64_t cend) {
-    C *A = A0 + (int64_t)blockIdx.y * sA;
-    __shared__ C L[SB][SB];
-    for (int t = threadIdx.x; t < kb * kb; t += blockDim.x) {
-        int r = t / kb, l = t % kb;
-        L[r][l] = A[(k + r) * n + k + l];
-    }
-    __syncthreads();
-    const int64_t c = cbeg + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cend) return;
-    C x[SB];
-#pragma unroll
-    for (int r = 0; r < SB; ++r) {
-        if (r < kb) {
-            C v = A[(k + r) * n + c];
-#pragma unroll
-            for (int l = 0; l < r; ++l) cfma(v, -L[r][l], x[l]);
-            x[r] = v;
-            A[(k + r) * n + c] = v;
-        }
-    }
-}
-
 // Row permutation of the factorisation: perm[i] = original row now at row i,
 // from LAPACK's sequential interchanges (one workgroup per system, LDS).
 __global__ __launch_bounds__(256) void k_make_perm(int64_t n, const int32_t *ipiv0, int64_t sP,
@@ -541,18 +514,14 @@ __global__ void k_unperm_rows(int64_t n, int nrhs, const int32_t *perm0, int64_t
     }
 }
 
-// Inverses of the 64 x 64 diagonal blocks of L (unit lower, which 0) and U (upper,
-// which 1) of each factored system, row-major 64 x 64 each, zero beyond the block
-// size: Dinv[((system * 2 + which) * nblk + block) * 4096].  One thread per column
-// of the inverse, solving L x = e_c (or U x = e_c) with the block in LDS.
+// Inverse of one triangular diagonal block of a row-major factor, rows and columns
+// [b0, b0 + bs) (bs <= 64): unit lower (upper false) or upper.  Called by all 64
+// threads of a workgroup; thread c computes column c by solving L x = e_c (U x =
+// e_c) with the block in LDS.  Writes the row-major 64 x 64 inverse (zero beyond
+// bs) to out.
 template <class C>
-__global__ __launch_bounds__(64) void k_diag_inv(int64_t n, const C *LU0, int64_t sA, C *Dinv) {
-    const int64_t nblk = (n + 63) / 64;
-    const int64_t b = blockIdx.x, sys = blockIdx.y;
-    const int which = blockIdx.z;
-    const int64_t b0 = 64 * b;
-    const int bs = n - b0 < 64 ? (int)(n - b0) : 64;
-    const C *LU = LU0 + sys * sA;
+__device__ void tri_inverse_block(const C *LU, int64_t n, int64_t b0, int bs, bool upper,
+                                  C *out) {
     __shared__ C D[64][65];
     __shared__ C X[64][65];
     const int c = threadIdx.x;
@@ -565,7 +534,7 @@ __global__ __launch_bounds__(64) void k_diag_inv(int64_t n, const C *LU0, int64_
     }
     __syncthreads();
     if (c < bs) {
-        if (which == 0) {
+        if (!upper) {
             X[c][c] = one;
             for (int r = c + 1; r < bs; ++r) {
                 C v = zero;
@@ -582,8 +551,47 @@ __global__ __launch_bounds__(64) void k_diag_inv(int64_t n, const C *LU0, int64_
         }
     }
     __syncthreads();
-    C *o = Dinv + ((sys * 2 + which) * nblk + b) * 4096;
-    for (int r = 0; r < 64; ++r) o[r * 64 + c] = X[r][c];
+    for (int r = 0; r < 64; ++r) out[r * 64 + c] = X[r][c];
+}
+
+// Inverses of the 64 x 64 diagonal blocks of L (unit lower, which 0) and U (upper,
+// which 1) of each factored system: Dinv[((system * 2 + which) * nblk + block) *
+// 4096], row-major 64 x 64.
+template <class C>
+__global__ __launch_bounds__(64) void k_diag_inv(int64_t n, const C *LU0, int64_t sA, C *Dinv) {
+    const int64_t nblk = (n + 63) / 64;
+    const int64_t b = blockIdx.x, sys = blockIdx.y;
+    const int which = blockIdx.z;
+    const int64_t b0 = 64 * b;
+    const int bs = n - b0 < 64 ? (int)(n - b0) : 64;
+    tri_inverse_block<C>(LU0 + sys * sA, n, b0, bs, which == 1,
+                         Dinv + ((sys * 2 + which) * nblk + b) * 4096);
+}
+
+// Inverses of the unit-lower diagonal blocks of panel [k, k + kb) (one or two
+// blocks of SB rows): Linv[(system * 2 + h) * 4096].
+template <class C>
+__global__ __launch_bounds__(64) void k_panel_linv(int64_t n, const C *A0, int64_t sA, int64_t k,
+                                                   int kb, C *Linv) {
+    const int h = blockIdx.x;
+    const int64_t sys = blockIdx.y;
+    const int bs = kb - SB * h < SB ? kb - SB * h : SB;
+    tri_inverse_block<C>(A0 + sys * sA, n, k + SB * h, bs, false, Linv + (sys * 2 + h) * 4096);
+}
+
+// T = rows [r0, r0 + nr) x columns [c0, n) of the row-major A (nr x (n - c0),
+// leading dimension n - c0) per system.
+template <class C>
+__global__ void k_copy_rows(int64_t n, int64_t r0, int nr, int64_t c0, const C *A0, int64_t sA,
+                            C *T0, int64_t sT) {
+    const int64_t cols = n - c0, tot = nr * cols;
+    const C *A = A0 + (int64_t)blockIdx.y * sA;
+    C *T = T0 + (int64_t)blockIdx.y * sT;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = t / cols, j = t - l * cols;
+        T[t] = A[(r0 + l) * n + c0 + j];
+    }
 }
 
 // One step of the blocked triangular solves L y = b (unit lower, UP false) and
@@ -936,24 +944,39 @@ void rowswap_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, const int32_t *ip
     HIPCHECK(hipGetLastError());
 }
 
-// U12 of panel [k, k + kb) for the columns [k + kb, n): the first SB rows against
-// their unit-lower triangle, the rank-SB update of the remaining panel rows (a
-// GEMM on the transposed row-major views), then their own triangle.
+// U12 of panel [k, k + kb) for the columns [k + kb, n), per block of SB rows: the
+// inverse of its unit-lower diagonal block times the block's rows (copied to T
+// first, one GEMM on the transposed row-major views); between the blocks the
+// rank-SB update of the second block's rows.  Linv holds 2 x 4096 and T SB x n
+// entries per system.
 template <class C>
-void trsm_u12_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, int batch, hipStream_t st) {
+void trsm_u12_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, int batch, C *Linv, C *T,
+                hipStream_t st) {
     const int64_t c0 = k + kb, cols = n - c0;
     if (cols <= 0) return;
-    dim3 g((unsigned)((cols + 63) / 64), (unsigned)batch);
-    hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k, std::min(kb, SB), A, sA, c0, n);
+    const int nh = (kb + SB - 1) / SB;
+    hipLaunchKernelGGL(k_panel_linv<C>, dim3((unsigned)nh, (unsigned)batch), dim3(64), 0, st, n, A,
+                       sA, k, kb, Linv);
     HIPCHECK(hipGetLastError());
-    if (kb > SB) {
-        C mone, one;
-        mone.x = -1; mone.y = 0;
-        one.x = 1; one.y = 0;
-        gemm_t<C>((int)cols, kb - SB, SB, mone, A + k * n + c0, n, sA, A + (k + SB) * n + k, n, sA,
-                  one, A + (k + SB) * n + c0, n, sA, batch, st);
-        hipLaunchKernelGGL(k_trsm_u12<C>, g, dim3(64), 0, st, n, k + SB, kb - SB, A, sA, c0, n);
+    C mone, one, zero;
+    mone.x = -1; mone.y = 0;
+    one.x = 1; one.y = 0;
+    zero.x = 0; zero.y = 0;
+    const int64_t sT = (int64_t)SB * n;
+    for (int h = 0; h < nh; ++h) {
+        const int64_t r0 = k + SB * h;
+        const int hb = std::min(SB, kb - SB * h);
+        if (h == 1)
+            gemm_t<C>((int)cols, hb, SB, mone, A + k * n + c0, n, sA, A + (k + SB) * n + k, n, sA,
+                      one, A + (k + SB) * n + c0, n, sA, batch, st);
+        const int64_t tot = (int64_t)hb * cols;
+        hipLaunchKernelGGL(k_copy_rows<C>,
+                           dim3((unsigned)std::min<int64_t>((tot + 255) / 256, 1024),
+                                (unsigned)batch),
+                           dim3(256), 0, st, n, r0, hb, c0, A, sA, T, sT);
         HIPCHECK(hipGetLastError());
+        gemm_t<C>((int)cols, hb, hb, one, T, cols, sT, Linv + 4096 * h, 64, 2 * 4096, zero,
+                  A + r0 * n + c0, n, sA, batch, st);
     }
 }
 
@@ -1017,9 +1040,10 @@ void panel_leaves(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv,
 template <class C>
 void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
                  hipStream_t st) {
-    // column-major panel scratch, n x NB per system
+    // column-major panel scratch (n x NB per system), the U12 block rows (SB x n) and
+    // the panel's diagonal-block inverses (2 x 64 x 64)
     const int64_t sPt = n * NB;
-    DBuf<C> Pt((size_t)sPt * batch);
+    DBuf<C> Pt((size_t)sPt * batch), T((size_t)SB * n * batch), Linv((size_t)2 * 4096 * batch);
     for (int64_t k = 0; k < n; k += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k);
         stage_begin(3, st);
@@ -1031,7 +1055,7 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
             HIPCHECK(hipGetLastError());
         }
         rowswap_t<C>(n, k, kb, A, sA, ipiv, sP, 0, n, k, k + kb, batch, st);
-        trsm_u12_t<C>(n, k, kb, A, sA, batch, st);
+        trsm_u12_t<C>(n, k, kb, A, sA, batch, Linv.p, T.p, st);
         stage_end(3, st);
         const int64_t rest = n - k - kb;
         if (rest > 0) {
