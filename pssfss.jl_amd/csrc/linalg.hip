@@ -278,7 +278,7 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
     }
 #pragma unroll
     for (int j = 0; j < LW; ++j) {
-        if (j >= lw) break;
+        if (j < lw) {
         Rl best = (Rl)-1;
         int bi = 0x7fffffff;
 #pragma unroll
@@ -343,6 +343,7 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
                 for (int c = j + 1; c < LW; ++c) cfma(x[r][c], ml, xch[0][c]);
             }
         }
+        }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -366,8 +367,8 @@ __global__ __launch_bounds__(NB) void k_leaf_apply(int64_t m, int kb, int ce, in
     const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
     __shared__ C L[LW][LW];
     const int c = threadIdx.x;
-    if (c < lw * lw) {
-        const int a = c / lw, b = c % lw;
+    for (int t = c; t < lw * lw; t += blockDim.x) {
+        const int a = t / lw, b = t % lw;
         L[a][b] = P[(int64_t)(c0 + b) * m + c0 + a];
     }
     __syncthreads();
@@ -890,10 +891,13 @@ void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, c
 constexpr int64_t LEAF_MAX_ROWS = 12288;
 
 // Leaf width for a panel of m rows: as wide as R x LW complex registers allow
-// (R = ceil(m / 1024) rows per thread, at most 24 FP64 or 48 FP32 complex).
+// (R = ceil(m / 1024) rows per thread, at most 24 FP64 or 48 FP32 complex, and
+// no more than 16 per row: larger register arrays are placed in scratch memory).
+// Wider leaves mean fewer rank-LW updates of the sub-panel.
 template <class C>
 int leaf_width(int64_t m) {
     constexpr bool F = sizeof(C) == sizeof(cf);
+    if (m <= 1024) return F ? 16 : 8;
     if (m <= 3072) return 8;
     if (m <= 6144) return F ? 8 : 4;
     return F ? 4 : 2;
@@ -905,25 +909,30 @@ void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, in
     dim3 g((unsigned)batch), b(1024);
 #define LF(RR, LL) hipLaunchKernelGGL((k_leaf<C, RR, LL>), g, b, 0, st, m, c0, lw, P, sPt, ipiv, sP, k)
     constexpr bool F = sizeof(C) == sizeof(cf);
-    if (m <= 1024) LF(1, 8);
-    else if (m <= 2048) LF(2, 8);
-    else if (m <= 3072) LF(3, 8);
-    else if constexpr (F) {
-        if (m <= 6144) LF(6, 8);
+    if constexpr (F) {
+        if (m <= 1024) LF(1, 16);
+        else if (m <= 2048) LF(2, 8);
+        else if (m <= 3072) LF(3, 8);
+        else if (m <= 6144) LF(6, 8);
         else LF(12, 4);
     } else {
-        if (m <= 6144) LF(6, 4);
+        if (m <= 1024) LF(1, 8);
+        else if (m <= 2048) LF(2, 8);
+        else if (m <= 3072) LF(3, 8);
+        else if (m <= 6144) LF(6, 4);
         else LF(12, 2);
     }
 #undef LF
     HIPCHECK(hipGetLastError());
 }
-
 template <class C>
 void launch_leaf_apply(int64_t m, int kb, int ce, int c0, int lw, C *P, int64_t sPt,
                        const int32_t *ipiv, int64_t sP, int64_t k, int batch, hipStream_t st) {
     dim3 g((unsigned)batch), b(NB);
-    if (lw > 4)
+    if (lw > 8)
+        hipLaunchKernelGGL((k_leaf_apply<C, 16>), g, b, 0, st, m, kb, ce, c0, lw, P, sPt, ipiv,
+                           sP, k);
+    else if (lw > 4)
         hipLaunchKernelGGL((k_leaf_apply<C, 8>), g, b, 0, st, m, kb, ce, c0, lw, P, sPt, ipiv, sP,
                            k);
     else
