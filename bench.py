@@ -30,8 +30,8 @@ for _sub in ("pssfss.jl_amd", "oracle"):
 
 # HBM bytes per launch of the mixed spatial-sum kernel on this workload, from the
 # FETCH_SIZE / WRITE_SIZE passes in profiles/ring_on_slab_traffic_pmc.md (counters
-# calibrated with a 256 MiB copy kernel: FETCH x2.00, WRITE x1.00).
-JK_MFMA_TRAFFIC_BYTES = 3195.75 * 1024 * 2.0 + 43239769.0 * 1024 * 1.0
+# calibrated with the 256 MiB copy of --calibrate copy: FETCH x2.00, WRITE x1.00).
+JK_MFMA_TRAFFIC_BYTES = 3161.9 * 1024 * 2.0 + 43239628.8 * 1024 * 1.0
 PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 (vector = matrix) / FP32 matrix
 
 
@@ -58,7 +58,19 @@ def main():
                     help="skip the 4-sheet meanderline polarizer sample")
     ap.add_argument("--no-cross", action="store_true",
                     help="skip the batched cross-problem sample")
+    ap.add_argument("--calibrate", choices=["copy", "fma"],
+                    help="run only a counter-calibration kernel of known work (under rocprofv3): "
+                         "copy = 256 MiB device copy x 3, fma = FP64 and FP32 FMA loops of "
+                         "2 * 4096 * 256 * 4096 flops each")
     args = ap.parse_args()
+    if args.calibrate:
+        import ctypes
+        cl = ctypes.CDLL(os.path.join(ROOT, "pssfss.jl_amd", "libpssfss_calib.so"))
+        cl.pssfss_calib_copy.argtypes = [ctypes.c_int64, ctypes.c_int]
+        rc = (cl.pssfss_calib_copy(256 << 20, 3) if args.calibrate == "copy"
+              else cl.pssfss_calib_fma(4096, 4096))
+        print(f"[bench] calibration {args.calibrate}: status {rc}", flush=True)
+        sys.exit(0 if rc == 0 else 1)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(points shard across GPUs independently; see DESIGN.md)", file=sys.stderr)
