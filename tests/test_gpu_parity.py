@@ -62,10 +62,9 @@ def test_lu_pivoting(ctx):
 @pytest.mark.parametrize("n", [1100, 2100, 3100, 4500])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_lu_pivoting_large(ctx, n, prec):
-    """Unscaled Gaussian matrices: pivots land anywhere in tall panels of 1-6 rows per
-    thread (register panels of R = 2, 3 and 6; panels above 3072 rows go through the
-    8-column leaves) and solutions come from both the LDS-resident and the
-    streaming triangular-solve variants."""
+    """Unscaled Gaussian matrices: pivots land anywhere in the leaves of 1 to 5 rows
+    per thread (leaf widths 16/8 FP32 and 8/4 FP64), across the two 64-column
+    sub-panels of each 128-column panel, and through the blocked solves."""
     _say(f"lu_solve random Gaussian n={n} {prec}")
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
@@ -75,6 +74,24 @@ def test_lu_pivoting_large(ctx, n, prec):
     err = _rel(X[0], ref)
     print(f"  rel err {err:.2e}")
     assert err < (1e-10 if prec == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("n", [7000, 12600])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lu_large_backward_error(ctx, n, prec):
+    """Large unscaled Gaussian systems: leaves of 12 rows per thread (widths 4 FP32,
+    2 FP64) and, at 12600, first panels taller than 12288 rows (the streaming panel
+    kernel).  Checked by the normwise backward error max|b - A x| / (||A||_inf
+    max|x|), which a backward-stable solve keeps near n eps."""
+    _say(f"lu_solve random Gaussian n={n} {prec}, backward error")
+    rng = np.random.default_rng(n + 1)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    X, _, _ = ctx.lu_solve(A, B, prec)
+    r = B - A @ X[0]
+    be = np.abs(r).max() / (np.abs(A).sum(axis=1).max() * np.abs(X[0]).max())
+    print(f"  backward error {be:.2e}")
+    assert be < 1e-13
 
 
 @pytest.mark.parametrize("n1,n2,n3", [(1, 1, 1), (2, 2, 2), (2, 6, 3), (10, 20, 15), (34, 34, 34)])
