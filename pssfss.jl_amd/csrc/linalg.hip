@@ -207,10 +207,10 @@ __global__ __launch_bounds__(1024) void k_panel(int64_t n, int64_t k, int kb, C 
     }
 }
 
-// The 64-column panel (rows [k, n)) is factored in a column-major scratch copy
+// The NB-column panel (rows [k, n)) is factored in a column-major scratch copy
 // (m = n - k rows, leading dimension m, one per system), so the thread-per-row
 // accesses of the leaf kernel are coalesced.  dir 0 copies the panel in, dir 1
-// copies it back; 64-row tiles are transposed through LDS.
+// copies it back; 64-row x 64-column tiles are transposed through LDS.
 template <class C>
 __global__ __launch_bounds__(256) void k_panel_copy(int64_t n, int64_t k, int kb, C *A0,
                                                     int64_t sA, C *P0, int64_t sPt, int dir) {
