@@ -163,6 +163,14 @@ int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
 int pssfss_fill(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt, double *zmat,
                 pssfss_point_info *info);
 
+/* d/dk0 of the fill (no counterpart in the reference): Z as pssfss_fill and its
+ * derivative with respect to the free-space wavenumber pt->k0 (1/m) at fixed
+ * phasing (pt->psi1, pt->psi2) and fixed u, both nbf x nbf complex column-major.
+ * The spectral tables are differentiated summand by summand (same table size M);
+ * the spatial sums and singular integrals do not depend on k0.  'J' sheets. */
+int pssfss_fill_dk0(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt,
+                    double *zmat, double *dzmat, pssfss_point_info *info);
+
 /* filljk! (src/Zint.jl:54-179): jk = (6, nufp) complex (J, J_xi, J_eta, K, K_xi,
  * K_eta per class), rho_r = (2, nufp), rinv = nufp.  u_local in 1/sheet unit. */
 int pssfss_filljk(pssfss_ctx *ctx, pssfss_sheet *sheet, double u_local, double psi1,

@@ -37,6 +37,21 @@ cd d3_calc(double k0, double u, cd mu1, cd eps1, cd mu2, cd eps2) {
     return num / ((2.0 * (mu1 + mu2)) * (eps1 + eps2));
 }
 
+// d/dk0 of c3_calc / d3_calc: both are linear in w_i = k0^2 mu_i eps_i + u^2, so
+// the derivative is the same formula with dw_i/dk0 = 2 k0 mu_i eps_i in place of w_i.
+cd c3_dk0(double k0, cd mu1, cd eps1, cd mu2, cd eps2) {
+    cd w1 = ((2.0 * k0) * mu1) * eps1;
+    cd w2 = ((2.0 * k0) * mu2) * eps2;
+    return (mu1 * w2 + mu2 * w1) / (2.0 * (mu1 + mu2));
+}
+cd d3_dk0(double k0, cd mu1, cd eps1, cd mu2, cd eps2) {
+    cd w1 = ((2.0 * k0) * mu1) * eps1;
+    cd w2 = ((2.0 * k0) * mu2) * eps2;
+    cd num = mu1 * (w2 * (2.0 * eps1 + eps2) - w1 * eps1) +
+             mu2 * (w1 * (2.0 * eps2 + eps1) - w2 * eps2);
+    return num / ((2.0 * (mu1 + mu2)) * (eps1 + eps2));
+}
+
 cd hcis(double t) { return cd{cos(t), sin(t)}; }
 
 // The mmax sequence of the table convergence loop (src/PGF.jl:223-233):
@@ -224,6 +239,8 @@ TabInput make_tab_input(const pssfss_point *pt, bool magnetic, const double *b1,
     if (!magnetic) {
         c.c3 = c3_calc(k0, u, ms, es, ms1, es1);
         c.d3 = d3_calc(k0, u, ms, es, ms1, es1);
+        c.dc3 = c3_dk0(k0, ms, es, ms1, es1);
+        c.dd3 = d3_dk0(k0, ms, es, ms1, es1);
     } else {
         cd c3s = c3_calc(k0, u, es, ms, es, ms), c3sp1 = c3_calc(k0, u, es1, ms1, es1, ms1);
         cd d3s = d3_calc(k0, u, es, ms, es, ms), d3sp1 = d3_calc(k0, u, es1, ms1, es1, ms1);
@@ -257,6 +274,9 @@ struct TableSet {
         return tabs.p + p * tab_stride + E * E;
     }
 };
+
+void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap,
+                   const cd *t1, const cd *t2, TableSet &out);
 
 // Tables of several points: summands ring by ring until the convergence test of
 // src/PGF.jl:291-298 passes at an mmax of the loop's sequence (rings below 40
@@ -339,6 +359,15 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
         }
         if (all) break;
     }
+    finish_tables(ctx, in, cap, t1.p, t2.p, out);
+}
+
+// Tables from ring-major summands (rings 0..M/2 of each point; out.M set): the
+// square gather, the 2-D DFT as two GEMMs, the phase of eq. (5.32) and the ghost rows.
+void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap,
+                   const cd *t1, const cd *t2, TableSet &out) {
+    hipStream_t st = ctx->st;
+    const int np = (int)in.size();
     int Mmax = 0;
     for (int p = 0; p < np; ++p) Mmax = std::max(Mmax, out.M[p]);
     out.npts = np;
@@ -360,7 +389,7 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
     dpsi.upload(hpsi.data(), 2 * np, st);
     darea.upload(harea.data(), np, st);
     stage_begin(1, st);
-    launch_gather_square(np, dM.p, Mmax, cap, t1.p, t2.p, S.p, sq, st);
+    launch_gather_square(np, dM.p, Mmax, cap, t1, t2, S.p, sq, st);
     const cd one = {1.0, 0.0}, zero = {0.0, 0.0};
     for (int p = 0; p < np; ++p) {
         const int M = out.M[p];
@@ -376,6 +405,31 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
                        Mmax, st);
     stage_end(1, st);
     sync(ctx);   // temporaries die at return
+}
+
+// d/dk0 of the tables `val` of the points `in` (electric) at fixed phasing and u:
+// the derivative summands over the rings the tables use, then the same linear
+// steps as the values, with the same M per point.
+void build_table_dk0(pssfss_ctx *ctx, const std::vector<TabInput> &in, const TableSet &val,
+                     TableSet &out) {
+    hipStream_t st = ctx->st;
+    const int np = (int)in.size();
+    std::vector<TabParams> tp(np);
+    int Mmax = 0;
+    for (int p = 0; p < np; ++p) {
+        if (in[p].tp.magnetic) throw ArgError("k0 derivative: electric ('J') sheets only");
+        tp[p] = in[p].tp;
+        Mmax = std::max(Mmax, val.M[p]);
+    }
+    DBuf<TabParams> d_tp(np);
+    d_tp.upload(tp.data(), np, st);
+    const int R = Mmax / 2;
+    const int64_t cap = (int64_t)(2 * R + 1) * (2 * R + 1);
+    DBuf<cd> t1((size_t)cap * np), t2((size_t)cap * np);
+    launch_summands_dk0(d_tp.p, np, cap, 0, R, t1.p, t2.p, st);
+    out.M = val.M;
+    out.converged = val.converged;
+    finish_tables(ctx, in, cap, t1.p, t2.p, out);
 }
 
 // --------------------------------------------------------- J/K spatial sums
@@ -1368,6 +1422,61 @@ int pssfss_fill(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt, do
         d2h(ctx, h.data(), ctx->wZ.p, h.size());
         sync(ctx);
         cd *o = (cd *)zmat;
+        for (int64_t m = 0; m < n; ++m)
+            for (int64_t s = 0; s < n; ++s) o[m + n * s] = h[m * n + s];
+        if (info) {
+            memset(info, 0, sizeof(*info));
+            info->mmax = cr.M[0];
+            info->table_converged = cr.conv[0];
+        }
+    });
+}
+
+int pssfss_fill_dk0(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt,
+                    double *zmat, double *dzmat, pssfss_point_info *info) {
+    return guarded(ctx, [&] {
+        if (!sheet || !zmat || !dzmat) throw ArgError("null argument");
+        if (sheet->cls != 'J') throw ArgError("k0 derivative: electric ('J') sheets only");
+        HIPCHECK(hipSetDevice(ctx->device));
+        check_point(sheet, pt, false);
+        hipStream_t st = ctx->st;
+        ChunkResult cr;
+        TableSet tabs;
+        std::vector<FillParams> fp;
+        fill_chunk(ctx, sheet, pt, 1, 0, cr, &tabs, &fp);
+        const int64_t n = sheet->nbf;
+        std::vector<cd> h((size_t)n * n);
+        d2h(ctx, h.data(), ctx->wZ.p, h.size());
+        sync(ctx);
+        cd *o = (cd *)zmat;
+        for (int64_t m = 0; m < n; ++m)
+            for (int64_t s = 0; s < n; ++s) o[m + n * s] = h[m * n + s];
+        std::vector<TabInput> tin = {sheet_tab_input(sheet, pt)};
+        TableSet dt;
+        build_table_dk0(ctx, tin, tabs, dt);
+        FillParams f = fp[0];
+        const int sl = pt->s;
+        const cd e1 = lay(pt->epsr, sl - 1), m1 = lay(pt->mur, sl - 1);
+        const cd e2 = lay(pt->epsr, sl), m2 = lay(pt->mur, sl);
+        f.jk = sheet->jk.p;   // the sheet's cache holds this point's spatial sums now
+        f.dtab1 = dt.tab1(0);
+        f.dtab2 = dt.tab2(0);
+        f.dc3 = c3_dk0(pt->k0, m1, e1, m2, e2);
+        f.dd3 = d3_dk0(pt->k0, m1, e1, m2, e2);
+        DBuf<cd> dZ((size_t)n * n);
+        f.Z = dZ.p;
+        DBuf<FillParams> dfp(1);
+        dfp.upload(&f, 1, st);
+        const int64_t gstride = 9 * sheet->nufp;
+        ctx->wG.ensure((size_t)gstride);
+        launch_fill_classes_dk0(sheet->dev(), dfp.p, sheet->rho_r.p, sheet->rinv.p, ctx->wG.p,
+                                st);
+        // the surface-resistance term does not depend on k0
+        launch_fill_gather(sheet->dev(), dfp.p, 1, sheet->bf_face.p, sheet->bf_loc.p,
+                           sheet->fpcls.p, ctx->wG.p, gstride, 0, 0, st);
+        d2h(ctx, h.data(), dZ.p, h.size());
+        sync(ctx);
+        o = (cd *)dzmat;
         for (int64_t m = 0; m < n; ++m)
             for (int64_t s = 0; s < n; ++s) o[m + n * s] = h[m * n + s];
         if (info) {

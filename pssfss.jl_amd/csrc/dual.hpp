@@ -72,6 +72,8 @@ HD cdd operator/(cdd a, cdd b) {
     return cdd{(a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den};
 }
 HD cdd operator/(cd a, cdd b) { return mkc(a) / b; }
+HD cdd operator/(cdd a, cd b) { return a * (1.0 / b); }
+HD cdd operator+(double a, cdd b) { return cdd{a + b.x, b.y}; }
 HD cdd &operator+=(cdd &a, cdd b) { a = a + b; return a; }
 // exp(i t) of a real dual
 HD cdd dcis(dd t) { return cdd{dcos(t), dsin(t)}; }

@@ -150,6 +150,94 @@ __global__ __launch_bounds__(256) void k_fill_classes(SheetDev sh, const FillPar
     }
 }
 
+// d/dk0 of the electric class terms of k_fill_classes at fixed phasing and u.  k0
+// enters through the two spectral tables (P.dtab1, P.dtab2 hold their derivatives,
+// interpolated with the same weights), c3, d3 and omega = c k0; the spatial sums,
+// the singular integrals and the Floquet flags do not depend on k0.  Planes as G.
+__global__ __launch_bounds__(256) void k_fill_classes_dk0(SheetDev sh, const FillParams *fps,
+                                                          const double *rho_r,
+                                                          const double *rinv_, cd *G) {
+    const int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iu >= sh.nufp) return;
+    const FillParams &P = *fps;
+    const double EPS0 = 1.0 / (MU0 * (C0 * C0));
+    int64_t ifm, ifs;
+    class_faces(sh, iu, ifm, ifs);
+    double rs[3][2], rm[3][2];
+    {
+        const double *a = sh.fxm + 6 * ifs, *b = sh.fxm + 6 * ifm;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            rs[i][0] = a[2 * i]; rs[i][1] = a[2 * i + 1];
+            rm[i][0] = b[2 * i]; rm[i][1] = b[2 * i + 1];
+        }
+    }
+    const double rmcx = ((rm[0][0] + rm[1][0]) + rm[2][0]) / 3;
+    const double rmcy = ((rm[0][1] + rm[1][1]) + rm[2][1]) / 3;
+    cd I1x = {0, 0}, I1e = {0, 0}, I1 = {0, 0}, I2 = {0, 0};
+    cd D1x = {0, 0}, D1e = {0, 0}, D1 = {0, 0}, D2 = {0, 0};
+    for (int i = 0; i < 7; ++i) {
+        double rtx = (rs[0][0] + (rs[1][0] - rs[0][0]) * QXI[i]) + (rs[2][0] - rs[0][0]) * QETA[i];
+        double rty = (rs[0][1] + (rs[1][1] - rs[0][1]) * QXI[i]) + (rs[2][1] - rs[0][1]) * QETA[i];
+        double rdx = rmcx - rtx, rdy = rmcy - rty;
+        const cd s1 = sigma_interp(P.tab1, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy) * QW[i];
+        const cd s2 = sigma_interp(P.tab2, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy);
+        const cd t1 = sigma_interp(P.dtab1, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy) * QW[i];
+        const cd t2 = sigma_interp(P.dtab2, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy);
+        I1x += s1 * QXI[i];
+        I1e += s1 * QETA[i];
+        I1 += s1;
+        I2 += s2 * QW[i];
+        D1x += t1 * QXI[i];
+        D1e += t1 * QETA[i];
+        D1 += t1;
+        D2 += t2 * QW[i];
+    }
+    const int64_t nu = sh.nufp;
+    const cd J = P.jk[iu], Jx = P.jk[nu + iu], Je = P.jk[2 * nu + iu];
+    const cd K = P.jk[3 * nu + iu], Kx = P.jk[4 * nu + iu], Ke = P.jk[5 * nu + iu];
+    const double ri = rinv_[iu];
+    const double rrx = rho_r[2 * iu], rry = rho_r[2 * iu + 1];
+    const cd I1z = (I1 - I1x) - I1e, D1z = (D1 - D1x) - D1e;
+    const cd Jz = (J - Jx) - Je, Kz = (K - Kx) - Ke;
+    double rc2[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        rc2[i][0] = 0.5 * (rmcx - rm[i][0]);
+        rc2[i][1] = 0.5 * (rmcy - rm[i][1]);
+    }
+    const double u = P.u, w = P.omega, dw = C0;
+    const cd phi_core = ((FOURPI * I2) + u * (ri + J)) + (P.d3 / u) * K;
+    const cd dphi_core = (FOURPI * D2) + (P.dd3 / u) * K;
+    // Phi = pf phi_core / omega
+    const cd pf = P.psf_base * (1.0 / (TWOPI * EPS0));
+    const cd dPhi = pf * ((1.0 / w) * dphi_core - (dw / (w * w)) * phi_core);
+    const cd asf = (MU0 / FOURPI) * P.mut;
+    const cd iw = cd{0.0, w}, idw = cd{0.0, dw};
+#pragma unroll
+    for (int isl = 0; isl < 3; ++isl) {
+        const double singx = rrx + ((u * ri) * (rmcx - rs[isl][0]));
+        const double singy = rry + ((u * ri) * (rmcy - rs[isl][1]));
+        const double sing[2] = {singx, singy};
+        cd Ai[2], dAi[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            cd I1i = ((rs[0][d] * I1x + rs[1][d] * I1e) + rs[2][d] * I1z) - rs[isl][d] * I1;
+            cd D1i = ((rs[0][d] * D1x + rs[1][d] * D1e) + rs[2][d] * D1z) - rs[isl][d] * D1;
+            cd Ji = ((rs[0][d] * Jx + rs[1][d] * Je) + rs[2][d] * Jz) - rs[isl][d] * J;
+            cd Ki = ((rs[0][d] * Kx + rs[1][d] * Ke) + rs[2][d] * Kz) - rs[isl][d] * K;
+            Ai[d] = asf * ((((FOURPI * I1i) + sing[d]) + u * Ji) + (P.c3 / u) * Ki);
+            dAi[d] = asf * ((FOURPI * D1i) + (P.dc3 / u) * Ki);
+        }
+#pragma unroll
+        for (int iml = 0; iml < 3; ++iml) {
+            const cd dot = rc2[iml][0] * Ai[0] + rc2[iml][1] * Ai[1];
+            const cd ddot = rc2[iml][0] * dAi[0] + rc2[iml][1] * dAi[1];
+            G[(int64_t)(3 * iml + isl) * nu + iu] = (idw * dot + iw * ddot) - dPhi;
+        }
+    }
+}
+
 // Phase 2: one lane per Z entry (point, row m, column s).  Z[m][s] sums the
 // contributions of the (<= 4) face pairs formed by the two halves of m and the
 // two halves of s (the scatter of src/FillZY.jl:201-240 turned into a gather),
@@ -233,6 +321,13 @@ void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
     else
         hipLaunchKernelGGL(k_fill_classes<false>, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv,
                            G, gstride);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_fill_classes_dk0(const SheetDev &sh, const FillParams *d_fp, const double *rho_r,
+                             const double *rinv, cd *G, hipStream_t st) {
+    dim3 grid((unsigned)((sh.nufp + 255) / 256));
+    hipLaunchKernelGGL(k_fill_classes_dk0, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv, G);
     HIPCHECK(hipGetLastError());
 }
 

@@ -75,6 +75,64 @@ __device__ void summand(const TabParams &c, int m, int n, cd &t1, cd &t2) {
     }
 }
 
+// k0-derivative helpers on complex duals: the value is the library's own routine,
+// the derivative its analytic one (sqrt: dz / 2a; tanh: (1 - t^2) dz).
+__device__ __forceinline__ cdd dmysqrt(cdd z) {
+    const cd a = mysqrt(val(z));
+    const cd d = der(z) / (2.0 * a);
+    return cdd{dd{a.x, d.x}, dd{a.y, d.y}};
+}
+__device__ __forceinline__ cdd dctanh(cdd z) {
+    const cd t = ctanh_(val(z));
+    const cd d = (1.0 - t * t) * der(z);
+    return cdd{dd{t.x, d.x}, dd{t.y, d.y}};
+}
+
+// d/dk0 of the electric summands of eqs. (5.19a,b) at fixed lattice wavevector
+// (fixed phasing) and fixed u, forward mode through the same transmission-line
+// recursion as summand().  k0 enters through k0^2 in every layer's gamma, c3, d3
+// and the small-beta floor b2min (proportional to k0^2).
+__device__ void summand_dk0(const TabParams &c, int m, int n, cd &d1, cd &d2) {
+    const double bx = (c.b00[0] + (double)m * c.b1[0]) + (double)n * c.b2[0];
+    const double by = (c.b00[1] + (double)m * c.b1[1]) + (double)n * c.b2[1];
+    const double k0 = sqrt(c.k0sq);
+    dd b2 = mkd(bx * bx + by * by);
+    if (!(b2.v >= c.b2min)) b2 = dd{c.b2min, 2.0 * c.b2min / k0};
+    const dd k0sq = dd{c.k0sq, 2.0 * k0};
+    const dd kap2 = b2 + c.u * c.u;
+    const dd kap = dsqrt(kap2);
+    const int nl = c.nl, s = c.s;
+    auto gamma = [&](int i) { return dmysqrt(mkc(b2) - (k0sq * c.eps[i]) * c.mu[i]); };
+    cdd g = gamma(0);
+    cdd zlte = c.mu[0] / g, zltm = g / c.eps[0];
+    for (int i = 1; i < s; ++i) {
+        g = gamma(i);
+        const cdd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
+        const cdd th = dctanh(c.width[i] * g);
+        zlte = (z0te * (zlte + z0te * th)) / (z0te + zlte * th);
+        zltm = (z0tm * (zltm + z0tm * th)) / (z0tm + zltm * th);
+    }
+    g = gamma(nl - 1);
+    cdd zrte = c.mu[nl - 1] / g, zrtm = g / c.eps[nl - 1];
+    for (int i = nl - 2; i >= s; --i) {
+        g = gamma(i);
+        const cdd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
+        const cdd th = dctanh(c.width[i] * g);
+        zrte = (z0te * (zrte + z0te * th)) / (z0te + zrte * th);
+        zrtm = (z0tm * (zrtm + z0tm * th)) / (z0tm + zrtm * th);
+    }
+    cdd vte = (zlte * zrte) / (zlte + zrte);
+    cdd vtm = (zltm * zrtm) / (zltm + zrtm);
+    vte = (2.0 * vte) / c.mut;
+    const cdd c3 = cdd{dd{c.c3.x, c.dc3.x}, dd{c.c3.y, c.dc3.y}};
+    const cdd d3 = cdd{dd{c.d3.x, c.dd3.x}, dd{c.d3.y, c.dd3.y}};
+    const cdd t1 = vte - (1.0 + c3 / kap2) / kap;
+    vtm = (2.0 * c.epsb) * vtm;
+    const cdd t2 = (vtm + ((k0sq * c.epsb) * c.mut) * vte) / b2 - (1.0 + d3 / kap2) / kap;
+    d1 = der(t1);
+    d2 = der(t2);
+}
+
 __global__ void k_summands(const TabParams *tp, int64_t cap, int64_t e0, int64_t e1, cd *t1,
                            cd *t2) {
     int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -85,6 +143,20 @@ __global__ void k_summands(const TabParams *tp, int64_t cap, int64_t e0, int64_t
     ring_mn(r, (int)(e - ring_base(r)), m, n);
     cd a, b;
     summand(tp[p], m, n, a, b);
+    t1[p * cap + e] = a;
+    t2[p * cap + e] = b;
+}
+
+__global__ void k_summands_dk0(const TabParams *tp, int64_t cap, int64_t e0, int64_t e1, cd *t1,
+                               cd *t2) {
+    int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    const int p = blockIdx.y;
+    int r = ring_of(e);
+    int m, n;
+    ring_mn(r, (int)(e - ring_base(r)), m, n);
+    cd a, b;
+    summand_dk0(tp[p], m, n, a, b);
     t1[p * cap + e] = a;
     t2[p * cap + e] = b;
 }
@@ -954,6 +1026,17 @@ void launch_summands(const TabParams *d_tp, int npts, int64_t cap, int r0, int r
     int64_t cnt = e1 - e0;
     dim3 grid((unsigned)((cnt + 127) / 128), (unsigned)npts);
     hipLaunchKernelGGL(k_summands, grid, dim3(128), 0, st, d_tp, cap, e0, e1, t1, t2);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_summands_dk0(const TabParams *d_tp, int npts, int64_t cap, int r0, int r1, cd *t1,
+                         cd *t2, hipStream_t st) {
+    int64_t e0 = r0 == 0 ? 0 : (int64_t)(2 * r0 - 1) * (2 * r0 - 1);
+    int64_t e1 = (int64_t)(2 * r1 + 1) * (2 * r1 + 1);
+    if (e1 > cap) throw ArgError("ring capacity exceeded");
+    int64_t cnt = e1 - e0;
+    dim3 grid((unsigned)((cnt + 127) / 128), (unsigned)npts);
+    hipLaunchKernelGGL(k_summands_dk0, grid, dim3(128), 0, st, d_tp, cap, e0, e1, t1, t2);
     HIPCHECK(hipGetLastError());
 }
 

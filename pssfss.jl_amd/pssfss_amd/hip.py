@@ -95,6 +95,7 @@ def lib():
         L.pssfss_face_pair_classes.argtypes = [vp, C.c_int64, _dp, C.c_int64, _ip,
                                                C.POINTER(C.c_int64), _ip, _ip, _ip]
         L.pssfss_fill.argtypes = [vp, vp, C.POINTER(Point), _dp, C.POINTER(PointInfo)]
+        L.pssfss_fill_dk0.argtypes = [vp, vp, C.POINTER(Point), _dp, _dp, C.POINTER(PointInfo)]
         L.pssfss_filljk.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.pssfss_modal_tables.argtypes = [vp, C.c_int, C.POINTER(Point), _dp, _dp, _dp,
                                           C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -107,7 +108,7 @@ def lib():
         L.pssfss_stats_read.argtypes = [vp, C.POINTER(Stats), C.c_int]
         for name in ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_device_info",
                      "pssfss_sheet_upload", "pssfss_sheet_free", "pssfss_sheet_gsm",
-                     "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk",
+                     "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_fill_dk0", "pssfss_filljk",
                      "pssfss_modal_tables", "pssfss_rwgbfft", "pssfss_lu_solve_batch",
                      "pssfss_cascade_batch", "pssfss_stats_enable", "pssfss_stats_read"):
             getattr(L, name).restype = C.c_int
@@ -120,7 +121,7 @@ EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssf
            "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk", "pssfss_modal_tables",
            "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
            "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint",
-           "pssfss_face_pair_classes")
+           "pssfss_face_pair_classes", "pssfss_fill_dk0")
 
 
 def _d(a):
@@ -311,6 +312,17 @@ class DeviceSheet:
         self.ctx.check(lib().pssfss_fill(self.ctx.h, self.h, C.byref(pb.point), _d(Z),
                                          C.byref(info)))
         return Z.view(np.complex128).reshape((nbf, nbf), order="F"), info
+
+    def fill_dk0(self, pb: "PointBuilder"):
+        """(Z, dZ/dk0, info) at the point's phasing and u ('J' sheets); k0 in 1/m."""
+        nbf = self.rwgdat.nbf
+        Z = np.zeros(2 * nbf * nbf)
+        dZ = np.zeros(2 * nbf * nbf)
+        info = PointInfo()
+        self.ctx.check(lib().pssfss_fill_dk0(self.ctx.h, self.h, C.byref(pb.point), _d(Z),
+                                             _d(dZ), C.byref(info)))
+        f = lambda a: a.view(np.complex128).reshape((nbf, nbf), order="F")
+        return f(Z), f(dZ), info
 
 
 class PointBuilder:
