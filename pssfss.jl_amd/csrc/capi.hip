@@ -581,6 +581,10 @@ FillParams make_fill_params(const pssfss_sheet *sh, const pssfss_point *pt) {
     F.ff[2] = hcis(-pt->psi1);
     F.ff[3] = cd{1, 0};
     F.ff[4] = hcis(-pt->psi2);
+    for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+            F.shift9[3 * (a + 1) + (b + 1)] =
+                hcis(-((double)a * pt->psi1 + (double)b * pt->psi2));
     return F;
 }
 

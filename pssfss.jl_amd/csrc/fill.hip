@@ -25,9 +25,11 @@ __constant__ double QW[7] = {0.1125, 0.06296959027241358, 0.06296959027241358,
                              0.06619707639425308};
 
 // 6-point interpolation (AMS-55 25.2.67) with the cell and shift decisions taken on
-// the same IEEE values as the reference (no contraction).
+// the same IEEE values as the reference (no contraction).  The Floquet shift
+// cis(-(msh psi1 + nsh psi2)) for |msh|, |nsh| <= 1 comes from the point's 9-entry
+// table (FillParams::shift9, computed on the host).
 __device__ cd sigma_interp(const cd *tab, int M, const double *b1, const double *b2,
-                           double psi1, double psi2, double rx, double ry) {
+                           double psi1, double psi2, const cd *shift9, double rx, double ry) {
 #pragma clang fp contract(off)
     double x1o = (b1[0] * rx + b1[1] * ry) / TWOPI;
     if (fabs(x1o) < 1e-8) x1o = 0.0;
@@ -54,8 +56,41 @@ __device__ cd sigma_interp(const cd *tab, int M, const double *b1, const double 
     sm = sm + (0.5 * ((q2 - 2 * pq) + q)) * T(m, n + 1);
     sm = sm + pq * T(m + 1, n + 1);
 #undef T
-    if (msh != 0 || nsh != 0) sm = sm * cisd(-(msh * psi1 + nsh * psi2));
+    if (msh != 0 || nsh != 0) {
+        if (fabs(msh) <= 1 && fabs(nsh) <= 1)
+            sm = sm * shift9[(int)(3 * (msh + 1) + (nsh + 1))];
+        else
+            sm = sm * cisd(-(msh * psi1 + nsh * psi2));
+    }
     return sm;
+}
+
+// Thread t -> class.  With every face pair its own class, a wavefront takes an
+// 8 x 8 block of (match, source) faces, so that its interpolation points (match
+// centroids minus source points) cluster in the tables; otherwise thread t takes
+// class t.  Threads needed: fill_class_threads().
+__device__ __forceinline__ bool fill_class_of(const SheetDev &sh, int64_t t, int64_t &iu,
+                                              int64_t &ifm, int64_t &ifs) {
+    if (sh.identity) {
+        const int64_t nf = sh.nface, nb = (nf + 7) >> 3;
+        const int64_t tile = t >> 6, w = t & 63;
+        const int64_t bm = tile / nb, bs = tile - bm * nb;
+        ifm = 8 * bm + (w >> 3);
+        ifs = 8 * bs + (w & 7);
+        if (bm >= nb || ifm >= nf || ifs >= nf) return false;
+        iu = ifm * nf + ifs;
+        return true;
+    }
+    iu = t;
+    if (iu >= sh.nufp) return false;
+    class_faces(sh, iu, ifm, ifs);
+    return true;
+}
+
+int64_t fill_class_threads(const SheetDev &sh) {
+    if (!sh.identity) return sh.nufp;
+    const int64_t nb = (sh.nface + 7) / 8;
+    return nb * nb * 64;
 }
 
 // Phase 1: one lane per (class, point).  The contribution of local edge pair
@@ -69,13 +104,11 @@ __global__ __launch_bounds__(256) void k_fill_classes(SheetDev sh, const FillPar
                                                       const double *rho_r, const double *rinv_,
                                                       cd *G0, int64_t gstride) {
 #pragma clang fp contract(off)
-    const int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (iu >= sh.nufp) return;
     const FillParams &P = fps[blockIdx.y];
+    int64_t iu, ifm, ifs;
+    if (!fill_class_of(sh, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, iu, ifm, ifs)) return;
     cd *G = G0 + (int64_t)blockIdx.y * gstride;
     const double EPS0 = 1.0 / (MU0 * (C0 * C0));
-    int64_t ifm, ifs;
-    class_faces(sh, iu, ifm, ifs);
     double rs[3][2], rm[3][2];
     {
         const double *a = sh.fxm + 6 * ifs, *b = sh.fxm + 6 * ifm;
@@ -92,8 +125,8 @@ __global__ __launch_bounds__(256) void k_fill_classes(SheetDev sh, const FillPar
         double rtx = (rs[0][0] + (rs[1][0] - rs[0][0]) * QXI[i]) + (rs[2][0] - rs[0][0]) * QETA[i];
         double rty = (rs[0][1] + (rs[1][1] - rs[0][1]) * QXI[i]) + (rs[2][1] - rs[0][1]) * QETA[i];
         double rdx = rmcx - rtx, rdy = rmcy - rty;
-        cd s1 = sigma_interp(P.tab1, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy);
-        cd s2 = sigma_interp(P.tab2, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy);
+        cd s1 = sigma_interp(P.tab1, P.M, P.b1, P.b2, P.psi1, P.psi2, P.shift9, rdx, rdy);
+        cd s2 = sigma_interp(P.tab2, P.M, P.b1, P.b2, P.psi1, P.psi2, P.shift9, rdx, rdy);
         cd sw = s1 * QW[i];
         I1x += sw * QXI[i];
         I1e += sw * QETA[i];
@@ -158,8 +191,8 @@ __global__ __launch_bounds__(256) void k_fill_classes_dk0(SheetDev sh, const Fil
                                                           const double *rho_r,
                                                           const double *rinv_, cd *G) {
     const int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (iu >= sh.nufp) return;
     const FillParams &P = *fps;
+    if (iu >= sh.nufp) return;
     const double EPS0 = 1.0 / (MU0 * (C0 * C0));
     int64_t ifm, ifs;
     class_faces(sh, iu, ifm, ifs);
@@ -180,10 +213,12 @@ __global__ __launch_bounds__(256) void k_fill_classes_dk0(SheetDev sh, const Fil
         double rtx = (rs[0][0] + (rs[1][0] - rs[0][0]) * QXI[i]) + (rs[2][0] - rs[0][0]) * QETA[i];
         double rty = (rs[0][1] + (rs[1][1] - rs[0][1]) * QXI[i]) + (rs[2][1] - rs[0][1]) * QETA[i];
         double rdx = rmcx - rtx, rdy = rmcy - rty;
-        const cd s1 = sigma_interp(P.tab1, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy) * QW[i];
-        const cd s2 = sigma_interp(P.tab2, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy);
-        const cd t1 = sigma_interp(P.dtab1, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy) * QW[i];
-        const cd t2 = sigma_interp(P.dtab2, P.M, P.b1, P.b2, P.psi1, P.psi2, rdx, rdy);
+        const cd s1 =
+            sigma_interp(P.tab1, P.M, P.b1, P.b2, P.psi1, P.psi2, P.shift9, rdx, rdy) * QW[i];
+        const cd s2 = sigma_interp(P.tab2, P.M, P.b1, P.b2, P.psi1, P.psi2, P.shift9, rdx, rdy);
+        const cd t1 =
+            sigma_interp(P.dtab1, P.M, P.b1, P.b2, P.psi1, P.psi2, P.shift9, rdx, rdy) * QW[i];
+        const cd t2 = sigma_interp(P.dtab2, P.M, P.b1, P.b2, P.psi1, P.psi2, P.shift9, rdx, rdy);
         I1x += s1 * QXI[i];
         I1e += s1 * QETA[i];
         I1 += s1;
@@ -314,7 +349,7 @@ void launch_zero(cd *p, int64_t n, hipStream_t st) {
 void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
                          const double *rho_r, const double *rinv, int magnetic, cd *G,
                          int64_t gstride, hipStream_t st) {
-    dim3 grid((unsigned)((sh.nufp + 255) / 256), (unsigned)npts);
+    dim3 grid((unsigned)((fill_class_threads(sh) + 255) / 256), (unsigned)npts);
     if (magnetic)
         hipLaunchKernelGGL(k_fill_classes<true>, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv,
                            G, gstride);

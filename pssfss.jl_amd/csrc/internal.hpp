@@ -115,6 +115,7 @@ struct FillParams {
     cd I1fact, KFfact, I2fact, KPfact;
     cd psf_base;                // electric: im/epsb;  magnetic: 2im/(pi*omega*mu0*mut)
     cd ff[5];
+    cd shift9[9];               // cis(-(a psi1 + b psi2)), entry 3 (a + 1) + (b + 1), |a|, |b| <= 1
     int M, pad;
     const cd *tab1, *tab2;      // (M+2)^2 each
     const cd *jk;               // 6*nufp: J, Jxi, Jeta, K, Kxi, Keta (SoA)
