@@ -190,12 +190,10 @@ __global__ __launch_bounds__(256) void k_fill_classes(SheetDev sh, const FillPar
 __global__ __launch_bounds__(256) void k_fill_classes_dk0(SheetDev sh, const FillParams *fps,
                                                           const double *rho_r,
                                                           const double *rinv_, cd *G) {
-    const int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t iu, ifm, ifs;
+    if (!fill_class_of(sh, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, iu, ifm, ifs)) return;
     const FillParams &P = *fps;
-    if (iu >= sh.nufp) return;
     const double EPS0 = 1.0 / (MU0 * (C0 * C0));
-    int64_t ifm, ifs;
-    class_faces(sh, iu, ifm, ifs);
     double rs[3][2], rm[3][2];
     {
         const double *a = sh.fxm + 6 * ifs, *b = sh.fxm + 6 * ifm;
@@ -361,7 +359,7 @@ void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
 
 void launch_fill_classes_dk0(const SheetDev &sh, const FillParams *d_fp, const double *rho_r,
                              const double *rinv, cd *G, hipStream_t st) {
-    dim3 grid((unsigned)((sh.nufp + 255) / 256));
+    dim3 grid((unsigned)((fill_class_threads(sh) + 255) / 256));
     hipLaunchKernelGGL(k_fill_classes_dk0, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv, G);
     HIPCHECK(hipGetLastError());
 }

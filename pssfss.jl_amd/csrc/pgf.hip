@@ -740,7 +740,7 @@ __global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, La
 // power tables in LDS.  Tiles are double-buffered, one barrier per chunk; their
 // rows are padded so the operand reads are free of bank conflicts.
 template <int WH>
-__global__ __launch_bounds__(256 * WH) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
+__global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
                                                       const LatPt *lat, const float *phase,
                                                       int ldp, int nf, cd *jk, int64_t jk_stride,
                                                       unsigned long long *bcnt) {
@@ -823,6 +823,16 @@ __global__ __launch_bounds__(256 * WH) void k_jk_mfma(SheetDev sh, int64_t nufp,
     const unsigned live0 = ~done & 0xffu;
     unsigned long long npts = 0;     // lattice points of the reference's sums
     int rlast = 0;
+    // lattice offsets (terms) and indices (phases) of the next chunk, loaded one
+    // chunk ahead
+    float2 nxt_t[2 / WH];
+    int2 nxt_q;
+#pragma unroll
+    for (int h = 0; h < 2 / WH; ++h) {
+        const LatPt &L = lat[ring_base(1) + wave + 4 * WH * h];
+        nxt_t[h] = float2{L.ox, L.oy};
+    }
+    nxt_q = int2{lat[ring_base(1) + pkk].m, lat[ring_base(1) + pkk].n};
     for (int r = 1; r <= JK_RINGMAX; ++r) {
         if (!__syncthreads_or(done != 0xffu)) break;
         rlast = r;
@@ -836,11 +846,26 @@ __global__ __launch_bounds__(256 * WH) void k_jk_mfma(SheetDev sh, int64_t nufp,
             }
         const int64_t rb = ring_base(r);
         for (int c = 0; c < r; ++c) {
+            float2 cur_t[2 / WH];
+#pragma unroll
+            for (int h = 0; h < 2 / WH; ++h) cur_t[h] = nxt_t[h];
+            const int2 cur_q = nxt_q;
+            {
+                const int64_t nb = c + 1 < r ? rb + 8 * (c + 1)
+                                             : (r < JK_RINGMAX ? ring_base(r + 1) : -1);
+                if (nb >= 0) {
+#pragma unroll
+                    for (int h = 0; h < 2 / WH; ++h) {
+                        const LatPt &L = lat[nb + wave + 4 * WH * h];
+                        nxt_t[h] = float2{L.ox, L.oy};
+                    }
+                    nxt_q = int2{lat[nb + pkk].m, lat[nb + pkk].n};
+                }
+            }
 #pragma unroll
             for (int h = 0; h < 2 / WH; ++h) {
                 const int kk = wave + 4 * WH * h;
-                const LatPt L = lat[rb + 8 * c + kk];
-                const float dx = urx - L.ox, dy = ury - L.oy;
+                const float dx = urx - cur_t[h].x, dy = ury - cur_t[h].y;
                 const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
                 const float ex = __expf(-a);
                 TL.Aj[buf][kk][lane] = __fdividef(ex, a);
@@ -848,9 +873,8 @@ __global__ __launch_bounds__(256 * WH) void k_jk_mfma(SheetDev sh, int64_t nufp,
             }
             float br, bi;
             if (r <= JK_RT) {
-                const LatPt L = lat[rb + 8 * c + pkk];
-                const float c1 = P1[0][pf][L.m + JK_RT], s1 = P1[1][pf][L.m + JK_RT];
-                const float c2 = P2[0][pf][L.n + JK_RT], s2 = P2[1][pf][L.n + JK_RT];
+                const float c1 = P1[0][pf][cur_q.x + JK_RT], s1 = P1[1][pf][cur_q.x + JK_RT];
+                const float c2 = P2[0][pf][cur_q.y + JK_RT], s2 = P2[1][pf][cur_q.y + JK_RT];
                 br = c1 * c2 - s1 * s2;
                 bi = c1 * s2 + s1 * c2;
             } else {
