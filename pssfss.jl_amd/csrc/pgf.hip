@@ -16,6 +16,12 @@ namespace {
 constexpr double TWOPI = 6.283185307179586;
 constexpr double PI_ = 3.141592653589793;
 
+// x / y for the FP32 lattice terms: the hardware reciprocal (1 ulp) and a product
+// (a correctly rounded FP32 division costs about ten instructions more).
+__device__ __forceinline__ float fdiv_rcp(float x, float y) {
+    return x * __builtin_amdgcn_rcpf(y);
+}
+
 // Summands of eqs. (5.19a,b) (electric) or (5.26a,b) (magnetic) for lattice index
 // (m, n): transmission-line impedances stepped through the layers from both ends.
 __device__ void summand(const TabParams &c, int m, int n, cd &t1, cd &t2) {
@@ -374,7 +380,7 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                     float fx = (float)dx, fy = (float)dy;
                     float a = __fsqrt_rn(fx * fx + fy * fy);
                     float e = __expf(-a);
-                    float ea = __fdividef(e, a);
+                    float ea = fdiv_rcp(e, a);
                     if (r <= RT) {
                         const int off = (m + RT) * W + (n + RT);
 #pragma unroll
@@ -599,7 +605,7 @@ __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, dou
                     float ef = __expf(-af);
                     a = af;
                     e = ef;
-                    ea = __fdividef(ef, af);
+                    ea = fdiv_rcp(ef, af);
                 } else {
                     a = sqrt(dx * dx + dy * dy);
                     e = exp(-(double)a);
@@ -819,23 +825,58 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
     }
     const float *pcol = phase + 2 * (f0 + pf);
     const int64_t pstride = 2 * (int64_t)ldp;
-    int buf = 0;
     const unsigned live0 = ~done & 0xffu;
     unsigned long long npts = 0;     // lattice points of the reference's sums
     int rlast = 0;
-    // lattice offsets (terms) and indices (phases) of the next chunk, loaded one
-    // chunk ahead
-    float2 nxt_t[2 / WH];
-    int2 nxt_q;
+    // Chunk k holds lattice points 1 + 8 k .. 8 k + 8 (ring r is chunks
+    // r (r - 1) / 2 .. r (r + 1) / 2 - 1).  The tiles of chunk k + 1 are formed
+    // while the MFMAs of chunk k run, so that the term arithmetic and the matrix
+    // products overlap; lattice data is loaded one chunk earlier still.
+    constexpr int NCH = JK_RINGMAX * (JK_RINGMAX + 1) / 2;
+    float2 lt[2 / WH];               // term offsets of the next chunk to be formed
+    int2 lq;                         // phase indices of that chunk
+    auto load_lat = [&](int k) {
 #pragma unroll
-    for (int h = 0; h < 2 / WH; ++h) {
-        const LatPt &L = lat[ring_base(1) + wave + 4 * WH * h];
-        nxt_t[h] = float2{L.ox, L.oy};
-    }
-    nxt_q = int2{lat[ring_base(1) + pkk].m, lat[ring_base(1) + pkk].n};
-    for (int r = 1; r <= JK_RINGMAX; ++r) {
-        if (!__syncthreads_or(done != 0xffu)) break;
-        rlast = r;
+        for (int h = 0; h < 2 / WH; ++h) {
+            const LatPt &L = lat[1 + 8 * (int64_t)k + wave + 4 * WH * h];
+            lt[h] = float2{L.ox, L.oy};
+        }
+        const LatPt &Q = lat[1 + 8 * (int64_t)k + pkk];
+        lq = int2{Q.m, Q.n};
+    };
+    // the phase tile of chunk k (ring rk) into buffer b, from lq
+    auto form_phase = [&](int k, int rk, int b) {
+        float br, bi;
+        if (rk <= JK_RT) {
+            const float c1 = P1[0][pf][lq.x + JK_RT], s1 = P1[1][pf][lq.x + JK_RT];
+            const float c2 = P2[0][pf][lq.y + JK_RT], s2 = P2[1][pf][lq.y + JK_RT];
+            br = c1 * c2 - s1 * s2;
+            bi = c1 * s2 + s1 * c2;
+        } else {
+            const int64_t e = 1 + 8 * (int64_t)k + pkk;
+            br = pcol[e * pstride];
+            bi = pcol[e * pstride + 1];
+        }
+        TL.Bre[b][pkk][pf] = br;
+        TL.Bim[b][pkk][pf] = bi;
+    };
+    // term h of this thread for the chunk in lt, into buffer b
+    auto form_term = [&](int b, int h) {
+        const int kk = wave + 4 * WH * h;
+        const float dx = urx - lt[h].x, dy = ury - lt[h].y;
+        const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+        const float ex = __expf(-a);
+        TL.Aj[b][kk][lane] = fdiv_rcp(ex, a);
+        TL.Ak[b][kk][lane] = ex;
+    };
+    static_assert(WH == 1, "the interleaved chunk loop forms two terms per thread");
+    if (__syncthreads_or(done != 0xffu)) {
+        load_lat(0);
+        form_phase(0, 1, 0);
+        form_term(0, 0);
+        form_term(0, 1);
+        load_lat(1);
+        __syncthreads();
         V aJ[2][2], aK[2][2];        // [g][cos/sin]: this ring's sums
 #pragma unroll
         for (int g = 0; g < 2; ++g)
@@ -844,87 +885,85 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
                 aJ[g][c] = V{0, 0, 0, 0};
                 aK[g][c] = V{0, 0, 0, 0};
             }
-        const int64_t rb = ring_base(r);
-        for (int c = 0; c < r; ++c) {
-            float2 cur_t[2 / WH];
-#pragma unroll
-            for (int h = 0; h < 2 / WH; ++h) cur_t[h] = nxt_t[h];
-            const int2 cur_q = nxt_q;
-            {
-                const int64_t nb = c + 1 < r ? rb + 8 * (c + 1)
-                                             : (r < JK_RINGMAX ? ring_base(r + 1) : -1);
-                if (nb >= 0) {
-#pragma unroll
-                    for (int h = 0; h < 2 / WH; ++h) {
-                        const LatPt &L = lat[nb + wave + 4 * WH * h];
-                        nxt_t[h] = float2{L.ox, L.oy};
-                    }
-                    nxt_q = int2{lat[nb + pkk].m, lat[nb + pkk].n};
-                }
-            }
-#pragma unroll
-            for (int h = 0; h < 2 / WH; ++h) {
-                const int kk = wave + 4 * WH * h;
-                const float dx = urx - cur_t[h].x, dy = ury - cur_t[h].y;
-                const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
-                const float ex = __expf(-a);
-                TL.Aj[buf][kk][lane] = __fdividef(ex, a);
-                TL.Ak[buf][kk][lane] = ex;
-            }
-            float br, bi;
-            if (r <= JK_RT) {
-                const float c1 = P1[0][pf][cur_q.x + JK_RT], s1 = P1[1][pf][cur_q.x + JK_RT];
-                const float c2 = P2[0][pf][cur_q.y + JK_RT], s2 = P2[1][pf][cur_q.y + JK_RT];
-                br = c1 * c2 - s1 * s2;
-                bi = c1 * s2 + s1 * c2;
-            } else {
-                const int64_t e = rb + 8 * c + pkk;
-                br = pcol[e * pstride];
-                bi = pcol[e * pstride + 1];
-            }
-            TL.Bre[buf][pkk][pf] = br;
-            TL.Bim[buf][pkk][pf] = bi;
-            __syncthreads();
+        int r = 1, kend = 1;         // ring r ends with chunk kend - 1
+        for (int k = 0; k < NCH; ++k) {
+            const int b = k & 1;
+            // chunk whose tiles are formed during this one (the last chunk's tiles
+            // are formed again at the very end, into a buffer nobody reads)
+            const int kn = k + 1 < NCH ? k + 1 : NCH - 1;
+            form_phase(kn, k + 1 < kend ? r : r + 1, b ^ 1);
+            float a_j[2], a_k[2], bre[2][2], bim[2][2];
 #pragma unroll
             for (int k4 = 0; k4 < 2; ++k4) {
                 const int kr = 4 * k4 + (lane >> 4);
-                const float a_j = TL.Aj[buf][kr][16 * wr + pc];
-                const float a_k = TL.Ak[buf][kr][16 * wr + pc];
+                a_j[k4] = TL.Aj[b][kr][16 * wr + pc];
+                a_k[k4] = TL.Ak[b][kr][16 * wr + pc];
 #pragma unroll
                 for (int g = 0; g < 2; ++g) {
-                    const float bre = TL.Bre[buf][kr][32 * wh + 16 * g + pc];
-                    const float bim = TL.Bim[buf][kr][32 * wh + 16 * g + pc];
-                    aJ[g][0] = MT::mma(a_j, bre, aJ[g][0]);
-                    aJ[g][1] = MT::mma(a_j, bim, aJ[g][1]);
-                    aK[g][0] = MT::mma(a_k, bre, aK[g][0]);
-                    aK[g][1] = MT::mma(a_k, bim, aK[g][1]);
+                    bre[k4][g] = TL.Bre[b][kr][32 * wh + 16 * g + pc];
+                    bim[k4][g] = TL.Bim[b][kr][32 * wh + 16 * g + pc];
                 }
             }
-            buf ^= 1;
-        }
+            // the term tile of chunk kn and the MFMAs of chunk k, interleaved
+            // (one MFMA, then two VALU instructions) in two halves whose eight
+            // MFMAs each use distinct accumulators
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+            for (int k4 = 0; k4 < 2; ++k4) {
+                form_term(b ^ 1, k4);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const unsigned bit = 1u << (4 * g + e);
-                if (!(done & bit)) {
-                    const cd a = cd{(double)aJ[g][0][e], (double)aJ[g][1][e]};
-                    const cd b = cd{(double)aK[g][0][e], (double)aK[g][1][e]};
-                    js[g][e] += a;
-                    ks[g][e] += b;
-                    if (r > 5) {
-                        // |ring/sum| < 1e-8 for J and K, as squared magnitudes
-                        const double aj = a.x * a.x + a.y * a.y;
-                        const double sj = js[g][e].x * js[g][e].x + js[g][e].y * js[g][e].y;
-                        const double ak = b.x * b.x + b.y * b.y;
-                        const double sk = ks[g][e].x * ks[g][e].x + ks[g][e].y * ks[g][e].y;
-                        if (aj < 1e-16 * sj && ak < 1e-16 * sk) {
-                            done |= bit;
-                            npts += lattice_points(r);
+                for (int g = 0; g < 2; ++g) {
+                    aJ[g][0] = MT::mma(a_j[k4], bre[k4][g], aJ[g][0]);
+                    aJ[g][1] = MT::mma(a_j[k4], bim[k4][g], aJ[g][1]);
+                    aK[g][0] = MT::mma(a_k[k4], bre[k4][g], aK[g][0]);
+                    aK[g][1] = MT::mma(a_k[k4], bim[k4][g], aK[g][1]);
+                }
+                if (k4 == 1) load_lat(kn + 1 < NCH ? kn + 1 : NCH - 1);
+                if (k4 == 0) __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            __syncthreads();
+            if (k + 1 < kend) continue;
+            // ring r complete: fold its sums into the FP64 running sums
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const unsigned bit = 1u << (4 * g + e);
+                    if (!(done & bit)) {
+                        const cd a = cd{(double)aJ[g][0][e], (double)aJ[g][1][e]};
+                        const cd bb = cd{(double)aK[g][0][e], (double)aK[g][1][e]};
+                        js[g][e] += a;
+                        ks[g][e] += bb;
+                        if (r > 5) {
+                            // |ring/sum| < 1e-8 for J and K, as squared magnitudes
+                            const double aj = a.x * a.x + a.y * a.y;
+                            const double sj = js[g][e].x * js[g][e].x + js[g][e].y * js[g][e].y;
+                            const double ak = bb.x * bb.x + bb.y * bb.y;
+                            const double sk = ks[g][e].x * ks[g][e].x + ks[g][e].y * ks[g][e].y;
+                            if (aj < 1e-16 * sj && ak < 1e-16 * sk) {
+                                done |= bit;
+                                npts += lattice_points(r);
+                            }
                         }
                     }
                 }
-            }
+            rlast = r;
+            if (!__syncthreads_or(done != 0xffu)) break;
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    aJ[g][c] = V{0, 0, 0, 0};
+                    aK[g][c] = V{0, 0, 0, 0};
+                }
+            ++r;
+            kend += r;
+        }
     }
     if (bcnt) {
         npts += (unsigned long long)__popc(live0 & ~done) * lattice_points(rlast);
