@@ -142,7 +142,8 @@ def main():
     # lattice point) that the reference's ring rule sums, counted on the device.
     st = stages[head]
     jk = st["spatial_sums"]
-    jk_kernel = ("k_jk_mfma<1> (spatial sums, FP32 MFMA)" if head == "mixed"
+    jk_kernel = ("k_jk_mfma<1,2> + k_jk_mfma<1,1> (spatial sums, FP32 MFMA; the second "
+                 "launch is the last phasing block at half width)" if head == "mixed"
                  else "k_jk_tiled<double> (spatial sums, FP64 VALU)")
     achieved = jk["flops"] / (jk["ms"] * 1e-3) / 1e12 if jk["ms"] > 0 else 0.0
     roof = {"bound": "mfma", "kernel": jk_kernel, "achieved": round(achieved, 3),

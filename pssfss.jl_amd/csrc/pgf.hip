@@ -744,12 +744,13 @@ __global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, La
 // end and applies the break test of src/PGF.jl:86-103.  Lattice offsets come from
 // an FP32 table (k_lattice); phases of rings <= JK_RT are products of per-phasing
 // power tables in LDS.  Tiles are double-buffered, one barrier per chunk; their
-// rows are padded so the operand reads are free of bank conflicts.
-template <int WH>
-__global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) void k_jk_mfma(SheetDev sh, int64_t nufp, double ul,
-                                                      const LatPt *lat, const float *phase,
-                                                      int ldp, int nf, cd *jk, int64_t jk_stride,
-                                                      unsigned long long *bcnt) {
+// rows are padded so the operand reads are free of bank conflicts.  Phasing
+// blocks start at block fblk0; NG = 1 serves a last block with at most 16 live
+// phasings, whose second group of 16 would be all padding, with half the MFMAs.
+template <int WH, int NG>
+__global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) void k_jk_mfma(
+    SheetDev sh, int64_t nufp, double ul, const LatPt *lat, const float *phase, int ldp, int nf,
+    int fblk0, cd *jk, int64_t jk_stride, unsigned long long *bcnt) {
     typedef MfmaT<float> MT;
     typedef MT::V V;
     constexpr int NT = 256 * WH, FT = 32 * WH, AP = 80, BP = 16 + FT;
@@ -768,7 +769,7 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave & 3, wh = wave >> 2;
     const int64_t u0 = (int64_t)blockIdx.x * 9;
-    const int f0 = blockIdx.y * FT;
+    const int f0 = (fblk0 + (int)blockIdx.y) * FT;
     if (tid < 64) {
         const int r = tid;
         const int64_t iu = u0 + r / 7;
@@ -911,7 +912,7 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
             for (int k4 = 0; k4 < 2; ++k4) {
                 form_term(b ^ 1, k4);
 #pragma unroll
-                for (int g = 0; g < 2; ++g) {
+                for (int g = 0; g < NG; ++g) {
                     aJ[g][0] = MT::mma(a_j[k4], bre[k4][g], aJ[g][0]);
                     aJ[g][1] = MT::mma(a_j[k4], bim[k4][g], aJ[g][1]);
                     aK[g][0] = MT::mma(a_k[k4], bre[k4][g], aK[g][0]);
@@ -920,7 +921,7 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
                 if (k4 == 1) load_lat(kn + 1 < NCH ? kn + 1 : NCH - 1);
                 if (k4 == 0) __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
+                for (int i = 0; i < 4 * NG; ++i) {
                     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
                     __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
                 }
@@ -930,7 +931,7 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
             if (k + 1 < kend) continue;
             // ring r complete: fold its sums into the FP64 running sums
 #pragma unroll
-            for (int g = 0; g < 2; ++g)
+            for (int g = 0; g < NG; ++g)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const unsigned bit = 1u << (4 * g + e);
@@ -1192,8 +1193,21 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
         hipLaunchKernelGGL(k_phase_table<float>, pg, dim3(256), 0, st, d_psi, nf, ldp, (float *)ph);
         hipLaunchKernelGGL(k_lattice, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, us1[0],
                            us1[1], us2[0], us2[1], lat);
-        hipLaunchKernelGGL(k_jk_mfma<JK_MFMA_WH>, grid, dim3(256 * JK_MFMA_WH), 0, st, sh, nufp,
-                           ulocal, lat, (const float *)ph, ldp, nf, jk, jk_stride, bcnt);
+        // a last block with at most 16 live phasings runs the half-width kernel
+        const int nblk_f = (int)grid.y;
+        const bool half_tail = JK_MFMA_WH == 1 && nf - (nblk_f - 1) * FT <= 16;
+        dim3 gmain(grid.x, (unsigned)(half_tail ? nblk_f - 1 : nblk_f));
+        if (gmain.y > 0)
+            hipLaunchKernelGGL((k_jk_mfma<JK_MFMA_WH, 2>), gmain, dim3(256 * JK_MFMA_WH), 0, st,
+                               sh, nufp, ulocal, lat, (const float *)ph, ldp, nf, 0, jk,
+                               jk_stride, bcnt);
+        if (half_tail) {
+            HIPCHECK(hipGetLastError());
+            hipLaunchKernelGGL((k_jk_mfma<JK_MFMA_WH, 1>), dim3(grid.x, 1), dim3(256 * JK_MFMA_WH),
+                               0, st, sh, nufp, ulocal, lat, (const float *)ph, ldp, nf,
+                               nblk_f - 1, jk, jk_stride,
+                               bcnt ? bcnt + (int64_t)(nblk_f - 1) * grid.x : nullptr);
+        }
     } else {
         hipLaunchKernelGGL(k_phase_table<double>, pg, dim3(256), 0, st, d_psi, nf, ldp,
                            (double *)ph);
