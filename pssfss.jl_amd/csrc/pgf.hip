@@ -740,8 +740,9 @@ __global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, La
 // tile; the term tile is the A operand and the phase tile (cos and sin columns)
 // the B operand of 16x16x4 FP32 MFMAs.  Wavefront (wr, wh) accumulates
 // rows 16 wr..16 wr + 15 against phasings 32 wh..32 wh + 31; each lane folds the
-// ring sums of its 8 (row, phasing) pairs into FP64 running sums at every ring
-// end and applies the break test of src/PGF.jl:86-103.  Lattice offsets come from
+// ring sums of its 8 (row, phasing) pairs into FP64 running sums at the end of
+// ring 5 (rings 1..5 together, summed in the FP32 accumulators) and of every later
+// ring, and applies the break test of src/PGF.jl:86-103.  Lattice offsets come from
 // an FP32 table (k_lattice); phases of rings <= JK_RT are products of per-phasing
 // power tables in LDS.  Tiles are double-buffered, one barrier per chunk; their
 // rows are padded so the operand reads are free of bank conflicts.  Phasing
@@ -929,39 +930,43 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
             }
             __syncthreads();
             if (k + 1 < kend) continue;
-            // ring r complete: fold its sums into the FP64 running sums
+            // ring r complete.  Rings 1..4 stay in the FP32 accumulators and are
+            // folded with ring 5 (the break test starts after ring 5); from ring 5
+            // on, every ring is folded into the FP64 running sums and tested.
+            if (r >= 5) {
 #pragma unroll
-            for (int g = 0; g < NG; ++g)
+                for (int g = 0; g < NG; ++g)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const unsigned bit = 1u << (4 * g + e);
-                    if (!(done & bit)) {
-                        const cd a = cd{(double)aJ[g][0][e], (double)aJ[g][1][e]};
-                        const cd bb = cd{(double)aK[g][0][e], (double)aK[g][1][e]};
-                        js[g][e] += a;
-                        ks[g][e] += bb;
-                        if (r > 5) {
-                            // |ring/sum| < 1e-8 for J and K, as squared magnitudes
-                            const double aj = a.x * a.x + a.y * a.y;
-                            const double sj = js[g][e].x * js[g][e].x + js[g][e].y * js[g][e].y;
-                            const double ak = bb.x * bb.x + bb.y * bb.y;
-                            const double sk = ks[g][e].x * ks[g][e].x + ks[g][e].y * ks[g][e].y;
-                            if (aj < 1e-16 * sj && ak < 1e-16 * sk) {
-                                done |= bit;
-                                npts += lattice_points(r);
+                    for (int e = 0; e < 4; ++e) {
+                        const unsigned bit = 1u << (4 * g + e);
+                        if (!(done & bit)) {
+                            const cd a = cd{(double)aJ[g][0][e], (double)aJ[g][1][e]};
+                            const cd bb = cd{(double)aK[g][0][e], (double)aK[g][1][e]};
+                            js[g][e] += a;
+                            ks[g][e] += bb;
+                            if (r > 5) {
+                                // |ring/sum| < 1e-8 for J and K, as squared magnitudes
+                                const double aj = a.x * a.x + a.y * a.y;
+                                const double sj = js[g][e].x * js[g][e].x + js[g][e].y * js[g][e].y;
+                                const double ak = bb.x * bb.x + bb.y * bb.y;
+                                const double sk = ks[g][e].x * ks[g][e].x + ks[g][e].y * ks[g][e].y;
+                                if (aj < 1e-16 * sj && ak < 1e-16 * sk) {
+                                    done |= bit;
+                                    npts += lattice_points(r);
+                                }
                             }
                         }
                     }
-                }
-            rlast = r;
-            if (!__syncthreads_or(done != 0xffu)) break;
+                rlast = r;
+                if (!__syncthreads_or(done != 0xffu)) break;
 #pragma unroll
-            for (int g = 0; g < 2; ++g)
+                for (int g = 0; g < 2; ++g)
 #pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    aJ[g][c] = V{0, 0, 0, 0};
-                    aK[g][c] = V{0, 0, 0, 0};
-                }
+                    for (int c = 0; c < 2; ++c) {
+                        aJ[g][c] = V{0, 0, 0, 0};
+                        aK[g][c] = V{0, 0, 0, 0};
+                    }
+            }
             ++r;
             kend += r;
         }
