@@ -624,6 +624,9 @@ __global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, dou
             __syncthreads();
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) {
+                // fused multiply-adds: the sums' order already differs from the
+                // reference's sequential one, so per-step rounding is not matched
+#pragma clang fp contract(fast)
                 T aj[4], ak[4], pr[TN], pi[TN];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
