@@ -144,7 +144,7 @@ def main():
     jk = st["spatial_sums"]
     jk_kernel = ("k_jk_mfma<1,2> + k_jk_mfma<1,1> (spatial sums, FP32 MFMA; the second "
                  "launch is the last phasing block at half width)" if head == "mixed"
-                 else "k_jk_tiled<double> (spatial sums, FP64 VALU)")
+                 else "k_jk_mfma64<2> + k_jk_mfma64<1> (spatial sums, FP64 MFMA)")
     achieved = jk["flops"] / (jk["ms"] * 1e-3) / 1e12 if jk["ms"] > 0 else 0.0
     roof = {"bound": "mfma", "kernel": jk_kernel, "achieved": round(achieved, 3),
             "peak": PEAK[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK[head], 4),

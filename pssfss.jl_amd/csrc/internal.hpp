@@ -63,7 +63,7 @@ struct DBuf {
 constexpr int MAXL = 16;        // layers per Gblock region
 constexpr int JK_RT = 10;       // rings whose phases are tabulated in LDS
 constexpr int JK_RINGMAX = 65;  // src/PGF.jl:15
-constexpr int JK_TILED_MIN = 24; // phasings from which launch_jk uses the tiled kernel
+constexpr int JK_MFMA_MIN = 24;  // phasings from which launch_jk uses the MFMA kernels
 
 // Geometry of a sheet on the device (structure of arrays, 0-based indices).
 struct SheetDev {

@@ -508,266 +508,74 @@ __global__ void k_sum_u64(const unsigned long long *a, int64_t n, unsigned long 
     if ((threadIdx.x & 63) == 0) atomicAdd(total, v);
 }
 
-// Tiled form of the spatial sums for many phasings: a workgroup takes 64 rows
-// (9 classes x 7 quadrature points) and 16*TN phasings.  Lattice points are
-// visited ring by ring in chunks of 8 (ring r has 8r points), the lattice term
-// tile exp(-d)/d, exp(-d) (64 rows x 8 points) is computed once into LDS and
-// multiplies the phase tile (8 points x 16*TN phasings) read from a table, so
-// the distance/exp/reciprocal work is shared by every phasing of the tile.
-// Ring sums are folded and the break test of jksums applied per (row, phasing)
-// at each ring end, exactly as src/PGF.jl:86-103 does per call.
-// T = double: FP64 terms and ring sums; T = float: FP32 terms and ring sums
-// (FP64 running sums and extracted (0,0) term).
-template <class T, int TN>
-__global__ __launch_bounds__(256) void k_jk_tiled(SheetDev sh, int64_t nufp, double ul,
-                                                  double us1x, double us1y, double us2x,
-                                                  double us2y, const T *phase, int ldp, int nf,
-                                                  cd *jk, int64_t jk_stride,
-                                                  unsigned long long *bcnt) {
-#pragma clang fp contract(off)
-    constexpr int FT = 16 * TN;              // phasings per workgroup
-    __shared__ T Aj[8][64], Ak[8][64];
-    __shared__ T Bre[8][FT], Bim[8][FT];
-    __shared__ double urxs[64], urys[64];
-    __shared__ cd Cs[64][FT][2];
-    const int tid = threadIdx.x, rg = tid >> 4, pg = tid & 15;
-    const int64_t u0 = (int64_t)blockIdx.x * 9;
-    const int f0 = blockIdx.y * FT;
-    if (tid < 64) {
-        const int r = tid;
-        const int64_t iu = u0 + r / 7;
-        double urx = 0.0, ury = 0.0;
-        if (r < 63 && iu < nufp) {
-            const int qp = r % 7;
-            int64_t ifm, ifs;
-            class_faces(sh, iu, ifm, ifs);
-            const double *rs = sh.fxs + 6 * ifs;
-            const double *rm = sh.fxs + 6 * ifm;
-            double rs12x = rs[0] - rs[2], rs12y = rs[1] - rs[3];
-            double rs31x = rs[4] - rs[0], rs31y = rs[5] - rs[1];
-            double rs21x = -rs12x, rs21y = -rs12y;
-            double rmcx = ((rm[0] + rm[2]) + rm[4]) / 3.0;
-            double rmcy = ((rm[1] + rm[3]) + rm[5]) / 3.0;
-            double rtx = (rs[0] + rs21x * QXI[qp]) + rs31x * QETA[qp];
-            double rty = (rs[1] + rs21y * QXI[qp]) + rs31y * QETA[qp];
-            urx = ul * (rmcx - rtx);
-            ury = ul * (rmcy - rty);
-        }
-        urxs[r] = urx;
-        urys[r] = ury;
-    }
-    __syncthreads();
-    // Rows of this thread: 4*rg .. 4*rg+3; phasings: f0 + pg + 16*j.
-    cd js[4][TN], ks[4][TN];
-    unsigned done = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int r = 4 * rg + i;
-        const bool live = r < 63 && u0 + r / 7 < nufp;
-        double arg = sqrt(urxs[r] * urxs[r] + urys[r] * urys[r]);
-        double rterm = exp(-arg);
-        double j0 = arg <= 1e-5 ? (-arg / 6 + 0.5) * arg - 1 : (rterm - 1) / arg;
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            js[i][j] = cd{j0, 0.0};
-            ks[i][j] = cd{rterm, 0.0};
-            if (!live || f0 + pg + 16 * j >= nf) done |= 1u << (i * TN + j);
-        }
-    }
-    const unsigned all = (1u << (4 * TN)) - 1u;
-    const unsigned live0 = ~done & all;
-    unsigned long long npts = 0;     // lattice points of the reference's sums
-    int rlast = 0;
-    for (int r = 1; r <= JK_RINGMAX; ++r) {
-        if (!__syncthreads_or(done != all)) break;
-        rlast = r;
-        T jr[4][TN][2], kr[4][TN][2];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                jr[i][j][0] = 0; jr[i][j][1] = 0;
-                kr[i][j][0] = 0; kr[i][j][1] = 0;
-            }
-        const int64_t ebase = ring_base(r);
-        for (int c = 0; c < r; ++c) {
-            // lattice term tile: 64 rows x 8 points
-            for (int t = tid; t < 512; t += 256) {
-                const int row = t & 63, kk = t >> 6;
-                int m, n;
-                ring_mn(r, 8 * c + kk, m, n);
-                double dx = urxs[row] - ((double)m * us1x + (double)n * us2x);
-                double dy = urys[row] - ((double)m * us1y + (double)n * us2y);
-                T a, e, ea;
-                if (sizeof(T) == 4) {
-                    float fx = (float)dx, fy = (float)dy;
-                    float af = __fsqrt_rn(fx * fx + fy * fy);
-                    float ef = __expf(-af);
-                    a = af;
-                    e = ef;
-                    ea = fdiv_rcp(ef, af);
-                } else {
-                    a = sqrt(dx * dx + dy * dy);
-                    e = exp(-(double)a);
-                    ea = e * (1.0 / (double)a);
-                }
-                Aj[kk][row] = ea;
-                Ak[kk][row] = e;
-            }
-            // phase tile: 8 points x FT phasings
-            for (int t = tid; t < 8 * FT; t += 256) {
-                const int kk = t / FT, f = t % FT;
-                const T *pp = phase + (size_t)(ebase + 8 * c + kk) * (2 * ldp) + 2 * (f0 + f);
-                Bre[kk][f] = pp[0];
-                Bim[kk][f] = pp[1];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-                // fused multiply-adds: the sums' order already differs from the
-                // reference's sequential one, so per-step rounding is not matched
-#pragma clang fp contract(fast)
-                T aj[4], ak[4], pr[TN], pi[TN];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    aj[i] = Aj[kk][4 * rg + i];
-                    ak[i] = Ak[kk][4 * rg + i];
-                }
-#pragma unroll
-                for (int j = 0; j < TN; ++j) {
-                    pr[j] = Bre[kk][pg + 16 * j];
-                    pi[j] = Bim[kk][pg + 16 * j];
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < TN; ++j) {
-                        jr[i][j][0] += aj[i] * pr[j];
-                        jr[i][j][1] += aj[i] * pi[j];
-                        kr[i][j][0] += ak[i] * pr[j];
-                        kr[i][j][1] += ak[i] * pi[j];
-                    }
-            }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                const unsigned bit = 1u << (i * TN + j);
-                if (!(done & bit)) {
-                    cd a = cd{(double)jr[i][j][0], (double)jr[i][j][1]};
-                    cd b = cd{(double)kr[i][j][0], (double)kr[i][j][1]};
-                    js[i][j] += a;
-                    ks[i][j] += b;
-                    if (r > 5) {
-                        double c1 = cabs(a / js[i][j]), c2 = cabs(b / ks[i][j]);
-                        if ((c1 > c2 ? c1 : c2) < 1e-8) {
-                            done |= bit;
-                            npts += lattice_points(r);
-                        }
-                    }
-                }
-            }
-    }
-    if (bcnt) {
-        npts += (unsigned long long)__popc(live0 & ~done) * lattice_points(rlast);
-        block_count(npts, bcnt);
-    }
-    // Combine the 7 quadrature points of each class (src/Zint.jl:117-124).
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            Cs[4 * rg + i][pg + 16 * j][0] = js[i][j];
-            Cs[4 * rg + i][pg + 16 * j][1] = ks[i][j];
-        }
-    __syncthreads();
-    for (int t = tid; t < 9 * FT; t += 256) {
-        const int u = t / FT, f = t % FT;
-        const int64_t iu = u0 + u;
-        if (iu >= nufp || f0 + f >= nf) continue;
-        cd J = {0, 0}, Jx = {0, 0}, Je = {0, 0}, K = {0, 0}, Kx = {0, 0}, Ke = {0, 0};
-#pragma unroll
-        for (int q = 0; q < 7; ++q) {
-            cd jw = Cs[7 * u + q][f][0] * QW[q];
-            Jx += jw * QXI[q];
-            Je += jw * QETA[q];
-            J += jw;
-            cd kw = Cs[7 * u + q][f][1] * QW[q];
-            Kx += kw * QXI[q];
-            Ke += kw * QETA[q];
-            K += kw;
-        }
-        cd *o = jk + (int64_t)(f0 + f) * jk_stride;
-        o[0 * nufp + iu] = J;
-        o[1 * nufp + iu] = Jx;
-        o[2 * nufp + iu] = Je;
-        o[3 * nufp + iu] = K;
-        o[4 * nufp + iu] = Kx;
-        o[5 * nufp + iu] = Ke;
-    }
-}
-
 // Lattice points of rings 0..JK_RINGMAX in ring order: the offset u (m s1 + n s2)
-// in FP32 and (m, n), for the FP32 spatial sums.
+// in the spatial sums' precision T, and (m, n).
 // Phasing halves (32 phasings each) per workgroup of the MFMA spatial sums: one.
 // Two (512 threads, 64 phasings, the term tile shared by twice the MFMA work) needs
 // about 195 VGPRs, runs two waves per SIMD and measured slower on MI355X.
 constexpr int JK_MFMA_WH = 1;
 
-struct LatPt {
-    float ox, oy;
+template <class T>
+struct LatPtT {
+    T ox, oy;
     int m, n;
 };
+typedef LatPtT<float> LatPt;
 
-__global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, LatPt *lat) {
+template <class T>
+__global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, LatPtT<T> *lat) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
     if (e >= E) return;
     const int r = ring_of(e);
     int m, n;
     ring_mn(r, (int)(e - ring_base(r)), m, n);
-    LatPt L;
-    L.ox = (float)((double)m * us1x + (double)n * us2x);
-    L.oy = (float)((double)m * us1y + (double)n * us2y);
+    LatPtT<T> L;
+    L.ox = (T)((double)m * us1x + (double)n * us2x);
+    L.oy = (T)((double)m * us1y + (double)n * us2y);
     L.m = m;
     L.n = n;
     lat[e] = L;
 }
 
-// MFMA form of the many-phasing spatial sums (mixed precision).  A workgroup of
-// 4 WH wavefronts owns 64 rows (9 classes x 7 quadrature points) and 32 WH
-// phasings.  Per chunk of 8 lattice points the threads form the FP32 lattice term
-// pairs (J: e^-a / a, K: e^-a) of the 64 x 8 tile and the phases of the 8 x 32 WH
-// tile; the term tile is the A operand and the phase tile (cos and sin columns)
-// the B operand of 16x16x4 FP32 MFMAs.  Wavefront (wr, wh) accumulates
+// MFMA form of the many-phasing spatial sums.  A workgroup of 4 WH wavefronts
+// owns 64 rows (9 classes x 7 quadrature points) and 32 WH phasings.  Chunk k
+// holds lattice points 1 + 8 k .. 8 k + 8 in ring order.  Per chunk the threads
+// form the lattice term pairs (J: e^-a / a, K: e^-a) of the 64 x 8 tile and the
+// phases of the 8 x 32 WH tile; the term tile is the A operand and the phase tile
+// (cos and sin columns) the B operand of 16x16x4 MFMAs.  The tiles of chunk k + 1
+// are formed between the MFMAs of chunk k (double-buffered, one barrier per
+// chunk; rows padded against bank conflicts).  Wavefront (wr, wh) accumulates
 // rows 16 wr..16 wr + 15 against phasings 32 wh..32 wh + 31; each lane folds the
 // ring sums of its 8 (row, phasing) pairs into FP64 running sums at the end of
-// ring 5 (rings 1..5 together, summed in the FP32 accumulators) and of every later
-// ring, and applies the break test of src/PGF.jl:86-103.  Lattice offsets come from
-// an FP32 table (k_lattice); phases of rings <= JK_RT are products of per-phasing
-// power tables in LDS.  Tiles are double-buffered, one barrier per chunk; their
-// rows are padded so the operand reads are free of bank conflicts.  Phasing
-// blocks start at block fblk0; NG = 1 serves a last block with at most 16 live
-// phasings, whose second group of 16 would be all padding, with half the MFMAs.
-template <int WH, int NG>
-__global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) void k_jk_mfma(
-    SheetDev sh, int64_t nufp, double ul, const LatPt *lat, const float *phase, int ldp, int nf,
-    int fblk0, cd *jk, int64_t jk_stride, unsigned long long *bcnt) {
-    typedef MfmaT<float> MT;
-    typedef MT::V V;
+// ring 5 (rings 1..5 together, in the accumulators) and of every later ring, and
+// applies the break test of src/PGF.jl:86-103.  Lattice offsets come from a table
+// (k_lattice); phases of rings <= JK_RT are products of per-phasing power tables
+// in LDS.  Phasing blocks start at block fblk0; NG = 1 serves a last block with at
+// most 16 live phasings, whose second group of 16 would be all padding, with half
+// the MFMAs.
+template <class T, int WH, int NG>
+__device__ __forceinline__ void jk_mfma_body(SheetDev sh, int64_t nufp, double ul,
+                                             const LatPtT<T> *lat, const T *phase, int ldp,
+                                             int nf, int fblk0, cd *jk, int64_t jk_stride,
+                                             unsigned long long *bcnt) {
+    typedef MfmaT<T> MT;
+    typedef typename MT::V V;
+    constexpr bool F32 = std::is_same<T, float>::value;
     constexpr int NT = 256 * WH, FT = 32 * WH, AP = 80, BP = 16 + FT;
     struct Tiles {
-        float Aj[2][8][AP], Ak[2][8][AP];
-        float Bre[2][8][BP], Bim[2][8][BP];
+        T Aj[2][8][AP], Ak[2][8][AP];
+        T Bre[2][8][BP], Bim[2][8][BP];
+    };
+    struct T2 {
+        T x, y;
     };
     constexpr size_t CSB = sizeof(cd) * 64 * 32;
     constexpr size_t LBYTES = sizeof(Tiles) > CSB ? sizeof(Tiles) : CSB;
     __shared__ __align__(16) char lraw[LBYTES];
     __shared__ double urxs[64], urys[64];
     constexpr int PW = 2 * JK_RT + 1;
-    __shared__ float P1[2][FT][PW], P2[2][FT][PW];   // [cos/sin][phasing][m + JK_RT]
+    __shared__ T P1[2][FT][PW], P2[2][FT][PW];   // [cos/sin][phasing][m + JK_RT]
     Tiles &TL = *reinterpret_cast<Tiles *>(lraw);
     cd(*Cs)[32] = reinterpret_cast<cd(*)[32]>(lraw);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -802,14 +610,14 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
     for (int t = tid; t < 2 * FT * PW; t += NT) {
         const int which = t / (FT * PW), f = (t / PW) % FT, j = t % PW - JK_RT;
         const int64_t e = which == 0 ? ring_index(j, 0) : ring_index(0, j);
-        const float *pp = phase + e * 2 * (int64_t)ldp + 2 * (f0 + f);
+        const T *pp = phase + e * 2 * (int64_t)ldp + 2 * (f0 + f);
         if (which == 0) { P1[0][f][j + JK_RT] = pp[0]; P1[1][f][j + JK_RT] = pp[1]; }
         else { P2[0][f][j + JK_RT] = pp[0]; P2[1][f][j + JK_RT] = pp[1]; }
     }
     __syncthreads();
     // this thread's term row (its lane) relative to the lattice offsets; its
     // phase slot (point pkk, phasing pf)
-    const float urx = (float)urxs[lane], ury = (float)urys[lane];
+    const T urx = (T)urxs[lane], ury = (T)urys[lane];
     const int pkk = tid / FT, pf = tid % FT;
     const int pc = lane & 15;
     cd js[2][4], ks[2][4];
@@ -828,7 +636,7 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
             if (!live || f0 + 32 * wh + 16 * g + pc >= nf) done |= 1u << (4 * g + e);
         }
     }
-    const float *pcol = phase + 2 * (f0 + pf);
+    const T *pcol = phase + 2 * (f0 + pf);
     const int64_t pstride = 2 * (int64_t)ldp;
     const unsigned live0 = ~done & 0xffu;
     unsigned long long npts = 0;     // lattice points of the reference's sums
@@ -838,23 +646,23 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
     // while the MFMAs of chunk k run, so that the term arithmetic and the matrix
     // products overlap; lattice data is loaded one chunk earlier still.
     constexpr int NCH = JK_RINGMAX * (JK_RINGMAX + 1) / 2;
-    float2 lt[2 / WH];               // term offsets of the next chunk to be formed
+    T2 lt[2 / WH];                   // term offsets of the next chunk to be formed
     int2 lq;                         // phase indices of that chunk
     auto load_lat = [&](int k) {
 #pragma unroll
         for (int h = 0; h < 2 / WH; ++h) {
-            const LatPt &L = lat[1 + 8 * (int64_t)k + wave + 4 * WH * h];
-            lt[h] = float2{L.ox, L.oy};
+            const LatPtT<T> &L = lat[1 + 8 * (int64_t)k + wave + 4 * WH * h];
+            lt[h] = T2{L.ox, L.oy};
         }
-        const LatPt &Q = lat[1 + 8 * (int64_t)k + pkk];
+        const LatPtT<T> &Q = lat[1 + 8 * (int64_t)k + pkk];
         lq = int2{Q.m, Q.n};
     };
     // the phase tile of chunk k (ring rk) into buffer b, from lq
     auto form_phase = [&](int k, int rk, int b) {
-        float br, bi;
+        T br, bi;
         if (rk <= JK_RT) {
-            const float c1 = P1[0][pf][lq.x + JK_RT], s1 = P1[1][pf][lq.x + JK_RT];
-            const float c2 = P2[0][pf][lq.y + JK_RT], s2 = P2[1][pf][lq.y + JK_RT];
+            const T c1 = P1[0][pf][lq.x + JK_RT], s1 = P1[1][pf][lq.x + JK_RT];
+            const T c2 = P2[0][pf][lq.y + JK_RT], s2 = P2[1][pf][lq.y + JK_RT];
             br = c1 * c2 - s1 * s2;
             bi = c1 * s2 + s1 * c2;
         } else {
@@ -868,11 +676,18 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
     // term h of this thread for the chunk in lt, into buffer b
     auto form_term = [&](int b, int h) {
         const int kk = wave + 4 * WH * h;
-        const float dx = urx - lt[h].x, dy = ury - lt[h].y;
-        const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
-        const float ex = __expf(-a);
-        TL.Aj[b][kk][lane] = fdiv_rcp(ex, a);
-        TL.Ak[b][kk][lane] = ex;
+        const T dx = urx - lt[h].x, dy = ury - lt[h].y;
+        if constexpr (F32) {
+            const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
+            const float ex = __expf(-a);
+            TL.Aj[b][kk][lane] = fdiv_rcp(ex, a);
+            TL.Ak[b][kk][lane] = ex;
+        } else {
+            const double a = sqrt(dx * dx + dy * dy);
+            const double ex = exp(-a);
+            TL.Aj[b][kk][lane] = ex * (1.0 / a);
+            TL.Ak[b][kk][lane] = ex;
+        }
     };
     static_assert(WH == 1, "the interleaved chunk loop forms two terms per thread");
     if (__syncthreads_or(done != 0xffu)) {
@@ -897,7 +712,7 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
             // are formed again at the very end, into a buffer nobody reads)
             const int kn = k + 1 < NCH ? k + 1 : NCH - 1;
             form_phase(kn, k + 1 < kend ? r : r + 1, b ^ 1);
-            float a_j[2], a_k[2], bre[2][2], bim[2][2];
+            T a_j[2], a_k[2], bre[2][2], bim[2][2];
 #pragma unroll
             for (int k4 = 0; k4 < 2; ++k4) {
                 const int kr = 4 * k4 + (lane >> 4);
@@ -927,7 +742,7 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
 #pragma unroll
                 for (int i = 0; i < 4 * NG; ++i) {
                     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, F32 ? 2 : 6, 0);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -1012,6 +827,20 @@ __global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) v
                 o[2 * nufp + iu] = Se;
             }
         }
+}
+
+template <int WH, int NG>
+__global__ __launch_bounds__(256 * WH) __attribute__((amdgpu_waves_per_eu(3))) void k_jk_mfma(
+    SheetDev sh, int64_t nufp, double ul, const LatPt *lat, const float *phase, int ldp, int nf,
+    int fblk0, cd *jk, int64_t jk_stride, unsigned long long *bcnt) {
+    jk_mfma_body<float, WH, NG>(sh, nufp, ul, lat, phase, ldp, nf, fblk0, jk, jk_stride, bcnt);
+}
+
+template <int NG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_jk_mfma64(
+    SheetDev sh, int64_t nufp, double ul, const LatPtT<double> *lat, const double *phase,
+    int ldp, int nf, int fblk0, cd *jk, int64_t jk_stride, unsigned long long *bcnt) {
+    jk_mfma_body<double, 1, NG>(sh, nufp, ul, lat, phase, ldp, nf, fblk0, jk, jk_stride, bcnt);
 }
 
 // Lattice phase table exp(-i (m psi1 + n psi2)) over rings 0..JK_RINGMAX in ring
@@ -1153,7 +982,7 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
                const double *us2, const double *d_psi, int nf, cd *jk, int64_t jk_stride,
                int mixed, hipStream_t st, unsigned long long *total) {
     if (nf < 1) return;
-    if (nf < JK_TILED_MIN) {
+    if (nf < JK_MFMA_MIN) {
         const int per = mixed ? 16 : 8;
         dim3 grid((unsigned)((nufp + 35) / 36));
         for (int g = 0; g < nf; g += per) {
@@ -1176,8 +1005,7 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
         }
         return;
     }
-    // phasings per workgroup: 32 JK_MFMA_WH for the MFMA kernel, 32 (16 * TN) for the
-    // tiled one
+    // phasings per workgroup: 32 JK_MFMA_WH (FP32) or 32 (FP64)
     const int FT = mixed ? 32 * JK_MFMA_WH : 32;
     const int ldp = (nf + FT - 1) / FT * FT;
     const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
@@ -1186,42 +1014,57 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
     dim3 grid((unsigned)((nufp + 8) / 9), (unsigned)(ldp / FT));
     const int64_t nblk = (int64_t)grid.x * grid.y;
     const size_t phase_bytes = (size_t)E * 2 * ldp * elt;
-    const size_t lat_bytes = mixed ? (size_t)E * sizeof(LatPt) : 0;
+    const size_t lat_bytes = (size_t)E * (mixed ? sizeof(LatPtT<float>) : sizeof(LatPtT<double>));
     const size_t cnt_bytes = total ? (size_t)nblk * sizeof(unsigned long long) : 0;
     void *ph = nullptr;
     if (hipMallocAsync(&ph, phase_bytes + lat_bytes + cnt_bytes, st) != hipSuccess) {
         (void)hipGetLastError();
         throw HipError("hipMallocAsync of the lattice phase table failed");
     }
-    LatPt *lat = reinterpret_cast<LatPt *>((char *)ph + phase_bytes);
+    void *lat = (char *)ph + phase_bytes;
     unsigned long long *bcnt =
         total ? reinterpret_cast<unsigned long long *>((char *)ph + phase_bytes + lat_bytes)
               : nullptr;
+    const dim3 lg((unsigned)((E + 255) / 256));
     if (mixed) {
         hipLaunchKernelGGL(k_phase_table<float>, pg, dim3(256), 0, st, d_psi, nf, ldp, (float *)ph);
-        hipLaunchKernelGGL(k_lattice, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, us1[0],
-                           us1[1], us2[0], us2[1], lat);
-        // a last block with at most 16 live phasings runs the half-width kernel
-        const int nblk_f = (int)grid.y;
-        const bool half_tail = JK_MFMA_WH == 1 && nf - (nblk_f - 1) * FT <= 16;
-        dim3 gmain(grid.x, (unsigned)(half_tail ? nblk_f - 1 : nblk_f));
-        if (gmain.y > 0)
-            hipLaunchKernelGGL((k_jk_mfma<JK_MFMA_WH, 2>), gmain, dim3(256 * JK_MFMA_WH), 0, st,
-                               sh, nufp, ulocal, lat, (const float *)ph, ldp, nf, 0, jk,
-                               jk_stride, bcnt);
-        if (half_tail) {
-            HIPCHECK(hipGetLastError());
-            hipLaunchKernelGGL((k_jk_mfma<JK_MFMA_WH, 1>), dim3(grid.x, 1), dim3(256 * JK_MFMA_WH),
-                               0, st, sh, nufp, ulocal, lat, (const float *)ph, ldp, nf,
-                               nblk_f - 1, jk, jk_stride,
-                               bcnt ? bcnt + (int64_t)(nblk_f - 1) * grid.x : nullptr);
-        }
+        hipLaunchKernelGGL(k_lattice<float>, lg, dim3(256), 0, st, us1[0], us1[1], us2[0], us2[1],
+                           (LatPtT<float> *)lat);
     } else {
         hipLaunchKernelGGL(k_phase_table<double>, pg, dim3(256), 0, st, d_psi, nf, ldp,
                            (double *)ph);
-        hipLaunchKernelGGL((k_jk_tiled<double, 2>), grid, dim3(256), 0, st, sh, nufp, ulocal,
-                           us1[0], us1[1], us2[0], us2[1], (const double *)ph, ldp, nf, jk,
-                           jk_stride, bcnt);
+        hipLaunchKernelGGL(k_lattice<double>, lg, dim3(256), 0, st, us1[0], us1[1], us2[0], us2[1],
+                           (LatPtT<double> *)lat);
+    }
+    HIPCHECK(hipGetLastError());
+    // a last block with at most 16 live phasings runs the half-width kernel
+    const int nblk_f = (int)grid.y;
+    const bool half_tail = (!mixed || JK_MFMA_WH == 1) && nf - (nblk_f - 1) * FT <= 16;
+    const dim3 gmain(grid.x, (unsigned)(half_tail ? nblk_f - 1 : nblk_f)), gtail(grid.x, 1);
+    unsigned long long *btail = bcnt ? bcnt + (int64_t)(nblk_f - 1) * grid.x : nullptr;
+    if (mixed) {
+        const LatPtT<float> *lf = (const LatPtT<float> *)lat;
+        const float *pf = (const float *)ph;
+        if (gmain.y > 0)
+            hipLaunchKernelGGL((k_jk_mfma<JK_MFMA_WH, 2>), gmain, dim3(256 * JK_MFMA_WH), 0, st,
+                               sh, nufp, ulocal, lf, pf, ldp, nf, 0, jk, jk_stride, bcnt);
+        if (half_tail) {
+            HIPCHECK(hipGetLastError());
+            hipLaunchKernelGGL((k_jk_mfma<JK_MFMA_WH, 1>), gtail, dim3(256 * JK_MFMA_WH), 0, st,
+                               sh, nufp, ulocal, lf, pf, ldp, nf, nblk_f - 1, jk, jk_stride,
+                               btail);
+        }
+    } else {
+        const LatPtT<double> *ld = (const LatPtT<double> *)lat;
+        const double *pd = (const double *)ph;
+        if (gmain.y > 0)
+            hipLaunchKernelGGL((k_jk_mfma64<2>), gmain, dim3(256), 0, st, sh, nufp, ulocal, ld,
+                               pd, ldp, nf, 0, jk, jk_stride, bcnt);
+        if (half_tail) {
+            HIPCHECK(hipGetLastError());
+            hipLaunchKernelGGL((k_jk_mfma64<1>), gtail, dim3(256), 0, st, sh, nufp, ulocal, ld, pd,
+                               ldp, nf, nblk_f - 1, jk, jk_stride, btail);
+        }
     }
     HIPCHECK(hipGetLastError());
     if (total) {
