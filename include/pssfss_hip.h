@@ -167,7 +167,8 @@ int pssfss_fill(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt, do
  * derivative with respect to the free-space wavenumber pt->k0 (1/m) at fixed
  * phasing (pt->psi1, pt->psi2) and fixed u, both nbf x nbf complex column-major.
  * The spectral tables are differentiated summand by summand (same table size M);
- * the spatial sums and singular integrals do not depend on k0.  'J' sheets. */
+ * the spatial sums and singular integrals do not depend on k0.  Both sheet
+ * classes ('J': dZ/dk0 of the impedance matrix, 'M': of the admittance matrix). */
 int pssfss_fill_dk0(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt,
                     double *zmat, double *dzmat, pssfss_point_info *info);
 

@@ -248,6 +248,8 @@ TabInput make_tab_input(const pssfss_point *pt, bool magnetic, const double *b1,
         c.f3 = es * c3s + es1 * c3sp1;
         c.p1 = 2.0 / c.mut;
         c.p3 = d3s / ms + d3sp1 / ms1;
+        c.dc3 = es * c3_dk0(k0, es, ms, es, ms) + es1 * c3_dk0(k0, es1, ms1, es1, ms1);
+        c.dd3 = d3_dk0(k0, es, ms, es, ms) / ms + d3_dk0(k0, es1, ms1, es1, ms1) / ms1;
     }
     in.psi1 = pt->psi1;
     in.psi2 = pt->psi2;
@@ -407,7 +409,7 @@ void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap
     sync(ctx);   // temporaries die at return
 }
 
-// d/dk0 of the tables `val` of the points `in` (electric) at fixed phasing and u:
+// d/dk0 of the tables `val` of the points `in` at fixed phasing and u:
 // the derivative summands over the rings the tables use, then the same linear
 // steps as the values, with the same M per point.
 void build_table_dk0(pssfss_ctx *ctx, const std::vector<TabInput> &in, const TableSet &val,
@@ -417,7 +419,6 @@ void build_table_dk0(pssfss_ctx *ctx, const std::vector<TabInput> &in, const Tab
     std::vector<TabParams> tp(np);
     int Mmax = 0;
     for (int p = 0; p < np; ++p) {
-        if (in[p].tp.magnetic) throw ArgError("k0 derivative: electric ('J') sheets only");
         tp[p] = in[p].tp;
         Mmax = std::max(Mmax, val.M[p]);
     }
@@ -1440,7 +1441,6 @@ int pssfss_fill_dk0(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt
                     double *zmat, double *dzmat, pssfss_point_info *info) {
     return guarded(ctx, [&] {
         if (!sheet || !zmat || !dzmat) throw ArgError("null argument");
-        if (sheet->cls != 'J') throw ArgError("k0 derivative: electric ('J') sheets only");
         HIPCHECK(hipSetDevice(ctx->device));
         check_point(sheet, pt, false);
         hipStream_t st = ctx->st;
@@ -1465,19 +1465,28 @@ int pssfss_fill_dk0(pssfss_ctx *ctx, pssfss_sheet *sheet, const pssfss_point *pt
         f.jk = sheet->jk.p;   // the sheet's cache holds this point's spatial sums now
         f.dtab1 = dt.tab1(0);
         f.dtab2 = dt.tab2(0);
-        f.dc3 = c3_dk0(pt->k0, m1, e1, m2, e2);
-        f.dd3 = d3_dk0(pt->k0, m1, e1, m2, e2);
+        const double k0 = pt->k0, u = pt->u;
+        const int mag = sheet->cls == 'M' ? 1 : 0;
+        if (!mag) {
+            f.dc3 = c3_dk0(k0, m1, e1, m2, e2);
+            f.dd3 = d3_dk0(k0, m1, e1, m2, e2);
+        } else {   // derivatives of KFfact and KPfact of make_fill_params
+            f.dc3 = (c3_dk0(k0, m1, e1, m1, e1) * e1 + c3_dk0(k0, m2, e2, m2, e2) * e2) /
+                    ((2.0 * f.epsb) * u);
+            f.dd3 = (f.mut / (2.0 * u)) *
+                    (d3_dk0(k0, m1, e1, m1, e1) / m1 + d3_dk0(k0, m2, e2, m2, e2) / m2);
+        }
         DBuf<cd> dZ((size_t)n * n);
         f.Z = dZ.p;
         DBuf<FillParams> dfp(1);
         dfp.upload(&f, 1, st);
         const int64_t gstride = 9 * sheet->nufp;
         ctx->wG.ensure((size_t)gstride);
-        launch_fill_classes_dk0(sheet->dev(), dfp.p, sheet->rho_r.p, sheet->rinv.p, ctx->wG.p,
-                                st);
+        launch_fill_classes_dk0(sheet->dev(), dfp.p, sheet->rho_r.p, sheet->rinv.p, mag,
+                                ctx->wG.p, st);
         // the surface-resistance term does not depend on k0
         launch_fill_gather(sheet->dev(), dfp.p, 1, sheet->bf_face.p, sheet->bf_loc.p,
-                           sheet->fpcls.p, ctx->wG.p, gstride, 0, 0, st);
+                           sheet->fpcls.p, ctx->wG.p, gstride, 0, mag, st);
         d2h(ctx, h.data(), dZ.p, h.size());
         sync(ctx);
         o = (cd *)dzmat;

@@ -183,10 +183,13 @@ __global__ __launch_bounds__(256) void k_fill_classes(SheetDev sh, const FillPar
     }
 }
 
-// d/dk0 of the electric class terms of k_fill_classes at fixed phasing and u.  k0
-// enters through the two spectral tables (P.dtab1, P.dtab2 hold their derivatives,
-// interpolated with the same weights), c3, d3 and omega = c k0; the spatial sums,
-// the singular integrals and the Floquet flags do not depend on k0.  Planes as G.
+// d/dk0 of the class terms of k_fill_classes at fixed phasing and u.  k0 enters
+// through the two spectral tables (P.dtab1, P.dtab2 hold their derivatives,
+// interpolated with the same weights), the spatial-sum coefficients (electric c3,
+// d3; magnetic KFfact, KPfact; derivatives in P.dc3, P.dd3) and omega = c k0; the
+// spatial sums, the singular integrals and the Floquet flags do not depend on k0.
+// Planes as G.
+template <bool MAGNETIC>
 __global__ __launch_bounds__(256) void k_fill_classes_dk0(SheetDev sh, const FillParams *fps,
                                                           const double *rho_r,
                                                           const double *rinv_, cd *G) {
@@ -240,13 +243,23 @@ __global__ __launch_bounds__(256) void k_fill_classes_dk0(SheetDev sh, const Fil
         rc2[i][1] = 0.5 * (rmcy - rm[i][1]);
     }
     const double u = P.u, w = P.omega, dw = C0;
-    const cd phi_core = ((FOURPI * I2) + u * (ri + J)) + (P.d3 / u) * K;
-    const cd dphi_core = (FOURPI * D2) + (P.dd3 / u) * K;
-    // Phi = pf phi_core / omega
-    const cd pf = P.psf_base * (1.0 / (TWOPI * EPS0));
-    const cd dPhi = pf * ((1.0 / w) * dphi_core - (dw / (w * w)) * phi_core);
-    const cd asf = (MU0 / FOURPI) * P.mut;
-    const cd iw = cd{0.0, w}, idw = cd{0.0, dw};
+    cd phi_core, dphi_core, dPhi, asf;
+    if (!MAGNETIC) {
+        phi_core = ((FOURPI * I2) + u * (ri + J)) + (P.d3 / u) * K;
+        dphi_core = (FOURPI * D2) + (P.dd3 / u) * K;
+        // Phi = pf phi_core / omega
+        const cd pf = P.psf_base * (1.0 / (TWOPI * EPS0));
+        dPhi = pf * ((1.0 / w) * dphi_core - (dw / (w * w)) * phi_core);
+        asf = (MU0 / FOURPI) * P.mut;
+    } else {
+        phi_core = ((P.I2fact * I2) + u * (ri + J)) + P.KPfact * K;
+        dphi_core = (P.I2fact * D2) + P.dd3 * K;
+        // Phi = psf_base phi_core, psf_base proportional to 1 / omega
+        dPhi = P.psf_base * (dphi_core - (dw / w) * phi_core);
+        asf = (-EPS0 / PI_) * P.epsb;
+    }
+    const double sgn = MAGNETIC ? -1.0 : 1.0;
+    const cd iw = cd{0.0, sgn * w}, idw = cd{0.0, sgn * dw};
 #pragma unroll
     for (int isl = 0; isl < 3; ++isl) {
         const double singx = rrx + ((u * ri) * (rmcx - rs[isl][0]));
@@ -259,8 +272,13 @@ __global__ __launch_bounds__(256) void k_fill_classes_dk0(SheetDev sh, const Fil
             cd D1i = ((rs[0][d] * D1x + rs[1][d] * D1e) + rs[2][d] * D1z) - rs[isl][d] * D1;
             cd Ji = ((rs[0][d] * Jx + rs[1][d] * Je) + rs[2][d] * Jz) - rs[isl][d] * J;
             cd Ki = ((rs[0][d] * Kx + rs[1][d] * Ke) + rs[2][d] * Kz) - rs[isl][d] * K;
-            Ai[d] = asf * ((((FOURPI * I1i) + sing[d]) + u * Ji) + (P.c3 / u) * Ki);
-            dAi[d] = asf * ((FOURPI * D1i) + (P.dc3 / u) * Ki);
+            if (!MAGNETIC) {
+                Ai[d] = asf * ((((FOURPI * I1i) + sing[d]) + u * Ji) + (P.c3 / u) * Ki);
+                dAi[d] = asf * ((FOURPI * D1i) + (P.dc3 / u) * Ki);
+            } else {
+                Ai[d] = asf * ((((P.I1fact * I1i) + sing[d]) + u * Ji) + P.KFfact * Ki);
+                dAi[d] = asf * ((P.I1fact * D1i) + P.dc3 * Ki);
+            }
         }
 #pragma unroll
         for (int iml = 0; iml < 3; ++iml) {
@@ -358,9 +376,14 @@ void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
 }
 
 void launch_fill_classes_dk0(const SheetDev &sh, const FillParams *d_fp, const double *rho_r,
-                             const double *rinv, cd *G, hipStream_t st) {
+                             const double *rinv, int magnetic, cd *G, hipStream_t st) {
     dim3 grid((unsigned)((fill_class_threads(sh) + 255) / 256));
-    hipLaunchKernelGGL(k_fill_classes_dk0, grid, dim3(256), 0, st, sh, d_fp, rho_r, rinv, G);
+    if (magnetic)
+        hipLaunchKernelGGL(k_fill_classes_dk0<true>, grid, dim3(256), 0, st, sh, d_fp, rho_r,
+                           rinv, G);
+    else
+        hipLaunchKernelGGL(k_fill_classes_dk0<false>, grid, dim3(256), 0, st, sh, d_fp, rho_r,
+                           rinv, G);
     HIPCHECK(hipGetLastError());
 }
 

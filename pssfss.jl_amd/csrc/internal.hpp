@@ -104,7 +104,7 @@ struct TabParams {
     cd eps[MAXL], mu[MAXL];
     double width[MAXL];
     cd mut, epsb, c3, d3, f1, f3, p1, p3;
-    cd dc3, dd3;                // d c3 / d k0, d d3 / d k0 (electric)
+    cd dc3, dd3;                // d/dk0 of c3, d3 (electric) or of f3, p3 (magnetic)
 };
 
 // Per-point fill parameters.
@@ -121,13 +121,13 @@ struct FillParams {
     const cd *jk;               // 6*nufp: J, Jxi, Jeta, K, Kxi, Keta (SoA)
     cd *Z;                      // nbf*nbf
     const cd *dtab1, *dtab2;    // d/dk0 of the tables (k0-derivative fill only)
-    cd dc3, dd3;                // d c3 / d k0, d d3 / d k0
+    cd dc3, dd3;                // d/dk0 of c3, d3 (electric) or KFfact, KPfact (magnetic)
 };
 
 // ---- pgf.hip
 void launch_summands(const TabParams *d_tp, int npts, int64_t cap_entries, int r0, int r1,
                      cd *t1, cd *t2, hipStream_t st);
-// d/dk0 of the electric summands at fixed phasing and u, rings r0..r1.
+// d/dk0 of the summands at fixed phasing and u, rings r0..r1.
 void launch_summands_dk0(const TabParams *d_tp, int npts, int64_t cap_entries, int r0, int r1,
                          cd *t1, cd *t2, hipStream_t st);
 void launch_ringsums(const TabParams *d_tp, int npts, int64_t cap_entries, int r0, int r1,
@@ -154,9 +154,9 @@ void launch_zero(cd *p, int64_t n, hipStream_t st);
 void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
                          const double *rho_r, const double *rinv, int magnetic, cd *G,
                          int64_t gstride, hipStream_t st);
-// d/dk0 of the electric class terms of one point (P.dtab1, P.dtab2, P.dc3, P.dd3 set).
+// d/dk0 of the class terms of one point (P.dtab1, P.dtab2, P.dc3, P.dd3 set).
 void launch_fill_classes_dk0(const SheetDev &sh, const FillParams *d_fp, const double *rho_r,
-                             const double *rinv, cd *G, hipStream_t st);
+                             const double *rinv, int magnetic, cd *G, hipStream_t st);
 // Phase 2: Z of each point from G (fpcls: class of face pair ifm + nface*ifs, CSR only).
 void launch_fill_gather(const SheetDev &sh, const FillParams *d_fp, int npts,
                         const int32_t *bf_face, const int32_t *bf_loc, const int32_t *fpcls,

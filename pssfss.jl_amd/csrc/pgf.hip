@@ -94,10 +94,12 @@ __device__ __forceinline__ cdd dctanh(cdd z) {
     return cdd{dd{t.x, d.x}, dd{t.y, d.y}};
 }
 
-// d/dk0 of the electric summands of eqs. (5.19a,b) at fixed lattice wavevector
-// (fixed phasing) and fixed u, forward mode through the same transmission-line
-// recursion as summand().  k0 enters through k0^2 in every layer's gamma, c3, d3
-// and the small-beta floor b2min (proportional to k0^2).
+// d/dk0 of the summands of eqs. (5.19a,b) (electric) or (5.26a,b) (magnetic) at
+// fixed lattice wavevector (fixed phasing) and fixed u, forward mode through the
+// same transmission-line recursion as summand().  k0 enters through k0^2 in every
+// layer's gamma, the extraction constants (electric: c3, d3; magnetic: f3, p3,
+// whose derivatives TabParams holds in dc3, dd3) and the small-beta floor b2min
+// (proportional to k0^2).
 __device__ void summand_dk0(const TabParams &c, int m, int n, cd &d1, cd &d2) {
     const double bx = (c.b00[0] + (double)m * c.b1[0]) + (double)n * c.b2[0];
     const double by = (c.b00[1] + (double)m * c.b1[1]) + (double)n * c.b2[1];
@@ -109,32 +111,59 @@ __device__ void summand_dk0(const TabParams &c, int m, int n, cd &d1, cd &d2) {
     const dd kap = dsqrt(kap2);
     const int nl = c.nl, s = c.s;
     auto gamma = [&](int i) { return dmysqrt(mkc(b2) - (k0sq * c.eps[i]) * c.mu[i]); };
-    cdd g = gamma(0);
-    cdd zlte = c.mu[0] / g, zltm = g / c.eps[0];
-    for (int i = 1; i < s; ++i) {
-        g = gamma(i);
-        const cdd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
-        const cdd th = dctanh(c.width[i] * g);
-        zlte = (z0te * (zlte + z0te * th)) / (z0te + zlte * th);
-        zltm = (z0tm * (zltm + z0tm * th)) / (z0tm + zltm * th);
+    // value and derivative of the two extraction constants
+    const cdd x3 = c.magnetic ? cdd{dd{c.f3.x, c.dc3.x}, dd{c.f3.y, c.dc3.y}}
+                              : cdd{dd{c.c3.x, c.dc3.x}, dd{c.c3.y, c.dc3.y}};
+    const cdd y3 = c.magnetic ? cdd{dd{c.p3.x, c.dd3.x}, dd{c.p3.y, c.dd3.y}}
+                              : cdd{dd{c.d3.x, c.dd3.x}, dd{c.d3.y, c.dd3.y}};
+    cdd t1, t2;
+    if (!c.magnetic) {
+        cdd g = gamma(0);
+        cdd zlte = c.mu[0] / g, zltm = g / c.eps[0];
+        for (int i = 1; i < s; ++i) {
+            g = gamma(i);
+            const cdd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
+            const cdd th = dctanh(c.width[i] * g);
+            zlte = (z0te * (zlte + z0te * th)) / (z0te + zlte * th);
+            zltm = (z0tm * (zltm + z0tm * th)) / (z0tm + zltm * th);
+        }
+        g = gamma(nl - 1);
+        cdd zrte = c.mu[nl - 1] / g, zrtm = g / c.eps[nl - 1];
+        for (int i = nl - 2; i >= s; --i) {
+            g = gamma(i);
+            const cdd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
+            const cdd th = dctanh(c.width[i] * g);
+            zrte = (z0te * (zrte + z0te * th)) / (z0te + zrte * th);
+            zrtm = (z0tm * (zrtm + z0tm * th)) / (z0tm + zrtm * th);
+        }
+        cdd vte = (zlte * zrte) / (zlte + zrte);
+        cdd vtm = (zltm * zrtm) / (zltm + zrtm);
+        vte = (2.0 * vte) / c.mut;
+        t1 = vte - (1.0 + x3 / kap2) / kap;
+        vtm = (2.0 * c.epsb) * vtm;
+        t2 = (vtm + ((k0sq * c.epsb) * c.mut) * vte) / b2 - (1.0 + y3 / kap2) / kap;
+    } else {
+        cdd g = gamma(0);
+        cdd ylte = g / c.mu[0], yltm = c.eps[0] / g;
+        for (int i = 1; i < s; ++i) {
+            g = gamma(i);
+            const cdd y0te = g / c.mu[i], y0tm = c.eps[i] / g;
+            const cdd th = dctanh(c.width[i] * g);
+            ylte = (y0te * (ylte + y0te * th)) / (y0te + ylte * th);
+            yltm = (y0tm * (yltm + y0tm * th)) / (y0tm + yltm * th);
+        }
+        g = gamma(nl - 1);
+        cdd yrte = g / c.mu[nl - 1], yrtm = c.eps[nl - 1] / g;
+        for (int i = nl - 2; i >= s; --i) {
+            g = gamma(i);
+            const cdd y0te = g / c.mu[i], y0tm = c.eps[i] / g;
+            const cdd th = dctanh(c.width[i] * g);
+            yrte = (y0te * (yrte + y0te * th)) / (y0te + yrte * th);
+            yrtm = (y0tm * (yrtm + y0tm * th)) / (y0tm + yrtm * th);
+        }
+        t1 = (yltm + yrtm) - (c.f1 + x3 / kap2) / kap;
+        t2 = ((((yltm + yrtm) * k0sq) + ylte) + yrte) / b2 - (c.p1 + y3 / kap2) / kap;
     }
-    g = gamma(nl - 1);
-    cdd zrte = c.mu[nl - 1] / g, zrtm = g / c.eps[nl - 1];
-    for (int i = nl - 2; i >= s; --i) {
-        g = gamma(i);
-        const cdd z0te = c.mu[i] / g, z0tm = g / c.eps[i];
-        const cdd th = dctanh(c.width[i] * g);
-        zrte = (z0te * (zrte + z0te * th)) / (z0te + zrte * th);
-        zrtm = (z0tm * (zrtm + z0tm * th)) / (z0tm + zrtm * th);
-    }
-    cdd vte = (zlte * zrte) / (zlte + zrte);
-    cdd vtm = (zltm * zrtm) / (zltm + zrtm);
-    vte = (2.0 * vte) / c.mut;
-    const cdd c3 = cdd{dd{c.c3.x, c.dc3.x}, dd{c.c3.y, c.dc3.y}};
-    const cdd d3 = cdd{dd{c.d3.x, c.dd3.x}, dd{c.d3.y, c.dd3.y}};
-    const cdd t1 = vte - (1.0 + c3 / kap2) / kap;
-    vtm = (2.0 * c.epsb) * vtm;
-    const cdd t2 = (vtm + ((k0sq * c.epsb) * c.mut) * vte) / b2 - (1.0 + d3 / kap2) / kap;
     d1 = der(t1);
     d2 = der(t2);
 }

@@ -314,7 +314,7 @@ class DeviceSheet:
         return Z.view(np.complex128).reshape((nbf, nbf), order="F"), info
 
     def fill_dk0(self, pb: "PointBuilder"):
-        """(Z, dZ/dk0, info) at the point's phasing and u ('J' sheets); k0 in 1/m."""
+        """(Z, dZ/dk0, info) at the point's phasing and u; k0 in 1/m."""
         nbf = self.rwgdat.nbf
         Z = np.zeros(2 * nbf * nbf)
         dZ = np.zeros(2 * nbf * nbf)
