@@ -28,10 +28,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 for _sub in ("pssfss.jl_amd", "oracle"):
     sys.path.insert(0, os.path.join(ROOT, _sub))
 
-# HBM bytes per launch of the mixed spatial-sum kernel on this workload, from the
-# FETCH_SIZE / WRITE_SIZE passes in profiles/ring_on_slab_traffic_pmc.md (counters
-# calibrated with the 256 MiB copy of --calibrate copy: FETCH x2.00, WRITE x1.00).
-JK_MFMA_TRAFFIC_BYTES = 3161.9 * 1024 * 2.0 + 43239628.8 * 1024 * 1.0
+# HBM bytes per launch pair of the mixed spatial-sum kernels on this workload
+# (k_jk_mfma<1,2> and the half-width k_jk_mfma<1,1>), from the FETCH_SIZE /
+# WRITE_SIZE passes in profiles/ring_on_slab_traffic_pmc.md (counters calibrated
+# with the 256 MiB copy of --calibrate copy: FETCH x2.00, WRITE x1.00).
+JK_MFMA_TRAFFIC_BYTES = ((2775.5 + 558.5) * 1024 * 2.0
+                         + (40685444.3 + 2554272.0) * 1024 * 1.0)
 PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 (vector = matrix) / FP32 matrix
 
 

@@ -97,7 +97,8 @@ struct pssfss_ctx {
     std::string err;
     std::map<int, DBuf<cd>> dft;     // DFT matrices by size
     // reusable workspaces
-    DBuf<cd> wZ, wB, wX, wR, wft, wout, wjk;
+    DBuf<cd> wZ, wB, wX, wR, wft, wout;
+    DBuf<cd> wjk;                // spatial sums of the phasing keys of one fill call
     DBuf<cf> wZf, wRf;
     DBuf<int32_t> wpiv, wperm, wsel;
     DBuf<cd> wT;
@@ -604,9 +605,11 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     const double ul = pts[0].u / sh->upm;
     for (int p = 1; p < np; ++p)
         if (pts[p].u != pts[0].u) throw ArgError("batched points must share u");
-    DBuf<cd> slots;
+    // the spatial sums of every new phasing key go to the context's workspace,
+    // which keeps its size between calls (about 96 B x classes x phasings, 40 GB
+    // for a 102-point ring sweep: allocating and freeing it per call stalls)
     double key_u = ul;
-    std::vector<const cd *> jkp = resolve_jk(ctx, sh, pts, np, mixed, slots, key_u);
+    std::vector<const cd *> jkp = resolve_jk(ctx, sh, pts, np, mixed, ctx->wjk, key_u);
     ensure_singular(ctx, sh, key_u);
     std::vector<TabInput> tin(np);
     for (int p = 0; p < np; ++p) tin[p] = sheet_tab_input(sh, &pts[p]);
