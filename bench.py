@@ -75,7 +75,8 @@ def main():
         sys.exit(0 if rc == 0 else 1)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(points shard across GPUs independently; see DESIGN.md)", file=sys.stderr)
+              "(frequency points would shard across GPUs independently, one process "
+              "per GPU; this build runs on one)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
