@@ -36,83 +36,137 @@ __device__ __forceinline__ void face_dual(const double *fx, const double *ft, DV
 
 // ------------------------------------------------------------ spatial sums
 // One lane per (class, quadrature point), 9 classes x 7 points per wavefront; one
-// phasing.  Output: J, J_xi, J_eta, K, K_xi, K_eta per class as duals.
-__global__ __launch_bounds__(256) void k_jk_tan(SheetDev sh, int64_t nufp, double ul,
-                                                double us1x, double us1y, double us2x,
-                                                double us2y, double psi1, double psi2,
-                                                const double *ftan, cdd *out) {
-    __shared__ cdd sj[4][64], sk[4][64];
+// phasing.  Output per direction: J, J_xi, J_eta, K, K_xi, K_eta per class as duals.
+// The same spatial sums with the derivatives along up to ND directions at once:
+// the distances, exponentials and phases are those of the value and are shared;
+// direction d adds d(dist) = (r . dr_d) / |r| and the chain rule of each term.
+// ftan holds nd blocks of the faces' vertex directions (ftan_stride doubles
+// apart); out holds nd blocks of 6 nufp duals (out_stride apart), each as the
+// single-direction kernel writes them.
+template <int ND>
+__global__ __launch_bounds__(256) void k_jk_tan_multi(SheetDev sh, int64_t nufp, double ul,
+                                                      double us1x, double us1y, double us2x,
+                                                      double us2y, double psi1, double psi2,
+                                                      const double *ftan, int64_t ftan_stride,
+                                                      int nd, cdd *out, int64_t out_stride) {
+    __shared__ cd sj[4][64][ND + 1], sk[4][64][ND + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int qp = lane % 7;
     const int64_t iu = ((int64_t)blockIdx.x * 4 + wave) * 9 + lane / 7;
     const bool active = lane < 63 && iu < nufp;
-    cdd js = mkc(cd{0, 0}), ks = mkc(cd{0, 0});
+    cd js = {0, 0}, ks = {0, 0}, jsd[ND], ksd[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) jsd[d] = ksd[d] = cd{0, 0};
     if (active) {
         int64_t ifm, ifs;
         class_faces(sh, iu, ifm, ifs);
-        DVec rs[3], rm[3];
-        face_dual(sh.fxs + 6 * ifs, ftan + 6 * ifs, rs);
-        face_dual(sh.fxs + 6 * ifm, ftan + 6 * ifm, rm);
-        dd rmcx = ((rm[0].x + rm[1].x) + rm[2].x) / 3.0;
-        dd rmcy = ((rm[0].y + rm[1].y) + rm[2].y) / 3.0;
-        dd rtx = (rs[0].x + (rs[1].x - rs[0].x) * QXI[qp]) + (rs[2].x - rs[0].x) * QETA[qp];
-        dd rty = (rs[0].y + (rs[1].y - rs[0].y) * QXI[qp]) + (rs[2].y - rs[0].y) * QETA[qp];
-        dd ux = ul * (rmcx - rtx), uy = ul * (rmcy - rty);
-        dd arg = dsqrt(ux * ux + uy * uy);
-        dd rterm = dexp(-arg);
-        dd j0 = arg.v <= 1e-5 ? ((-arg / 6.0 + 0.5) * arg - 1.0) : (rterm - 1.0) / arg;
-        js = mkc(j0);
-        ks = mkc(rterm);
+        const double *fs = sh.fxs + 6 * ifs, *fm = sh.fxs + 6 * ifm;
+        const double rmcx = ((fm[0] + fm[2]) + fm[4]) / 3.0;
+        const double rmcy = ((fm[1] + fm[3]) + fm[5]) / 3.0;
+        const double rtx = (fs[0] + (fs[2] - fs[0]) * QXI[qp]) + (fs[4] - fs[0]) * QETA[qp];
+        const double rty = (fs[1] + (fs[3] - fs[1]) * QXI[qp]) + (fs[5] - fs[1]) * QETA[qp];
+        const double ux = ul * (rmcx - rtx), uy = ul * (rmcy - rty);
+        double dux[ND], duy[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            dux[d] = duy[d] = 0.0;
+            if (d < nd) {
+                const double *ts = ftan + d * ftan_stride + 6 * ifs;
+                const double *tm = ftan + d * ftan_stride + 6 * ifm;
+                const double dmx = ((tm[0] + tm[2]) + tm[4]) / 3.0;
+                const double dmy = ((tm[1] + tm[3]) + tm[5]) / 3.0;
+                const double dtx = (ts[0] + (ts[2] - ts[0]) * QXI[qp]) + (ts[4] - ts[0]) * QETA[qp];
+                const double dty = (ts[1] + (ts[3] - ts[1]) * QXI[qp]) + (ts[5] - ts[1]) * QETA[qp];
+                dux[d] = ul * (dmx - dtx);
+                duy[d] = ul * (dmy - dty);
+            }
+        }
+        // the extracted (0, 0) term
+        const double arg = sqrt(ux * ux + uy * uy);
+        const double rterm = exp(-arg);
+        const bool small = arg <= 1e-5;
+        const double j0 = small ? (-arg / 6.0 + 0.5) * arg - 1.0 : (rterm - 1.0) / arg;
+        const double dj0 = small ? -arg / 3.0 + 0.5 : (-rterm / arg) - (rterm - 1.0) / (arg * arg);
+        js = cd{j0, 0.0};
+        ks = cd{rterm, 0.0};
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const double darg = arg > 0 ? (ux * dux[d] + uy * duy[d]) / arg : 0.0;
+            jsd[d] = cd{dj0 * darg, 0.0};
+            ksd[d] = cd{-rterm * darg, 0.0};
+        }
         const bool zero_phase = psi1 == 0.0 && psi2 == 0.0;
         for (int r = 1; r <= JK_RINGMAX; ++r) {
-            cdd jr = mkc(cd{0, 0}), kr = mkc(cd{0, 0});
+            cd jr = {0, 0}, kr = {0, 0}, jrd[ND], krd[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) jrd[d] = krd[d] = cd{0, 0};
             for (int k = 0; k < 8 * r; ++k) {
                 int m, n;
                 ring_mn(r, k, m, n);
-                dd dx = ux - ((double)m * us1x + (double)n * us2x);
-                dd dy = uy - ((double)m * us1y + (double)n * us2y);
-                dd a = dsqrt(dx * dx + dy * dy);
-                dd e = dexp(-a);
-                dd ea = e / a;
-                if (zero_phase) {
-                    jr.x = jr.x + ea;
-                    kr.x = kr.x + e;
-                } else {
-                    cd ph = cisd(-((double)m * psi1 + (double)n * psi2));
-                    jr += ph * ea;
-                    kr += ph * e;
+                const double dx = ux - ((double)m * us1x + (double)n * us2x);
+                const double dy = uy - ((double)m * us1y + (double)n * us2y);
+                const double a = sqrt(dx * dx + dy * dy);
+                const double e = exp(-a);
+                const double ea = e / a;
+                // d(ea) = -(e + ea) da / a, d(e) = -e da, da = (dx ddx + dy ddy) / a
+                const double ia = 1.0 / a;
+                const double fj = -(e + ea) * ia * ia, fk = -e * ia;
+                cd ph = {1.0, 0.0};
+                if (!zero_phase) ph = cisd(-((double)m * psi1 + (double)n * psi2));
+                jr += ph * ea;
+                kr += ph * e;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const double dd_ = dx * dux[d] + dy * duy[d];
+                    jrd[d] += ph * (fj * dd_);
+                    krd[d] += ph * (fk * dd_);
                 }
             }
             js += jr;
             ks += kr;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                jsd[d] += jrd[d];
+                ksd[d] += krd[d];
+            }
             if (r > 5) {
-                double c1 = cabs(val(jr) / val(js)), c2 = cabs(val(kr) / val(ks));
+                double c1 = cabs(jr / js), c2 = cabs(kr / ks);
                 if ((c1 > c2 ? c1 : c2) < 1e-8) break;
             }
         }
     }
-    sj[wave][lane] = js;
-    sk[wave][lane] = ks;
+    sj[wave][lane][0] = js;
+    sk[wave][lane][0] = ks;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        sj[wave][lane][d + 1] = jsd[d];
+        sk[wave][lane][d + 1] = ksd[d];
+    }
     __syncthreads();
     if (active && qp == 0) {
         const int base = lane;        // lanes base..base+6 hold this class
-        cdd J = mkc(cd{0, 0}), Jx = J, Je = J, K = J, Kx = J, Ke = J;
-        for (int q = 0; q < 7; ++q) {
-            cdd jw = sj[wave][base + q] * QW[q];
-            Jx += jw * QXI[q];
-            Je += jw * QETA[q];
-            J += jw;
-            cdd kw = sk[wave][base + q] * QW[q];
-            Kx += kw * QXI[q];
-            Ke += kw * QETA[q];
-            K += kw;
+        for (int d = 0; d < nd; ++d) {
+            cdd J = mkc(cd{0, 0}), Jx = J, Je = J, K = J, Kx = J, Ke = J;
+            for (int q = 0; q < 7; ++q) {
+                const cd jv = sj[wave][base + q][0], jt = sj[wave][base + q][d + 1];
+                const cd kv = sk[wave][base + q][0], kt = sk[wave][base + q][d + 1];
+                cdd jw = cdd{dd{jv.x, jt.x}, dd{jv.y, jt.y}} * QW[q];
+                Jx += jw * QXI[q];
+                Je += jw * QETA[q];
+                J += jw;
+                cdd kw = cdd{dd{kv.x, kt.x}, dd{kv.y, kt.y}} * QW[q];
+                Kx += kw * QXI[q];
+                Ke += kw * QETA[q];
+                K += kw;
+            }
+            cdd *o = out + d * out_stride;
+            o[0 * nufp + iu] = J;
+            o[1 * nufp + iu] = Jx;
+            o[2 * nufp + iu] = Je;
+            o[3 * nufp + iu] = K;
+            o[4 * nufp + iu] = Kx;
+            o[5 * nufp + iu] = Ke;
         }
-        out[0 * nufp + iu] = J;
-        out[1 * nufp + iu] = Jx;
-        out[2 * nufp + iu] = Je;
-        out[3 * nufp + iu] = K;
-        out[4 * nufp + iu] = Kx;
-        out[5 * nufp + iu] = Ke;
     }
 }
 
@@ -429,11 +483,14 @@ __global__ void k_obs(int64_t nbf, const cd *ft, const int32_t *col_job, const d
 }  // namespace
 
 // ===================================================================== launchers
-void launch_jk_tan(const SheetDev &sh, double ul, const double *us1, const double *us2,
-                   double psi1, double psi2, const double *ftan_s, cdd *out, hipStream_t st) {
+void launch_jk_tan_multi(const SheetDev &sh, double ul, const double *us1, const double *us2,
+                         double psi1, double psi2, const double *ftan_s, int64_t ftan_stride,
+                         int nd, cdd *out, int64_t out_stride, hipStream_t st) {
+    if (nd < 1 || nd > 4) throw ArgError("adjoint: 1..4 directions per tangent pass");
     dim3 grid((unsigned)((sh.nufp + 35) / 36));
-    hipLaunchKernelGGL(k_jk_tan, grid, dim3(256), 0, st, sh, sh.nufp, ul, us1[0], us1[1],
-                       us2[0], us2[1], psi1, psi2, ftan_s, out);
+    hipLaunchKernelGGL(k_jk_tan_multi<4>, grid, dim3(256), 0, st, sh, sh.nufp, ul, us1[0],
+                       us1[1], us2[0], us2[1], psi1, psi2, ftan_s, ftan_stride, nd, out,
+                       out_stride);
     HIPCHECK(hipGetLastError());
 }
 

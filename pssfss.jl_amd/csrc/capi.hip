@@ -1075,74 +1075,95 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
     dA.upload(hA.data(), hA.size(), st);
     DBuf<FillParams> dfp(np);
     dfp.upload(fp.data(), np, st);
-    // per direction: tangent spatial sums / singular terms, class contraction,
-    // transform derivatives
+    // per group of up to ADJ_NDIR directions: tangent spatial sums (all directions
+    // of the group in one pass per phasing key) and singular terms, class
+    // contraction, transform derivatives
     const int64_t nu = sh->nufp;
-    DBuf<double> fts(6 * nf), ftm(6 * nf);
-    DBuf<cdd> jkd((size_t)6 * nu);
-    DBuf<dd> rrd((size_t)2 * nu), rid((size_t)nu);
-    DBuf<cd> part((size_t)((nu + 255) / 256) * nobj), zout((size_t)nobj * np);
-    DBuf<cd> dft((size_t)njobs * n * 2);
-    std::vector<double> hts(6 * nf), htm(6 * nf);
-    std::vector<cd> hz((size_t)nobj * np), hdft((size_t)njobs * n * 2);
+    constexpr int ADJ_NDIR = 4;
+    const int64_t sft = 6 * nf, sjk = 6 * nu, sdft = (int64_t)njobs * n * 2;
+    DBuf<double> fts((size_t)ADJ_NDIR * sft), ftm((size_t)ADJ_NDIR * sft);
+    DBuf<cdd> jkd((size_t)ADJ_NDIR * sjk);
+    DBuf<dd> rrd((size_t)ADJ_NDIR * 2 * nu), rid((size_t)ADJ_NDIR * nu);
+    DBuf<cd> part((size_t)((nu + 255) / 256) * nobj), zout((size_t)ADJ_NDIR * nobj * np);
+    DBuf<cd> dft((size_t)ADJ_NDIR * sdft);
+    std::vector<double> hts((size_t)ADJ_NDIR * sft), htm((size_t)ADJ_NDIR * sft);
+    std::vector<cd> hz((size_t)ADJ_NDIR * nobj * np), hdft((size_t)ADJ_NDIR * sdft);
     const double us1[2] = {ul * sh->s1[0], ul * sh->s1[1]};
     const double us2[2] = {ul * sh->s2[0], ul * sh->s2[1]};
-    for (int k = 0; k < nparam; ++k) {
-        const double *vt = vtan + (size_t)k * 2 * sh->nnode;
-        for (int64_t f = 0; f < nf; ++f)
-            for (int a = 0; a < 3; ++a) {
-                const int v = sh->hfv[3 * f + a];
-                hts[6 * f + 2 * a] = vt[2 * v];
-                hts[6 * f + 2 * a + 1] = vt[2 * v + 1];
-                htm[6 * f + 2 * a] = vt[2 * v] / sh->upm;
-                htm[6 * f + 2 * a + 1] = vt[2 * v + 1] / sh->upm;
-            }
-        fts.upload(hts.data(), hts.size(), st);
-        ftm.upload(htm.data(), htm.size(), st);
-        launch_singular_tan(sh->dev(), ul, fts.p, rrd.p, rid.p, st);
+    for (int k0 = 0; k0 < nparam; k0 += ADJ_NDIR) {
+        const int nd = std::min(ADJ_NDIR, nparam - k0);
+        for (int d = 0; d < nd; ++d) {
+            const double *vt = vtan + (size_t)(k0 + d) * 2 * sh->nnode;
+            double *ts = &hts[d * sft], *tm = &htm[d * sft];
+            for (int64_t f = 0; f < nf; ++f)
+                for (int a = 0; a < 3; ++a) {
+                    const int v = sh->hfv[3 * f + a];
+                    ts[6 * f + 2 * a] = vt[2 * v];
+                    ts[6 * f + 2 * a + 1] = vt[2 * v + 1];
+                    tm[6 * f + 2 * a] = vt[2 * v] / sh->upm;
+                    tm[6 * f + 2 * a + 1] = vt[2 * v + 1] / sh->upm;
+                }
+        }
+        fts.upload(hts.data(), (size_t)nd * sft, st);
+        ftm.upload(htm.data(), (size_t)nd * sft, st);
+        for (int d = 0; d < nd; ++d)
+            launch_singular_tan(sh->dev(), ul, fts.p + d * sft, rrd.p + d * 2 * nu,
+                                rid.p + d * nu, st);
         double kpsi1 = NAN, kpsi2 = NAN;
         for (int p = 0; p < np; ++p) {
             if (!(pts[p].psi1 == kpsi1 && pts[p].psi2 == kpsi2)) {
                 kpsi1 = pts[p].psi1;
                 kpsi2 = pts[p].psi2;
-                launch_jk_tan(sh->dev(), ul, us1, us2, kpsi1, kpsi2, fts.p, jkd.p, st);
+                launch_jk_tan_multi(sh->dev(), ul, us1, us2, kpsi1, kpsi2, fts.p, sft, nd,
+                                    jkd.p, sjk, st);
             }
-            launch_class_tan(sh->dev(), dfp.p + p, jkd.p, rrd.p, rid.p, ftm.p,
-                             dA.p + (size_t)p * nobj * n * ncol, n * ncol,
-                             ctx->wB.p + (size_t)p * n * ncol, ncol, nobj, part.p,
-                             zout.p + (size_t)p * nobj, st);
+            for (int d = 0; d < nd; ++d)
+                launch_class_tan(sh->dev(), dfp.p + p, jkd.p + d * sjk, rrd.p + d * 2 * nu,
+                                 rid.p + d * nu, ftm.p + d * sft,
+                                 dA.p + (size_t)p * nobj * n * ncol, n * ncol,
+                                 ctx->wB.p + (size_t)p * n * ncol, ncol, nobj, part.p,
+                                 zout.p + ((size_t)d * np + p) * nobj, st);
         }
-        launch_bfft_tan(sh->dev(), sh->bf_face.p, sh->bf_loc.p, djobs.p, njobs, ftm.p, dft.p, st);
-        d2h(ctx, hz.data(), zout.p, hz.size());
-        d2h(ctx, hdft.data(), dft.p, hdft.size());
+        for (int d = 0; d < nd; ++d)
+            launch_bfft_tan(sh->dev(), sh->bf_face.p, sh->bf_loc.p, djobs.p, njobs,
+                            ftm.p + d * sft, dft.p + d * sdft, st);
+        d2h(ctx, hz.data(), zout.p, (size_t)nd * nobj * np);
+        d2h(ctx, hdft.data(), dft.p, (size_t)nd * sdft);
         sync(ctx);
-        for (int p = 0; p < np; ++p) {
-            const cd *X = &hX[(size_t)p * n * ncol];
-            const cd *L = &hL[(size_t)p * n * ncol];
-            // explicit terms per (observed, source) pair
-            std::vector<cd> ex((size_t)wsz);
-            for (int oc = 0; oc < ncol; ++oc) {
-                const cd *dg = &hdft[(size_t)jobof[p][oc] * n * 2];
-                const double tx = colt[2 * (p * ncol + oc)], ty = colt[2 * (p * ncol + oc) + 1];
-                for (int c = 0; c < ncol; ++c) {
-                    const cd *dv = &hdft[(size_t)jobof[p][c] * n * 2];
-                    const cd s0 = colsrc[2 * (p * ncol + c)], s1 = colsrc[2 * (p * ncol + c) + 1];
-                    cd t = {0, 0};
-                    for (int64_t m = 0; m < n; ++m) {
-                        cd g = tx * dg[2 * m] + ty * dg[2 * m + 1];
-                        cd v = conj(dv[2 * m]) * s0 + conj(dv[2 * m + 1]) * s1;
-                        t += g * X[m + n * c] + L[m + n * oc] * v;
+        for (int d = 0; d < nd; ++d) {
+            const int k = k0 + d;
+            const cd *hzd = &hz[(size_t)d * nobj * np];
+            const cd *hdftd = &hdft[(size_t)d * sdft];
+            for (int p = 0; p < np; ++p) {
+                const cd *X = &hX[(size_t)p * n * ncol];
+                const cd *L = &hL[(size_t)p * n * ncol];
+                // explicit terms per (observed, source) pair
+                std::vector<cd> ex((size_t)wsz);
+                for (int oc = 0; oc < ncol; ++oc) {
+                    const cd *dg = &hdftd[(size_t)jobof[p][oc] * n * 2];
+                    const double tx = colt[2 * (p * ncol + oc)];
+                    const double ty = colt[2 * (p * ncol + oc) + 1];
+                    for (int c = 0; c < ncol; ++c) {
+                        const cd *dv = &hdftd[(size_t)jobof[p][c] * n * 2];
+                        const cd s0 = colsrc[2 * (p * ncol + c)];
+                        const cd s1 = colsrc[2 * (p * ncol + c) + 1];
+                        cd t = {0, 0};
+                        for (int64_t m = 0; m < n; ++m) {
+                            cd g = tx * dg[2 * m] + ty * dg[2 * m + 1];
+                            cd v = conj(dv[2 * m]) * s0 + conj(dv[2 * m + 1]) * s1;
+                            t += g * X[m + n * c] + L[m + n * oc] * v;
+                        }
+                        ex[oc * ncol + c] = t;
                     }
-                    ex[oc * ncol + c] = t;
                 }
-            }
-            for (int o = 0; o < nobj; ++o) {
-                const cd *Wo = &Wk[((size_t)p * nobj + o) * wsz];
-                cd gsum = hz[(size_t)p * nobj + o];
-                for (int64_t e = 0; e < wsz; ++e) gsum = gsum - Wo[e] * ex[e];
-                double *gp = grad + 2 * (((size_t)p * nobj + o) * nparam + k);
-                gp[0] = gsum.x;
-                gp[1] = gsum.y;
+                for (int o = 0; o < nobj; ++o) {
+                    const cd *Wo = &Wk[((size_t)p * nobj + o) * wsz];
+                    cd gsum = hzd[(size_t)p * nobj + o];
+                    for (int64_t e = 0; e < wsz; ++e) gsum = gsum - Wo[e] * ex[e];
+                    double *gp = grad + 2 * (((size_t)p * nobj + o) * nparam + k);
+                    gp[0] = gsum.x;
+                    gp[1] = gsum.y;
+                }
             }
         }
     }

@@ -302,8 +302,11 @@ inline void stage_flops(int s, double flops) {
 // ---- adjoint.hip (forward-mode directions through the fill; transposed solves)
 // ftan_s / ftan_m: 6 per face, derivative of the face's vertex coordinates along
 // the direction (sheet units / metres).
-void launch_jk_tan(const SheetDev &sh, double ul, const double *us1, const double *us2,
-                   double psi1, double psi2, const double *ftan_s, cdd *out, hipStream_t st);
+// Tangent spatial sums (J, J_xi, J_eta, K, K_xi, K_eta as duals) along nd <= 4
+// directions in one pass: ftan and out hold nd blocks.
+void launch_jk_tan_multi(const SheetDev &sh, double ul, const double *us1, const double *us2,
+                         double psi1, double psi2, const double *ftan_s, int64_t ftan_stride,
+                         int nd, cdd *out, int64_t out_stride, hipStream_t st);
 void launch_singular_tan(const SheetDev &sh, double ul, const double *ftan_s, dd *rho_r,
                          dd *rinv, hipStream_t st);
 // out[o] = sum_{m,s} dZ_ms sum_c A[o*sAo + m + nbf*c] X[s + nbf*c]; part: one slot per
