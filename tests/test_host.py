@@ -8,6 +8,8 @@ import re
 import numpy as np
 import pytest
 
+from conftest import gpu_available
+
 
 def test_rings(pk):
     """test/Rings_test.jl."""
@@ -231,3 +233,14 @@ def test_cross_batch_generator(pk):
         assert 700 <= sa[1].nface <= 4500
         assert len(fa) == 64 and fa[0] == 1.0 and fa[-1] == 30.0
         assert st["theta"] == [0.0, 15.0, 30.0, 45.0]
+
+@pytest.mark.skipif(gpu_available(), reason="checks the status without a device")
+def test_context_without_device(pk):
+    """Without a GPU, context creation fails with PSSFSS_EHIP and no handle, and a
+    null context is reported as such (no crash, no fallback)."""
+    import ctypes as C
+    from pssfss_amd import hip
+    h = C.c_void_p()
+    rc = hip.lib().pssfss_ctx_create(0, C.byref(h))
+    assert rc == -2 and h.value is None
+    assert hip.lib().pssfss_last_error(None) == b"null context"
