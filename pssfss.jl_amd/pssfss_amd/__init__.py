@@ -7,6 +7,7 @@ from . import constants, rings, layers, sheets, rwg, modes, gsm, outputs, analyz
 from . import adjoint  # noqa: F401
 from .outputs import Result, outputs as output_functions, extract_result  # noqa: F401
 from .layers import Layer, TE, TM  # noqa: F401
-from .sheets import RWGSheet, rectstrip, recttri, annular_ring  # noqa: F401
+from .sheets import (RWGSheet, rectstrip, recttri, annular_ring, polyring, meander,  # noqa: F401
+                     loadedcross, jerusalemcross, combine)
 from .rwg import setup_rwg, RWGData  # noqa: F401
 from .gsm import GSM, cascade, cascade_layer  # noqa: F401

@@ -345,6 +345,114 @@ def annular_ring(*, a, b, P, nr, nphi, units, sheet_class="J", phi0=0.0, fufp=Fa
     return sh
 
 
+def _zip_loops(inner, outer, faces):
+    """Triangulate the strip between two closed node loops (lists of node indices in
+    counter-clockwise order, both starting on the same ray): advance along the loop
+    whose next node comes first in the normalised loop parameter."""
+    na, nb = len(inner), len(outer)
+    i = j = 0
+    while i < na or j < nb:
+        ua = (i + 1) / na if i < na else 2.0
+        ub = (j + 1) / nb if j < nb else 2.0
+        if ub <= ua:
+            faces.append((inner[i % na], outer[(j + 1) % nb], outer[j % nb]))
+            j += 1
+        else:
+            faces.append((inner[i % na], inner[(i + 1) % na], outer[j % nb]))
+            i += 1
+
+
+def polyring(*, s1, s2, a, b, sides: int, ntri: int, units, orient=0.0, sheet_class="J",
+             dx=0.0, dy=0.0, rot=0.0, Rsheet=0.0, fufp=False) -> RWGSheet:
+    """Concentric annular regions bounded by regular polygons (src/Elements.jl:368-558):
+    ring i lies between the polygons of circumradius a[i] and b[i] with `sides`
+    sides, the first vertex at angle `orient` (degrees) from the cell centre
+    (s1 + s2) / 2; a[0] = 0 makes the first ring a solid polygon.  The arguments
+    are checked as the reference checks them.
+
+    The reference meshes the rings with Triangle (src/Meshsub.jl:42-111), which is
+    absent here.  This is our own structured mesher, so mesh parity with the
+    reference is unpinned.  Each ring is a stack of similar polygons whose node
+    counts grow with their perimeter.  The strips between consecutive polygons are
+    zipped into triangles (a fan around the centre for a solid polygon), with a
+    target triangle area of (total metal area) / ntri.  The option of an outer ring
+    filling the cell to its boundary (b[-1] < 0) is not supported by this mesher."""
+    if not sides >= 3:
+        raise ValueError("sides must be at least 3")
+    if not ntri > 0:
+        raise ValueError("ntri must be positive")
+    if len(s1) != 2 or len(s2) != 2:
+        raise ValueError("s1 and s2 must have length 2")
+    a = [float(v) for v in a]
+    b = [float(v) for v in b]
+    if len(a) != len(b):
+        raise ValueError("length(a) !== length(b)")
+    nring = len(a)
+    if b[-1] < 0:
+        raise ValueError("an outer ring filling the unit cell (b[end] < 0) is not supported "
+                         "by the structured mesher")
+    for i in range(nring):
+        if a[i] >= b[i]:
+            raise ValueError(f"a[{i + 1}] >= b[{i + 1}]")
+        if a[i] < 0:
+            raise ValueError("radii must be non-negative")
+    for i in range(nring - 1):
+        if b[i] >= a[i + 1]:
+            raise ValueError(f"b[{i + 1}] >= a[{i + 2}]")
+    s1 = np.asarray(s1, float)
+    s2 = np.asarray(s2, float)
+    c = 0.5 * (s1 + s2)
+    alpha = 2.0 * math.pi / sides
+    t0 = math.radians(orient)
+    area_factor = 0.5 * sides * math.sin(alpha)
+    area = sum((bb * bb - aa * aa) * area_factor for aa, bb in zip(a, b))
+    h = math.sqrt(2.0 * area / ntri)           # leg of a right isosceles target triangle
+    side_len = 2.0 * math.sin(0.5 * alpha)     # side length per unit circumradius
+    apothem = math.cos(0.5 * alpha)
+    nodes = []
+
+    def loop(R, m):
+        """m nodes per side on the polygon of circumradius R, first at the first vertex."""
+        ids = []
+        for k in range(sides):
+            va = np.array([math.cos(t0 + k * alpha), math.sin(t0 + k * alpha)]) * R
+            vb = np.array([math.cos(t0 + (k + 1) * alpha), math.sin(t0 + (k + 1) * alpha)]) * R
+            for q in range(m):
+                ids.append(len(nodes))
+                nodes.append(c + va + (vb - va) * (q / m))
+        return ids
+
+    faces = []
+    for aa, bb in zip(a, b):
+        nr = max(1, int(round((bb - aa) * apothem / h)))
+        radii = [aa + (bb - aa) * j / nr for j in range(nr + 1)]
+        prev = None
+        for R in radii:
+            if R == 0.0:
+                prev = [len(nodes)]
+                nodes.append(c.copy())
+                continue
+            m = max(1, int(round(R * side_len / h)))
+            cur = loop(R, m)
+            if prev is not None:
+                if len(prev) == 1:
+                    for q in range(len(cur)):
+                        faces.append((prev[0], cur[q], cur[(q + 1) % len(cur)]))
+                else:
+                    _zip_loops(prev, cur, faces)
+            prev = cur
+    rho = np.asarray(nodes)
+    fv = np.asarray(faces, np.int64).T.copy()
+    # counter-clockwise faces
+    p0, p1, p2 = rho[fv[0]], rho[fv[1]], rho[fv[2]]
+    cross = ((p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1])
+             - (p1[:, 1] - p0[:, 1]) * (p2[:, 0] - p0[:, 0]))
+    flip = cross < 0
+    fv[1, flip], fv[2, flip] = fv[2, flip].copy(), fv[1, flip].copy()
+    return _element_sheet("polyring", rho, fv, s1, s2, units, sheet_class, dx, dy, rot, Rsheet,
+                          fufp)
+
+
 def _rect_union_mesh(solid, hollow, ntri: int):
     """Structured triangulation of a rectilinear metal region: the union of the
     axis-aligned rectangles `solid` minus the union of `hollow`, each given as

@@ -359,6 +359,28 @@ def _grating(kP, nterms=30):
     return G, 1 + G
 
 
+def test_analyze_polyring(ctx, pk, orc):
+    """A hexagonal ring (structured polyring mesh) on a slab, normal and oblique
+    incidence, against the oracle."""
+    _say("analyze hexagonal polyring on a slab, theta 0/20, 3 freqs")
+    L = pk.Layer.make
+    sh = pk.polyring(s1=[9, 0], s2=[0, 9], a=[2.0], b=[3.0], sides=6, ntri=300, units="mm",
+                     orient=15)
+    strata = [L(), sh, L(width=0.5, units="mm", epsr=2.5), L()]
+    freqs = [8.0, 14.0, 20.0]
+    steer = {"phi": [0.0], "theta": [0.0, 20.0]}
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    assert len(res) == len(ref) == 6
+    d = _max_dS(res, ref)
+    print(f"  faces {sh.nface}: max |dS| fp64 = {d:.3e}")
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
 @pytest.mark.parametrize("cls", ["J", "M"])
 def test_grating_closed_form(ctx, pk, cls):
     _say(f"symmetric strip grating vs Collin closed form, class {cls}")

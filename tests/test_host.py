@@ -219,6 +219,46 @@ def test_cross_meshes_area_and_conformity(pk, kind):
     assert rw.nbf == int((cnt == 2).sum())
 
 
+@pytest.mark.parametrize("a,b,sides,ntri", [([0.0], [3.0], 4, 200), ([1.0], [2.0], 6, 500),
+                                             ([0.0, 2.0], [1.0, 3.0], 8, 800),
+                                             ([1.0, 2.5], [2.0, 3.5], 3, 600)])
+def test_polyring_meshes(pk, a, b, sides, ntri):
+    """Structured polyring meshes: the triangle areas add up to the polygon rings'
+    closed-form area (sides/2 sin(2 pi/sides) (b^2 - a^2) each), every triangle is
+    counter-clockwise, every edge is shared by at most two faces, and the Euler
+    characteristic V - E + F counts one per solid polygon and zero per annulus."""
+    sh = pk.polyring(s1=[10, 0], s2=[0, 10], a=a, b=b, sides=sides, ntri=ntri, units="mm")
+    p = sh.rho[sh.fv]
+    ar = 0.5 * ((p[1, :, 0] - p[0, :, 0]) * (p[2, :, 1] - p[0, :, 1])
+                - (p[2, :, 0] - p[0, :, 0]) * (p[1, :, 1] - p[0, :, 1]))
+    exact = sum(0.5 * sides * math.sin(2 * math.pi / sides) * (bb * bb - aa * aa)
+                for aa, bb in zip(a, b))
+    assert (ar > 0).all()
+    assert abs(ar.sum() - exact) < 1e-12 * exact
+    assert 0.5 * ntri <= sh.nface <= 2 * ntri
+    cnt = np.bincount(sh.fe.reshape(-1), minlength=sh.nedge)
+    assert cnt.max() <= 2 and cnt.min() >= 1
+    assert sh.rho.shape[0] - sh.nedge + sh.nface == sum(1 for aa in a if aa == 0)
+    # every node on its polygon: |x - c| projected on the vertex directions
+    c = np.array([5.0, 5.0])
+    r = np.linalg.norm(sh.rho - c, axis=1)
+    assert r.max() <= max(b) + 1e-12
+    rw = pk.rwg.setup_rwg(sh)
+    assert rw.nbf == int((cnt == 2).sum())
+
+
+@pytest.mark.parametrize("kw,msg", [(dict(a=[1.0], b=[1.0]), "a\\[1\\] >= b\\[1\\]"),
+                                    (dict(a=[0.0, 1.0], b=[1.5, 2.0]), "b\\[1\\] >= a\\[2\\]"),
+                                    (dict(a=[0.0], b=[1.0, 2.0]), "length"),
+                                    (dict(a=[1.0], b=[-4.0]), "not supported"),
+                                    (dict(a=[0.0], b=[1.0], sides=2), "sides")])
+def test_polyring_argument_checks(pk, kw, msg):
+    args = dict(s1=[10, 0], s2=[0, 10], sides=6, ntri=100, units="mm")
+    args.update(kw)
+    with pytest.raises(ValueError, match=msg):
+        pk.polyring(**args)
+
+
 def test_cross_batch_generator(pk):
     """Seeded generator: reproducible, parameters inside the stated ranges, meshes
     between about 800 and 4000 faces."""
