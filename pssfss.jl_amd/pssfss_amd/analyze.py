@@ -233,16 +233,22 @@ def _check_inputs(layers):
 
 
 def analyze(strata, flist, steering, precision: str = "fp64", mode_override=None,
-            ctx=None, return_stats: bool = False, dbmin: float = DBMIN):
+            ctx=None, return_stats: bool = False, dbmin: float = DBMIN, outlist=()):
     """Analyze the structure over all steering values and frequencies on the GPU.
 
     Returns a list of Result records (gsm, FGHz, steering, beta00, ...) in the
     reference's loop order.  precision: "fp64" (plain FP64 path) or "mixed" (FP32 LU
-    and smooth sums, FP64 refinement).  Face-pair classes are formed on the device."""
+    and smooth sums, FP64 refinement).  Face-pair classes are formed on the device.
+    outlist: (CSV file name, output functors) pairs as the reference's analyze
+    takes them (src/PSSFSS.jl:67-110), e.g. [("s21.csv", outputs("FGHz",
+    "s21db(te,te)"))]; each file gets a header of labels and one row per result."""
     from .hip import SheetSolver, get_context, face_pair_classes
+    from .outputs import write_outputs_csv
     ctx = ctx or get_context()
     setup = prepare(strata, flist, steering, mode_override=mode_override, dbmin=dbmin,
                     classify=lambda sh: face_pair_classes(ctx, sh))
     solver = SheetSolver(ctx, setup, precision)
     results, stats = solver.run()
+    for path, ops in outlist:
+        write_outputs_csv(path, results, ops)
     return (results, stats) if return_stats else results

@@ -267,3 +267,27 @@ def extract_result(results: Union[Result, Sequence[Result]], ops: Sequence[Outfu
     if isinstance(results, Result):
         return np.array([[o(results) for o in ops]])
     return np.array([[o(r) for o in ops] for r in results])
+
+
+def _csv_value(v) -> str:
+    """One CSV field as the reference's writedlm prints it: shortest round-trip
+    reals, complex values as "re + imim" / "re - imim"."""
+    if isinstance(v, complex) or isinstance(v, np.complexfloating):
+        re_, im_ = float(v.real), float(v.imag)
+        sign = "-" if math.copysign(1.0, im_) < 0 else "+"
+        return f"{re_!r} {sign} {abs(im_)!r}im"
+    if isinstance(v, (bool, np.bool_)):
+        return "true" if v else "false"
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    return repr(float(v))
+
+
+def write_outputs_csv(path: str, results: Sequence[Result], ops: Sequence[Outfun]) -> None:
+    """The CSV file of one `outlist` entry of analyze (src/PSSFSS.jl:316-324): a
+    header row of the outputs' labels, then one row of their values per result, in
+    the analysis loop's order, comma-separated."""
+    with open(path, "w") as io:
+        io.write(",".join(o.label for o in ops) + "\n")
+        for r in results:
+            io.write(",".join(_csv_value(o(r)) for o in ops) + "\n")

@@ -98,3 +98,23 @@ def test_named_outputs_and_steering(pk):
     assert abs(row[6]) < 1e-12
     with pytest.raises(ValueError):
         outputs("s31db(te,te)")
+
+
+def test_outputs_csv(pk, tmp_path):
+    """The outlist CSV format of the reference (src/PSSFSS.jl:316-324, writedlm with
+    ','): a header of output labels, then one row per result; complex entries as
+    Julia prints them."""
+    from pssfss_amd.outputs import outputs, write_outputs_csv, extract_result
+    rs = [_result(pk, np.zeros((2, 2)), np.eye(2) * (0.5 + 0.25j), 0.0, 0.0, f)
+          for f in (10.0, 12.5)]
+    ops = outputs("FGHz", "s21(te,te)", "s21db(te,te)", "s11mag(tm,tm)")
+    path = tmp_path / "out.csv"
+    write_outputs_csv(str(path), rs, ops)
+    lines = path.read_text().splitlines()
+    assert lines[0] == ",".join(o.label for o in ops)
+    assert len(lines) == 3
+    f = lines[1].split(",")
+    assert f[0] == "10.0" and f[1] == "0.5 + 0.25im" and f[3] == "0.0"
+    assert abs(float(f[2]) - 10 * math.log10(0.5 ** 2 + 0.25 ** 2)) < 1e-12
+    vals = extract_result(rs, ops)
+    assert float(lines[2].split(",")[0]) == vals[1, 0].real == 12.5
