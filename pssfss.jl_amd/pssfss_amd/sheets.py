@@ -65,6 +65,40 @@ class RWGSheet:
         return units_per_meter(self.units)
 
 
+_SHEET_ARRAYS = ("s1", "s2", "beta1", "beta2", "rho", "e1", "e2", "fv", "fe", "fr")
+_SHEET_SCALARS = ("style", "units", "dx", "dy", "rot", "psi1", "psi2", "u", "sheet_class",
+                  "info", "xieta_check", "fufp")
+
+
+def write_sheet_data(filename: str, sheet: RWGSheet) -> None:
+    """Write a sheet's geometry to `filename` (src/Sheets.jl:108-117 writes a JLD2
+    file, which needs Julia): a NumPy .npz archive of the RWGSheet fields (indices
+    0-based as stored here), readable without pickling."""
+    data = {k: np.asarray(getattr(sheet, k)) for k in _SHEET_ARRAYS}
+    for k in _SHEET_SCALARS:
+        data[k] = np.asarray(getattr(sheet, k))
+    with open(filename, "wb") as f:
+        np.savez(f, **data)
+
+
+def read_sheet_data(filename: str) -> RWGSheet:
+    """Read a sheet written by write_sheet_data (src/Sheets.jl:94-106)."""
+    with np.load(filename, allow_pickle=False) as z:
+        missing = [k for k in _SHEET_ARRAYS + _SHEET_SCALARS if k not in z.files]
+        if missing:
+            raise ValueError(f"{filename} does not contain sheet data (missing {missing})")
+        sh = RWGSheet()
+        for k in _SHEET_ARRAYS:
+            setattr(sh, k, z[k].copy())
+        for k in ("style", "units", "sheet_class", "info"):
+            setattr(sh, k, str(z[k]))
+        for k in ("dx", "dy", "rot", "psi1", "psi2", "u"):
+            setattr(sh, k, float(z[k]))
+        for k in ("xieta_check", "fufp"):
+            setattr(sh, k, bool(z[k]))
+    return sh
+
+
 def recttri(rhobl, rhotr, nx: int, ny: int) -> RWGSheet:
     """Structured triangulation of a rectangle (src/Sheets.jl:398-480)."""
     rhobl = np.asarray(rhobl, float)

@@ -284,3 +284,22 @@ def test_context_without_device(pk):
     rc = hip.lib().pssfss_ctx_create(0, C.byref(h))
     assert rc == -2 and h.value is None
     assert hip.lib().pssfss_last_error(None) == b"null context"
+
+
+def test_sheet_data_round_trip(pk, tmp_path):
+    """write_sheet_data / read_sheet_data keep every field, and the read sheet
+    gives the same RWG data."""
+    sh = pk.sheets.meander(a=6.0, b=10.0, h=2.5, w1=0.3, w2=0.3, ntri=600, units="mm")
+    path = str(tmp_path / "sheet.npz")
+    pk.sheets.write_sheet_data(path, sh)
+    sh2 = pk.sheets.read_sheet_data(path)
+    for k in pk.sheets._SHEET_ARRAYS:
+        assert np.array_equal(getattr(sh, k), getattr(sh2, k)), k
+    for k in pk.sheets._SHEET_SCALARS:
+        assert getattr(sh, k) == getattr(sh2, k), k
+    r1, r2 = pk.setup_rwg(sh), pk.setup_rwg(sh2)
+    assert r1.nbf == r2.nbf and r1.nufp == r2.nufp
+    assert np.array_equal(r1.bff, r2.bff) and np.array_equal(r1.ebf, r2.ebf)
+    np.savez(str(tmp_path / "other.npz"), x=np.zeros(3))
+    with pytest.raises(ValueError, match="does not contain sheet data"):
+        pk.sheets.read_sheet_data(str(tmp_path / "other.npz"))
