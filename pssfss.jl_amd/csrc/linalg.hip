@@ -339,8 +339,11 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
                 const C l = nz ? x[r][j] * rp : x[r][j];
                 x[r][j] = l;
                 const C ml = -l;
+                // at LW = 16 a constant trip count lets this loop unroll fully
+                // once the column loop has (else x is demoted to scratch memory)
 #pragma unroll
-                for (int c = j + 1; c < LW; ++c) cfma(x[r][c], ml, xch[0][c]);
+                for (int c = (LW >= 16 ? 0 : j + 1); c < LW; ++c)
+                    if (c > j) cfma(x[r][c], ml, xch[0][c]);
             }
         }
         }
@@ -897,8 +900,7 @@ constexpr int64_t LEAF_MAX_ROWS = 12288;
 template <class C>
 int leaf_width(int64_t m) {
     constexpr bool F = sizeof(C) == sizeof(cf);
-    if (m <= 1024) return F ? 16 : 8;
-    if (m <= 3072) return 8;
+    if (m <= 3072) return F ? 16 : 8;
     if (m <= 6144) return F ? 8 : 4;
     return F ? 4 : 2;
 }
@@ -911,8 +913,8 @@ void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, in
     constexpr bool F = sizeof(C) == sizeof(cf);
     if constexpr (F) {
         if (m <= 1024) LF(1, 16);
-        else if (m <= 2048) LF(2, 8);
-        else if (m <= 3072) LF(3, 8);
+        else if (m <= 2048) LF(2, 16);
+        else if (m <= 3072) LF(3, 16);
         else if (m <= 6144) LF(6, 8);
         else LF(12, 4);
     } else {
