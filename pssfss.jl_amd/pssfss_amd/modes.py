@@ -72,36 +72,48 @@ def allocate_mode_arrays(layer: Layer) -> None:
 
 
 def setup_modes(layer: Layer, k0: float, kvec) -> None:
-    """Per-point modal constants beta, gamma, Y, c, tvec (src/Modes.jl:298-322)."""
+    """Per-point modal constants beta, gamma, Y, c, tvec (src/Modes.jl:298-322).
+
+    Scalar arithmetic per mode, in the same operation order as the vector form
+    (beta = (beta00 + m beta1) + n beta2), with the arrays written once."""
     nm = len(layer.M)
     if nm == 0:
         return
-    b00 = np.asarray(kvec, float)
-    b1, b2 = layer.beta1, layer.beta2
-    area = TWOPI ** 2 / abs(b1[0] * b2[1] - b1[1] * b2[0])
+    b0x, b0y = float(kvec[0]), float(kvec[1])
+    b1x, b1y = float(layer.beta1[0]), float(layer.beta1[1])
+    b2x, b2y = float(layer.beta2[0]), float(layer.beta2[1])
+    area = TWOPI ** 2 / abs(b1x * b2y - b1y * b2x)
     ksq = k0 ** 2 * layer.epsr * layer.mur
     if layer.beta.shape[0] != nm:
         allocate_mode_arrays(layer)
-    for mode in range(nm):
-        m, n, p = layer.M[mode], layer.N[mode], layer.P[mode]
-        beta = (b00 + m * b1) + n * b2
-        b2s = float(beta[0] * beta[0] + beta[1] * beta[1])
+    zte = 1j * (k0 * ETA0) * layer.mur       # TE: Y = gamma / (j k0 eta0 mur)
+    ytm = 1j * (k0 / ETA0) * layer.epsr      # TM: Y = j k0 epsr / (eta0 gamma)
+    beta, gamma, Ys, cs, tvec = [], [], [], [], []
+    for m, n, p in zip(layer.M, layer.N, layer.P):
+        bx = (b0x + m * b1x) + n * b2x
+        by = (b0y + m * b1y) + n * b2y
+        b2s = bx * bx + by * by
         if b2s * area < 1e-14:
-            bhat = np.array([1.0, 0.0])
+            hx, hy = 1.0, 0.0
         else:
-            bhat = beta / math.sqrt(b2s)
-        layer.beta[mode] = beta
+            r = math.sqrt(b2s)
+            hx, hy = bx / r, by / r
         g = mysqrt(b2s - ksq)
-        layer.gamma[mode] = g
         if p == TE:
-            Y = g / (1j * (k0 * ETA0) * layer.mur)
-            tvec = np.array([-bhat[1], bhat[0]])
+            Y = g / zte
+            tvec.append((-hy, hx))
         else:
-            Y = 1j * (k0 / ETA0) * layer.epsr / g
-            tvec = bhat
-        layer.Y[mode] = Y
-        layer.tvec[mode] = tvec
-        layer.c[mode] = mysqrt(1 / (area * Y))
+            Y = ytm / g
+            tvec.append((hx, hy))
+        beta.append((bx, by))
+        gamma.append(g)
+        Ys.append(Y)
+        cs.append(mysqrt(1 / (area * Y)))
+    layer.beta[:] = beta
+    layer.gamma[:] = gamma
+    layer.Y[:] = Ys
+    layer.c[:] = cs
+    layer.tvec[:] = tvec
 
 
 def choose_layer_modes(layers: List[Layer], sheets, junc, gbls, k0max: float,
