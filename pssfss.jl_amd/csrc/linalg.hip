@@ -749,11 +749,13 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
     }
 }
 
-// R = B - A X for row-major A, column-major X/B/R (n x nrhs <= NR), one wavefront
-// per row, reading A once; also max |R| and max |X| per (system, rhs) as monotone
-// bit patterns.  Used for few right-hand sides, where the GEMM's 64-column tile
-// would be mostly idle.
-template <int NR>
+// R = B - A X for row-major A, column-major X/B/R (n x nrhs <= NR), RW rows per
+// wavefront (each X element loaded once serves RW rows), reading A once; also
+// max |R| and max |X| per (system, rhs) as monotone bit patterns.  Each row's
+// sum has the same order as with one row per wavefront: lane l takes the columns
+// j = l mod 64 in turn, then a butterfly over the 64 lanes.  Used for few
+// right-hand sides, where the GEMM's 64-column tile would be mostly idle.
+template <int NR, int RW>
 __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd *A0, int64_t sA,
                                                   const cd *X0, const cd *B0, int64_t sB,
                                                   cd *R0, unsigned long long *rmax,
@@ -764,37 +766,54 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd 
     const cd *Bv = B0 + b * sB;
     cd *R = R0 + b * sB;
     const int lane = threadIdx.x & 63;
-    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    cd acc[NR];
+    const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
+    if (i0 >= n) return;
+    cd acc[RW][NR];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) acc[c] = cd{0.0, 0.0};
-    const cd *row = A + i * n;
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[r][c] = cd{0.0, 0.0};
     for (int64_t j = lane; j < n; j += 64) {
-        const cd a = row[j];
+        cd xv[NR];
 #pragma unroll
-        for (int c = 0; c < NR; ++c)
-            if (c < nrhs) cfma(acc[c], a, X[j + n * c]);
-    }
+        for (int c = 0; c < NR; ++c) xv[c] = c < nrhs ? X[j + n * c] : cd{0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < NR; ++c) {
-        for (int off = 32; off > 0; off >>= 1) {
-            acc[c].x += __shfl_xor(acc[c].x, off, 64);
-            acc[c].y += __shfl_xor(acc[c].y, off, 64);
+        for (int r = 0; r < RW; ++r) {
+            if (i0 + r < n) {
+                const cd a = A[(i0 + r) * n + j];
+#pragma unroll
+                for (int c = 0; c < NR; ++c)
+                    if (c < nrhs) cfma(acc[r][c], a, xv[c]);
+            }
         }
     }
-    if (lane < nrhs && lane < NR) {
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            if (c < nrhs)
+                for (int off = 32; off > 0; off >>= 1) {
+                    acc[r][c].x += __shfl_xor(acc[r][c].x, off, 64);
+                    acc[r][c].y += __shfl_xor(acc[r][c].y, off, 64);
+                }
+        }
+    // lane l writes row i0 + l / NR, right-hand side l % NR
+    const int rl = lane / NR, cl = lane % NR;
+    if (lane < RW * NR && cl < nrhs && i0 + rl < n) {
         cd a = cd{0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < NR; ++c)
-            if (c == lane) a = acc[c];
-        const cd r = Bv[i + n * lane] - a;
-        R[i + n * lane] = r;
-        const double rm = fmax(fabs(r.x), fabs(r.y));
-        const cd xv = X[i + n * lane];
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int c = 0; c < NR; ++c)
+                if (r == rl && c == cl) a = acc[r][c];
+        const int64_t i = i0 + rl;
+        const cd rv = Bv[i + n * cl] - a;
+        R[i + n * cl] = rv;
+        const double rm = fmax(fabs(rv.x), fabs(rv.y));
+        const cd xv = X[i + n * cl];
         const double xm = fmax(fabs(xv.x), fabs(xv.y));
-        atomicMax(rmax + b * nrhs + lane, (unsigned long long)__double_as_longlong(rm));
-        atomicMax(xmax + b * nrhs + lane, (unsigned long long)__double_as_longlong(xm));
+        atomicMax(rmax + b * nrhs + cl, (unsigned long long)__double_as_longlong(rm));
+        atomicMax(xmax + b * nrhs + cl, (unsigned long long)__double_as_longlong(xm));
     }
 }
 
@@ -900,6 +919,7 @@ constexpr int64_t LEAF_MAX_ROWS = 12288;
 template <class C>
 int leaf_width(int64_t m) {
     constexpr bool F = sizeof(C) == sizeof(cf);
+    if (m <= 1024) return 16;
     if (m <= 3072) return F ? 16 : 8;
     if (m <= 6144) return F ? 8 : 4;
     return F ? 4 : 2;
@@ -918,7 +938,7 @@ void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, in
         else if (m <= 6144) LF(6, 8);
         else LF(12, 4);
     } else {
-        if (m <= 1024) LF(1, 8);
+        if (m <= 1024) LF(1, 16);
         else if (m <= 2048) LF(2, 8);
         else if (m <= 3072) LF(3, 8);
         else if (m <= 6144) LF(6, 4);
@@ -1218,12 +1238,14 @@ void residual_batched(int64_t n, int nrhs, int batch, const cd *A, int64_t sA, c
         return;
     }
     if (nrhs <= 8) {
-        dim3 g((unsigned)((n + 3) / 4), (unsigned)ns);
+        // four wavefronts per workgroup: 16 rows (nrhs <= 4) or 8 rows per workgroup
         if (nrhs <= 4)
-            hipLaunchKernelGGL(k_residual<4>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
+            hipLaunchKernelGGL((k_residual<4, 4>), dim3((unsigned)((n + 15) / 16), (unsigned)ns),
+                               dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
                                (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
         else
-            hipLaunchKernelGGL(k_residual<8>, g, dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
+            hipLaunchKernelGGL((k_residual<8, 2>), dim3((unsigned)((n + 7) / 8), (unsigned)ns),
+                               dim3(256), 0, st, n, nrhs, A, sA, X, Bv, sB, R,
                                (unsigned long long *)rnorm, (unsigned long long *)xnorm, sel);
         HIPCHECK(hipGetLastError());
         return;

@@ -63,7 +63,7 @@ def test_lu_pivoting(ctx):
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_lu_pivoting_large(ctx, n, prec):
     """Unscaled Gaussian matrices: pivots land anywhere in the leaves of 1 to 5 rows
-    per thread (leaf widths 16/8 FP32 and 8/4 FP64), across the two 64-column
+    per thread (leaf widths 16/8 FP32, 16/8/4 FP64), across the two 64-column
     sub-panels of each 128-column panel, and through the blocked solves."""
     _say(f"lu_solve random Gaussian n={n} {prec}")
     rng = np.random.default_rng(n)
