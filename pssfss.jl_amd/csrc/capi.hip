@@ -661,14 +661,15 @@ constexpr int IR_MAXIT = 30;
 // through their inverses in ctx->wDinv, except for tr); each sweep then forms the
 // FP64 residual and stops with LAPACK zcgesv's rule (max_c ||r_c|| < sqrt(n)
 // ||x_c|| ||op(Z)|| eps), at IR_MAXIT sweeps, or when a sweep does not halve the
-// backward error.  Systems not listed keep whatever ctx->wX holds.
+// backward error.  Systems not listed keep whatever ctx->wX holds.  With
+// anorm_ready, ctx->wanorm already holds ||op(Z)||_inf of every system.
 struct Refined {
     std::vector<int> iters, conv;
     std::vector<double> best;      // backward error in units of sqrt(n) eps
 };
 
 Refined refine_mixed(pssfss_ctx *ctx, int64_t n, int ncol, int np, bool tr, const cd *B,
-                     const std::vector<int> &systems) {
+                     const std::vector<int> &systems, bool anorm_ready = false) {
     hipStream_t st = ctx->st;
     const int64_t nb2 = n * n, sB = n * ncol;
     ctx->wX.ensure((size_t)sB * np);
@@ -692,8 +693,10 @@ Refined refine_mixed(pssfss_ctx *ctx, int64_t n, int ncol, int np, bool tr, cons
     };
     upload_act();
     // ||op(Z)||_inf of each system for the stopping rule
-    if (tr) matrix_one_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
-    else matrix_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
+    if (!anorm_ready) {
+        if (tr) matrix_one_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
+        else matrix_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wanorm.p, st);
+    }
     std::vector<double> anorm(np + 1);
     d2h(ctx, anorm.data(), ctx->wanorm.p, np);
     launch_zero(ctx->wX.p, sB * np, st);
@@ -762,7 +765,8 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     // FP32 LU with FP64 iterative refinement.
     ctx->wZf.ensure((size_t)nb2 * np);
     stage_begin(3, st);
-    convert_z_to_f32(ctx->wZ.p, ctx->wZf.p, nb2 * np, st);
+    ctx->wanorm.ensure((size_t)np + 2);
+    convert_z_to_f32_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wZf.p, ctx->wanorm.p, st);
     stage_end(3, st);
     lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st);
     stage_begin(5, st);
@@ -771,7 +775,7 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     lu_diag_inverses_f32(n, np, ctx->wZf.p, nb2, ctx->wDinv.p, st);
     std::vector<int> all(np);
     for (int p = 0; p < np; ++p) all[p] = p;
-    Refined rf = refine_mixed(ctx, n, ncol, np, false, ctx->wB.p, all);
+    Refined rf = refine_mixed(ctx, n, ncol, np, false, ctx->wB.p, all, true);
     stage_end(5, st);
     // Points whose refinement did not reach an FP64-level backward error: FP64 LU
     // (in place: Z of those points is replaced by its factors).

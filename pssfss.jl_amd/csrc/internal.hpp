@@ -192,6 +192,10 @@ void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t 
 void lu_diag_inverses_f32(int64_t n, int batch, const cf *LU, int64_t sA, cf *dinv,
                           hipStream_t st);
 void convert_z_to_f32(const cd *Z, cf *Zf, int64_t count, hipStream_t st);
+// Zf = Z in FP32 and out[b] = infinity norm of system b (as matrix_inf_norms), in
+// one read of Z; Zf has Z's layout (row-major n x n, stride sA).
+void convert_z_to_f32_inf_norms(int64_t n, int batch, const cd *Z, int64_t sA, cf *Zf,
+                                double *out, hipStream_t st);
 // r = b - A x (FP64; a row-per-wavefront kernel for up to 8 right-hand sides, else
 // one transposed-A GEMM per system), batched; then the largest
 // |re| or |im| of each column of r and of x, at rnorm/xnorm[system * nrhs + rhs].

@@ -858,6 +858,26 @@ __global__ __launch_bounds__(256) void k_rownorm(int64_t n, const cd *A0, int64_
     if (lane == 0) atomicMax(out + b, (unsigned long long)__double_as_longlong(s));
 }
 
+// FP32 copy of each row-major n x n system and its infinity norm in one read of
+// A: one wavefront per row, the row sums in k_rownorm's order.
+__global__ __launch_bounds__(256) void k_cd_to_cf_rownorm(int64_t n, const cd *A0, int64_t sA,
+                                                          cf *F0, unsigned long long *out) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const cd *row = A0 + b * sA + i * n;
+    cf *frow = F0 + b * sA + i * n;
+    double s = 0;
+    for (int64_t j = lane; j < n; j += 64) {
+        const cd a = row[j];
+        frow[j] = cf{(float)a.x, (float)a.y};
+        s += cabs(a);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) atomicMax(out + b, (unsigned long long)__double_as_longlong(s));
+}
+
 // One norm (largest column sum of |a_ij|) of each row-major n x n system, one
 // thread per column (consecutive threads read consecutive columns of a row).
 __global__ __launch_bounds__(256) void k_colnorm(int64_t n, const cd *A0, int64_t sA,
@@ -1286,6 +1306,15 @@ void matrix_one_norms(int64_t n, int batch, const cd *A, int64_t sA, double *out
     launch_zero((cd *)out, (batch + 1) / 2, st);
     dim3 g((unsigned)((n + 255) / 256), (unsigned)batch);
     hipLaunchKernelGGL(k_colnorm, g, dim3(256), 0, st, n, A, sA, (unsigned long long *)out);
+    HIPCHECK(hipGetLastError());
+}
+
+void convert_z_to_f32_inf_norms(int64_t n, int batch, const cd *Z, int64_t sA, cf *Zf,
+                                double *out, hipStream_t st) {
+    launch_zero((cd *)out, (batch + 1) / 2, st);
+    dim3 g((unsigned)((n + 3) / 4), (unsigned)batch);
+    hipLaunchKernelGGL(k_cd_to_cf_rownorm, g, dim3(256), 0, st, n, Z, sA, Zf,
+                       (unsigned long long *)out);
     HIPCHECK(hipGetLastError());
 }
 

@@ -532,7 +532,9 @@ __device__ void block_count(unsigned long long v, unsigned long long *out) {
 
 __global__ void k_sum_u64(const unsigned long long *a, int64_t n, unsigned long long *total) {
     unsigned long long v = 0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v += a[i];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        v += a[i];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(total, v);
 }
@@ -1097,7 +1099,9 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
     }
     HIPCHECK(hipGetLastError());
     if (total) {
-        hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(1024), 0, st, bcnt, nblk, total);
+        const unsigned sg =
+            (unsigned)std::max<int64_t>(1, std::min<int64_t>((nblk + 255) / 256, 1024));
+        hipLaunchKernelGGL(k_sum_u64, dim3(sg), dim3(256), 0, st, bcnt, nblk, total);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipFreeAsync(ph, st));
