@@ -28,6 +28,12 @@ constexpr int SB = 64;   // sub-panel width: triangular blocks held in LDS
 // launches: the 128 x 128 tile (WT = 4) runs one wavefront per SIMD and measured
 // slower on MI355X.  TA: A is given transposed (C = A^T B, A stored k x m), which
 // is a row-major m x k operand such as Z.  sel (optional) lists the batch entries.
+// FP32 uses three real products per complex k instead (Gauss): T1 = Ar Br,
+// T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi), Re C = T1 - T2, Im C = T3 - T1 - T2, with
+// four complex k per MFMA step: 3/4 of the matrix-core work and half the LDS
+// operand reads.  Its error bound is normwise (Higham, "Stability of a method for
+// multiplying complex matrices with three real matrix multiplications", 1992);
+// the FP32 factors it builds are only used under FP64 refinement.
 template <class C, int WT, bool TA>
 __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                                                    const C *__restrict__ A, int64_t lda,
@@ -39,6 +45,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     typedef typename MT::V V;
     constexpr int BM = 32 * WT, BN = 32 * WT, BK = 16, WS = 16 * WT;
     constexpr int NL = BM * BK / 256;   // elements of each operand per thread and stage
+    constexpr bool G3 = sizeof(C) == sizeof(cf);   // three-product complex form
+    constexpr int NACC = G3 ? 3 : 2;
     __shared__ C As[2][BK][BM + (TA ? 1 : 0)];
     __shared__ C Bs[2][BK][BN + 1];   // padded: the k-fastest stores spread over banks
     const int64_t bz = sel ? sel[blockIdx.z] : blockIdx.z;
@@ -49,13 +57,13 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
     const C zero = CTraits<C>::zero();
-    V acc[WT][WT][2];
+    V acc[WT][WT][NACC];
 #pragma unroll
     for (int a = 0; a < WT; ++a)
 #pragma unroll
         for (int b = 0; b < WT; ++b)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) acc[a][b][c] = V{0, 0, 0, 0};
+            for (int c = 0; c < NACC; ++c) acc[a][b][c] = V{0, 0, 0, 0};
     C ra[NL], rb[NL];
 #define GEMM_LOAD(K0)                                                                          \
     _Pragma("unroll") for (int q = 0; q < NL; ++q) {                                          \
@@ -86,6 +94,35 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < nk) { GEMM_LOAD((kt + 1) * BK) }
+        if constexpr (G3) {
+#pragma unroll UNR
+        for (int s4 = 0; s4 < BK / 4; ++s4) {
+            const int kc = 4 * s4 + (lane >> 4);
+            R ar[WT], ai[WT], as[WT], br[WT], bi[WT], bs[WT];
+#pragma unroll
+            for (int ti = 0; ti < WT; ++ti) {
+                const C a = As[buf][kc][WS * wm + 16 * ti + (lane & 15)];
+                ar[ti] = a.x;
+                ai[ti] = a.y;
+                as[ti] = a.x + a.y;
+            }
+#pragma unroll
+            for (int tj = 0; tj < WT; ++tj) {
+                const C b = Bs[buf][kc][WS * wn + 16 * tj + (lane & 15)];
+                br[tj] = b.x;
+                bi[tj] = b.y;
+                bs[tj] = b.x + b.y;
+            }
+#pragma unroll
+            for (int ti = 0; ti < WT; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < WT; ++tj) {
+                    acc[ti][tj][0] = MT::mma(br[tj], ar[ti], acc[ti][tj][0]);
+                    acc[ti][tj][1] = MT::mma(bi[tj], ai[ti], acc[ti][tj][1]);
+                    acc[ti][tj][2] = MT::mma(bs[tj], as[ti], acc[ti][tj][2]);
+                }
+        }
+        } else {
 #pragma unroll UNR
         for (int s2 = 0; s2 < BK / 2; ++s2) {
             const int kc = 2 * s2 + kh;
@@ -109,6 +146,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                     acc[ti][tj][1] = MT::mma(bop[tj], aim[ti], acc[ti][tj][1]);
                 }
         }
+        }
         if (kt + 1 < nk) { GEMM_STORE(buf ^ 1) }
         __syncthreads();
     }
@@ -125,8 +163,14 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                 const int64_t j = n0 + WS * wn + 16 * tj + MT::row(lane, e);
                 if (i < m && j < n) {
                     C v;
-                    v.x = acc[ti][tj][0][e];
-                    v.y = acc[ti][tj][1][e];
+                    if constexpr (G3) {
+                        const R t1 = acc[ti][tj][0][e], t2 = acc[ti][tj][1][e];
+                        v.x = t1 - t2;
+                        v.y = acc[ti][tj][2][e] - t1 - t2;
+                    } else {
+                        v.x = acc[ti][tj][0][e];
+                        v.y = acc[ti][tj][1][e];
+                    }
                     C *p = Cm + i + ldc * j;
                     C o = alpha * v;
                     if (use_beta) o += beta * (*p);
