@@ -264,22 +264,33 @@ __global__ __launch_bounds__(256) void k_panel_copy(int64_t n, int64_t k, int kb
     C *A = A0 + (int64_t)blockIdx.y * sA;
     C *P = P0 + (int64_t)blockIdx.y * sPt;
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    // Loads go to registers first, from an in-bounds address where masked (element
+    // 0; those LDS entries are never read), so that all 16 issue before a wait.
+    C v[16];
     for (int cb = 0; cb < kb; cb += 64) {   // 64-column blocks of the panel
         if (dir == 0) {
+#pragma unroll
             for (int p = 0; p < 16; ++p) {
                 const int rr = q + 4 * p;
-                if (r0 + rr < m && cb + lane < kb) T[rr][lane] = A[(k + r0 + rr) * n + k + cb + lane];
+                const bool ok = r0 + rr < m && cb + lane < kb;
+                v[p] = A[ok ? (k + r0 + rr) * n + k + cb + lane : 0];
             }
+#pragma unroll
+            for (int p = 0; p < 16; ++p) T[q + 4 * p][lane] = v[p];
             __syncthreads();
             for (int p = 0; p < 16; ++p) {
                 const int cc = q + 4 * p;
                 if (r0 + lane < m && cb + cc < kb) P[(cb + cc) * m + r0 + lane] = T[lane][cc];
             }
         } else {
+#pragma unroll
             for (int p = 0; p < 16; ++p) {
                 const int cc = q + 4 * p;
-                if (r0 + lane < m && cb + cc < kb) T[lane][cc] = P[(cb + cc) * m + r0 + lane];
+                const bool ok = r0 + lane < m && cb + cc < kb;
+                v[p] = P[ok ? (cb + cc) * m + r0 + lane : 0];
             }
+#pragma unroll
+            for (int p = 0; p < 16; ++p) T[lane][q + 4 * p] = v[p];
             __syncthreads();
             for (int p = 0; p < 16; ++p) {
                 const int rr = q + 4 * p;
@@ -688,23 +699,37 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
         if (lane < rows && c < nc) y[s] = X[r0 + lane + n * c];
     }
     if (ss > 0) {
-        // factor tile rows [r0, r0 + rows) x columns [s0, s0 + ss) as Tt[column][row]
+        // factor tile rows [r0, r0 + rows) x columns [s0, s0 + ss) as Tt[column][row].
+        // Every lane loads from an in-bounds address (element 0 where masked) so
+        // that all loads issue before the first wait: a load under a branch is
+        // waited for before its LDS store.
+        C v[16];
+#pragma unroll
         for (int p = 0; p < 16; ++p) {
             const int rr = q + 4 * p;
-            C v = zero;
-            if (TR) {
-                if (rr < ss && lane < rows) v = LU[(s0 + rr) * n + r0 + lane];
-                Tt[rr][lane] = v;
-            } else {
-                if (rr < rows && lane < ss) v = LU[(r0 + rr) * n + s0 + lane];
-                Tt[lane][rr] = v;
-            }
+            const bool ok = TR ? (rr < ss && lane < rows) : (rr < rows && lane < ss);
+            const int64_t at = TR ? (s0 + rr) * n + r0 + lane : (r0 + rr) * n + s0 + lane;
+            const C t = LU[ok ? at : 0];
+            v[p] = ok ? t : zero;
         }
-        for (int t = tid; t < 64 * NRG; t += 256) {
-            const int l = t % 64, c = t / 64;
-            C v = zero;
-            if (l < ss && c < nc) v = X[s0 + l + n * c];
-            XS[l][c] = v;
+        C xv[NRG / 4];
+#pragma unroll
+        for (int u = 0; u < NRG / 4; ++u) {
+            const int t = tid + 256 * u, l = t % 64, c = t / 64;
+            const bool ok = l < ss && c < nc;
+            const C t2 = X[ok ? s0 + l + n * c : 0];
+            xv[u] = ok ? t2 : zero;
+        }
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const int rr = q + 4 * p;
+            if (TR) Tt[rr][lane] = v[p];
+            else Tt[lane][rr] = v[p];
+        }
+#pragma unroll
+        for (int u = 0; u < NRG / 4; ++u) {
+            const int t = tid + 256 * u;
+            XS[t % 64][t / 64] = xv[u];
         }
         __syncthreads();
         C acc[NS];
@@ -732,11 +757,13 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
         // product per right-hand side through LDS instead of the sequential solve
         const int64_t nblk = (n + 63) / 64;
         const C *Di = Dinv0 + ((sys * 2 + (UP ? 1 : 0)) * nblk + d0 / 64) * 4096;
+        C dv[16];
+#pragma unroll
+        for (int p = 0; p < 16; ++p) dv[p] = Di[(q + 4 * p) * 64 + lane];
         __syncthreads();
-        for (int p = 0; p < 16; ++p) {
-            const int rr = q + 4 * p;
-            Tt[lane][rr] = Di[rr * 64 + lane];      // Tt[col][row] = D^{-1}[row][col]
-        }
+#pragma unroll
+        for (int p = 0; p < 16; ++p)
+            Tt[lane][q + 4 * p] = dv[p];      // Tt[col][row] = D^{-1}[row][col]
 #pragma unroll
         for (int s = 0; s < NS; ++s) XS[lane][q + 4 * s] = y[s];
         __syncthreads();
@@ -757,12 +784,21 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
     // the diagonal block as Tt[col][row]; non-unit diagonal for U and U^T
     constexpr bool DIV = UP != TR;
     __syncthreads();
-    for (int p = 0; p < 16; ++p) {
-        const int rr = q + 4 * p;
-        C v = zero;
-        if (rr < ds && lane < ds) v = LU[(d0 + rr) * n + d0 + lane];
-        if (TR) Tt[rr][lane] = v;
-        else Tt[lane][rr] = v;
+    {
+        C v[16];
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const int rr = q + 4 * p;
+            const bool ok = rr < ds && lane < ds;
+            const C t = LU[ok ? (d0 + rr) * n + d0 + lane : 0];
+            v[p] = ok ? t : zero;
+        }
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const int rr = q + 4 * p;
+            if (TR) Tt[rr][lane] = v[p];
+            else Tt[lane][rr] = v[p];
+        }
     }
     __syncthreads();
     if (!UP) {
