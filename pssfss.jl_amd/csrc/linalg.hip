@@ -486,15 +486,19 @@ template <class C>
 __global__ void k_rowswap(int64_t n, int64_t k, int kb, C *A0, int64_t sA,
                           const int32_t *ipiv0, int64_t sP, int64_t cbeg, int64_t cend,
                           int64_t sbeg, int64_t send) {
-    // columns [cbeg, cend) without [sbeg, send), in pivot order of rows k..k+kb-1
+    // columns [cbeg, cend) without [sbeg, send), in pivot order of rows k..k+kb-1;
+    // the pivots are staged in LDS first, so that the loop waits only on swaps
+    __shared__ int32_t pv[NB];
     C *A = A0 + (int64_t)blockIdx.y * sA;
     const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.y * sP;
+    for (int t = threadIdx.x; t < kb; t += blockDim.x) pv[t] = ipiv[k + t];
+    __syncthreads();
     const int64_t c0 = cbeg + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t skip = send - sbeg;
     if (c0 >= cend - skip) return;
     const int64_t c = c0 < sbeg ? c0 : c0 + skip;
     for (int j = 0; j < kb; ++j) {
-        const int64_t r = k + j, p = ipiv[r];
+        const int64_t r = k + j, p = pv[j];
         if (p != r) {
             C t = A[r * n + c];
             A[r * n + c] = A[p * n + c];
