@@ -657,6 +657,16 @@ __global__ void k_copy_rows(int64_t n, int64_t r0, int nr, int64_t c0, const C *
     }
 }
 
+// a where ok, else 0, selected per component: a select of the whole struct is
+// lowered through a stack slot in scratch memory.
+template <class C>
+__device__ __forceinline__ C csel(bool ok, const C &a) {
+    C r;
+    r.x = ok ? a.x : 0;
+    r.y = ok ? a.y : 0;
+    return r;
+}
+
 // One step of the blocked triangular solves L y = b (unit lower, UP false) and
 // U x = y (UP true) on a column-major X (n x nrhs per system), in place.  With TR
 // the factors are used transposed, for Z^T x = b: U^T y = b (lower, UP false) and
@@ -714,7 +724,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
             const bool ok = TR ? (rr < ss && lane < rows) : (rr < rows && lane < ss);
             const int64_t at = TR ? (s0 + rr) * n + r0 + lane : (r0 + rr) * n + s0 + lane;
             const C t = LU[ok ? at : 0];
-            v[p] = ok ? t : zero;
+            v[p] = csel(ok, t);
         }
         C xv[NRG / 4];
 #pragma unroll
@@ -722,7 +732,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
             const int t = tid + 256 * u, l = t % 64, c = t / 64;
             const bool ok = l < ss && c < nc;
             const C t2 = X[ok ? s0 + l + n * c : 0];
-            xv[u] = ok ? t2 : zero;
+            xv[u] = csel(ok, t2);
         }
 #pragma unroll
         for (int p = 0; p < 16; ++p) {
@@ -756,9 +766,10 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
         }
         return;
     }
-    if (Dinv0) {
-        // the diagonal block's inverse (k_diag_inv): y <- D^{-1} y, a 64 x 64
-        // product per right-hand side through LDS instead of the sequential solve
+    // the diagonal block's inverse (k_diag_inv, made for the FP32 factors only):
+    // y <- D^{-1} y, a 64 x 64 product per right-hand side through LDS instead of
+    // the sequential solve
+    if constexpr (sizeof(C) == sizeof(cf)) if (Dinv0) {
         const int64_t nblk = (n + 63) / 64;
         const C *Di = Dinv0 + ((sys * 2 + (UP ? 1 : 0)) * nblk + d0 / 64) * 4096;
         C dv[16];
@@ -795,7 +806,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(int64_t n, int nrhs, const C 
             const int rr = q + 4 * p;
             const bool ok = rr < ds && lane < ds;
             const C t = LU[ok ? (d0 + rr) * n + d0 + lane : 0];
-            v[p] = ok ? t : zero;
+            v[p] = csel(ok, t);
         }
 #pragma unroll
         for (int p = 0; p < 16; ++p) {
