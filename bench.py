@@ -32,8 +32,8 @@ for _sub in ("pssfss.jl_amd", "oracle"):
 # (k_jk_mfma<1,2> and the half-width k_jk_mfma<1,1>), from the FETCH_SIZE /
 # WRITE_SIZE passes in profiles/ring_on_slab_traffic_pmc.md (counters calibrated
 # with the 256 MiB copy of --calibrate copy: FETCH x2.00, WRITE x1.00).
-JK_MFMA_TRAFFIC_BYTES = ((2775.5 + 558.5) * 1024 * 2.0
-                         + (40685444.3 + 2554272.0) * 1024 * 1.0)
+JK_MFMA_TRAFFIC_BYTES = ((2860.8 + 558.3) * 1024 * 2.0
+                         + (40685465.2 + 2554272.0) * 1024 * 1.0)
 PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 (vector = matrix) / FP32 matrix
 
 
