@@ -43,7 +43,10 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     typedef typename CTraits<C>::R R;
     typedef MfmaT<R> MT;
     typedef typename MT::V V;
-    constexpr int BM = 32 * WT, BN = 32 * WT, BK = 16, WS = 16 * WT;
+    // FP64: 8 complex k per LDS stage (33 KB double-buffered, three workgroups per
+    // CU by registers) instead of 16 (66 KB, two workgroups per CU by LDS)
+    constexpr int BM = 32 * WT, BN = 32 * WT, BK = sizeof(C) == sizeof(cd) ? 8 : 16,
+                  WS = 16 * WT;
     constexpr int NL = BM * BK / 256;   // elements of each operand per thread and stage
     constexpr bool G3 = sizeof(C) == sizeof(cf);   // three-product complex form
     constexpr int NACC = G3 ? 3 : 2;
