@@ -97,6 +97,7 @@ typedef struct {
 
 /* ---- context ---------------------------------------------------------------- */
 int pssfss_ctx_create(int device, pssfss_ctx **out);
+/* Releases the context's stream and workspaces and every sheet still uploaded to it. */
 int pssfss_ctx_destroy(pssfss_ctx *ctx);
 const char *pssfss_last_error(const pssfss_ctx *ctx);
 /* Device name, CU count and HBM size, for logs. */
@@ -107,6 +108,8 @@ int pssfss_device_info(pssfss_ctx *ctx, char *name, int name_len, int *num_cu,
 /* Upload a sheet; replaces setup_rwg's in-memory RWGData for the device
  * (src/RWG.jl:90-324 builds the maps; src/PSSFSS.jl:220 calls it). */
 int pssfss_sheet_upload(pssfss_ctx *ctx, const pssfss_sheet_desc *desc, pssfss_sheet **out);
+/* Frees a sheet of this context (its device mesh and J/K cache); PSSFSS_EINVAL for a
+ * sheet that is not live in the context.  NULL is a no-op. */
 int pssfss_sheet_free(pssfss_ctx *ctx, pssfss_sheet *sheet);
 
 /* Unique-face-pair classes of setup_rwg (src/RWG.jl:270-322) on the device, for

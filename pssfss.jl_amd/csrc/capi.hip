@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <map>
 #include <new>
+#include <set>
 #include <vector>
 
 #include "../../include/pssfss_hip.h"
@@ -89,6 +90,7 @@ bool approx2(const double *a, const double *b) {
 
 thread_local StageRecorder *g_stage_rec = nullptr;
 
+
 struct pssfss_ctx {
     int device = 0;
     bool stats_on = false;
@@ -107,6 +109,7 @@ struct pssfss_ctx {
     DBuf<double> wnorm;
     DBuf<cd> wG;                 // fill class terms
     DBuf<double> wanorm;         // infinity norms of the systems (refinement)
+    std::set<pssfss_sheet *> sheets;   // sheets uploaded and not yet freed
     const cd *dft_matrix(int M) {
         auto it = dft.find(M);
         if (it != dft.end()) return it->second.p;
@@ -129,8 +132,11 @@ struct pssfss_sheet {
     std::vector<int32_t> hfv;   // 3*nface 0-based face vertices (adjoint directions)
     bool has_fr = false;        // any face with nonzero sheet resistance
     DBuf<int64_t> uoff, umem;
-    // J/K cache with the key semantics of src/FillZY.jl:97-108
+    // J/K cache with the key semantics of src/FillZY.jl:97-108, plus the precision
+    // the integrals were summed in: FP32-summed rings (jk_mixed) serve only mixed
+    // consumers, FP64 integrals serve both
     bool jk_valid = false;
+    bool jk_mixed = false;
     double jk_psi1 = 0, jk_psi2 = 0, jk_u = 0;
     DBuf<cd> jk;
     // singular integrals (depend on u only)
@@ -489,7 +495,8 @@ std::vector<const cd *> resolve_jk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssf
                                    int np, int mixed, DBuf<cd> &slots, double &key_u) {
     std::vector<int> slot(np);
     std::vector<double> keys;   // psi1, psi2, u per new slot
-    bool valid = sh->jk_valid;
+    // an FP64 consumer treats FP32-summed integrals as a miss
+    bool valid = sh->jk_valid && (mixed || !sh->jk_mixed);
     double k1 = sh->jk_psi1, k2 = sh->jk_psi2, ku = sh->jk_u;
     int cur = -1;               // -1: the sheet's cached integrals
     bool uses_old = false;
@@ -539,6 +546,7 @@ std::vector<const cd *> resolve_jk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssf
     HIPCHECK(hipMemcpyAsync(sh->jk.p, slots.p + (nnew - 1) * stride, stride * sizeof(cd),
                             hipMemcpyDeviceToDevice, ctx->st));
     sh->jk_valid = true;
+    sh->jk_mixed = mixed != 0;
     sh->jk_psi1 = keys[3 * (nnew - 1)];
     sh->jk_psi2 = keys[3 * (nnew - 1) + 1];
     sh->jk_u = keys[3 * (nnew - 1) + 2];
@@ -1316,6 +1324,9 @@ int pssfss_ctx_destroy(pssfss_ctx *ctx) {
     if (!ctx) return PSSFSS_EINVAL;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->st);
+    // sheets the caller did not free go with their context
+    for (pssfss_sheet *sh : ctx->sheets) delete sh;
+    ctx->sheets.clear();
     ctx->dft.clear();
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
@@ -1346,6 +1357,7 @@ int pssfss_sheet_upload(pssfss_ctx *ctx, const pssfss_sheet_desc *desc, pssfss_s
         pssfss_sheet *sh = new pssfss_sheet();
         try {
             upload_sheet(ctx, desc, sh);
+            ctx->sheets.insert(sh);
         } catch (...) {
             delete sh;
             throw;
@@ -1356,6 +1368,9 @@ int pssfss_sheet_upload(pssfss_ctx *ctx, const pssfss_sheet_desc *desc, pssfss_s
 
 int pssfss_sheet_free(pssfss_ctx *ctx, pssfss_sheet *sheet) {
     return guarded(ctx, [&] {
+        if (!sheet) return;
+        if (!ctx->sheets.erase(sheet)) throw ArgError("sheet is not live in this context");
+        HIPCHECK(hipSetDevice(ctx->device));
         sync(ctx);
         delete sheet;
     });

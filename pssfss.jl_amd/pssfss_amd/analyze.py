@@ -247,8 +247,9 @@ def analyze(strata, flist, steering, precision: str = "fp64", mode_override=None
     ctx = ctx or get_context()
     setup = prepare(strata, flist, steering, mode_override=mode_override, dbmin=dbmin,
                     classify=lambda sh: face_pair_classes(ctx, sh))
-    solver = SheetSolver(ctx, setup, precision)
-    results, stats = solver.run()
+    # the device sheets (mesh and J/K cache) live for this call only
+    with SheetSolver(ctx, setup, precision) as solver:
+        results, stats = solver.run()
     for path, ops in outlist:
         write_outputs_csv(path, results, ops)
     return (results, stats) if return_stats else results

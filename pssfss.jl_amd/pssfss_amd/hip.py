@@ -472,3 +472,10 @@ class SheetSolver:
         for d in self.dsheets.values():
             d.free()
         self.dsheets = {}
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False

@@ -429,3 +429,62 @@ def test_face_pair_classes_device_vs_host(ctx, pk, mesh):
     assert np.array_equal(o1, o2)
     assert np.array_equal(m1, m2)
     assert np.array_equal(u1, u2)
+
+
+# ------------------------------------------------------------------ surface resistance
+@pytest.mark.parametrize("kind", ["uniform", "per_face"])
+def test_fill_surface_resistance(ctx, pk, orc, kind):
+    """The surface-resistance loading of the self-triangle entries (eqs. 7-34/7-35,
+    src/FillZY.jl:221-231): a resistive strip with the same R on every face, and
+    one with a different R per face, against the oracle's fill at 1e-10."""
+    _say(f"fill with sheet resistance ({kind})")
+    from pssfss_amd.hip import DeviceSheet, PointBuilder
+    layers, sh, u, k0 = _fillzy_case(pk, "J")
+    sh = pk.rectstrip(Nx=12, Ny=10, Lx=20, Ly=50, Px=100, Py=100, units="mil", Rsheet=35.0)
+    if kind == "per_face":
+        sh.fr = np.random.default_rng(5).uniform(5.0, 400.0, sh.nface)
+    rw = pk.setup_rwg(sh)
+    ds = DeviceSheet(ctx, sh, rw)
+    pb = PointBuilder(layers, sh, u, 2, k0, np.zeros(2), with_modes=False)
+    Z, info = ds.fill(pb)
+    ds.free()
+    upm = sh.upm
+    b1m, b2m = sh.beta1 * upm, sh.beta2 * upm
+    JK, rr, ri = orc.filljk(sh, rw, u / upm, 0.0, 0.0)
+    tabs = orc.modal_tables(False, k0, u, 0.0, 0.0, layers, 2, b1m, b2m, np.zeros(2), 1e-7)
+    Zo = orc.fill(sh, rw, JK, rr, ri, k0, u, layers, 2, tabs[:3], b1m, b2m, 0.0, 0.0)
+    sh0 = pk.rectstrip(Nx=12, Ny=10, Lx=20, Ly=50, Px=100, Py=100, units="mil")
+    Zo0 = orc.fill(sh0, rw, JK, rr, ri, k0, u, layers, 2, tabs[:3], b1m, b2m, 0.0, 0.0)
+    load = np.linalg.norm(Zo - Zo0) / np.linalg.norm(Zo0)
+    err = _rel(Z, Zo)
+    print(f"  rel Frobenius |Z-Zo|/|Zo| = {err:.3e} (resistive loading is {load:.2e} of Z)")
+    assert load > 1e-3          # the loading is large enough to be checked
+    assert err <= 1e-10
+
+
+def test_resistive_sheet_closed_form(ctx, pk, orc):
+    """A resistive sheet filling the whole cell (periodic basis functions across the
+    cell edges) in free space at normal incidence: a uniform sheet of R ohm/sq
+    reflects -eta0/(eta0 + 2R) and transmits 2R/(eta0 + 2R) for both
+    polarisations.  S against the oracle to 1e-9 and against the closed form to
+    1e-4 (the oracle itself meets it to about 1e-5 on this 8 x 8 mesh)."""
+    _say("full-cell resistive sheet vs the transmission-line closed form")
+    eta0 = 376.730313412
+    R = 120.0
+    sh = pk.rectstrip(Lx=10, Ly=10, Px=10, Py=10, Nx=8, Ny=8, units="mm", Rsheet=R)
+    strata = [pk.Layer.make(), sh, pk.Layer.make()]
+    freqs = [3.0, 9.0]
+    steer = {"theta": [0.0], "phi": [0.0]}
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    d = _max_dS(res, ref)
+    G = -eta0 / (eta0 + 2 * R)
+    T = 2 * R / (eta0 + 2 * R)
+    dc = 0.0
+    for r in res:
+        g = r["gsm"]
+        dc = max(dc, np.abs(g.s11 - G * np.eye(2)).max(), np.abs(g.s21 - T * np.eye(2)).max(),
+                 np.abs(g.s22 - G * np.eye(2)).max(), np.abs(g.s12 - T * np.eye(2)).max())
+    print(f"  max |dS| vs oracle {d:.3e}; vs closed form {dc:.3e} (Gamma {G:.4f}, T {T:.4f})")
+    assert d <= 1e-9
+    assert dc <= 1e-4
