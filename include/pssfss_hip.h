@@ -200,6 +200,21 @@ int pssfss_rwgbfft(pssfss_ctx *ctx, pssfss_sheet *sheet, const double *k, double
 int pssfss_lu_solve_batch(pssfss_ctx *ctx, int64_t n, int nrhs, int nbatch, const double *A,
                           double *B, int precision, int *iters, double *resid);
 
+/* FP64 lu! + ldiv! for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops): no counterpart in the reference; the
+ * allocation-free variant of pssfss_lu_solve_batch.  A and B are DEVICE pointers:
+ * A holds nbatch column-major n x n complex matrices and is overwritten by LU
+ * factors; B holds nbatch (n, nrhs) right-hand sides and is overwritten by X.
+ * Pivots, permutations and all other working memory come from one fixed 64 MiB
+ * device array per GPU, kept for the life of the process: a call needs about
+ * (3080 + 16 nrhs) n + 131072 bytes per system (nbatch x n up to about 20000 at
+ * nrhs = 8); larger calls fail with PSSFSS_EINVAL.  Nothing in that array needs to
+ * start at zero.  Every call on one GPU shares the array: calls that overlap in
+ * time on the same GPU (two threads, two contexts) race and must be serialised
+ * by the caller. */
+int pssfss_lu_solve_batch_noalloc(pssfss_ctx *ctx, int64_t n, int nrhs, int nbatch, double *A,
+                                  double *B);
+
 /* cascade (src/GSMs.jl:125-141) for nbatch independent GSM pairs: a has port
  * sizes (n1, n2), b has (n2, n3).  Each GSM is packed as s11, s12, s21, s22
  * consecutively, column-major complex.  c receives (n1, n3) GSMs. */

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <new>
 #include <set>
 #include <vector>
@@ -90,9 +91,36 @@ bool approx2(const double *a, const double *b) {
 
 thread_local StageRecorder *g_stage_rec = nullptr;
 
+// Device memory of pssfss_lu_solve_batch_noalloc (pivots, row permutations, the LU's
+// panel scratch and the solve's staging), one copy per GPU for the whole process.
+// Host code never uses the name `scratch` as an address: each device's copy is
+// reached through the pointer hipGetSymbolAddress returns with that device current.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_addr;   // device ordinal -> its copy of `scratch`
+
+// Address of `scratch` on `device`, looked up once per device; the device must be
+// current.  nullptr when the lookup fails.
+void *scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_scratch_addr[device] = ptr;
+    return ptr;
+}
+}  // namespace
 
 struct pssfss_ctx {
     int device = 0;
+    void *scratch_dev = nullptr;     // this device's copy of `scratch`
     bool stats_on = false;
     StageRecorder rec;
     hipStream_t st = nullptr;
@@ -1316,6 +1344,12 @@ int pssfss_ctx_create(int device, pssfss_ctx **out) {
         delete c;
         return PSSFSS_EHIP;
     }
+    c->scratch_dev = scratch_address(device);
+    if (!c->scratch_dev) {
+        (void)hipStreamDestroy(c->st);
+        delete c;
+        return PSSFSS_EHIP;
+    }
     *out = c;
     return PSSFSS_OK;
 }
@@ -1636,6 +1670,43 @@ int pssfss_lu_solve_batch(pssfss_ctx *ctx, int64_t n, int nrhs, int nbatch, cons
             if (iters) iters[b] = inf[b].refine_iters;
             if (resid) resid[b] = inf[b].refine_resid;
         }
+    });
+}
+
+int pssfss_lu_solve_batch_noalloc(pssfss_ctx *ctx, int64_t n, int nrhs, int nbatch, double *A,
+                                  double *B) {
+    return guarded(ctx, [&] {
+        g_stage_rec = nullptr;      // no stage events: nothing is created or allocated
+        if (n <= 0 || nrhs <= 0 || nbatch <= 0 || !A || !B) throw ArgError("bad arguments");
+        if (n > 40960) throw ArgError("n above 40960 (row permutation held in LDS)");
+        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t nb = (size_t)nbatch;
+        const size_t b_piv = up(sizeof(int32_t) * (size_t)n * nb);
+        const size_t b_ws = up(sizeof(cd) * (size_t)lu_factor_ws_elems(n) * nb);
+        const size_t b_t = up(sizeof(cd) * (size_t)n * (size_t)nrhs * nb);
+        const size_t need = 2 * b_piv + b_ws + b_t;
+        if (need > NOALLOC_SCRATCH_BYTES) {
+            char msg[160];
+            snprintf(msg, sizeof msg,
+                     "n = %lld, nrhs = %d, nbatch = %d need %zu bytes of scratch; the limit is %zu",
+                     (long long)n, nrhs, nbatch, need, NOALLOC_SCRATCH_BYTES);
+            throw ArgError(msg);
+        }
+        HIPCHECK(hipSetDevice(ctx->device));
+        unsigned char *base = (unsigned char *)ctx->scratch_dev;
+        int32_t *ipiv = (int32_t *)base;
+        int32_t *perm = (int32_t *)(base + b_piv);
+        cd *ws = (cd *)(base + 2 * b_piv);
+        cd *T = (cd *)(base + 2 * b_piv + b_ws);
+        // A column-major is A^T row-major: factor that in place, then solve
+        // (A^T)^T X = B with the transposed triangular solves.
+        cd *a = (cd *)A;
+        const int64_t nb2 = n * n;
+        lu_factor_batched_ws(n, nbatch, a, nb2, ipiv, n, ws, ctx->st);
+        lu_make_perm(n, nbatch, ipiv, n, perm, ctx->st);
+        lu_solve_batched(n, nrhs, nbatch, a, nb2, perm, n, (cd *)B, n * nrhs, T, ctx->st, nullptr,
+                         true);
+        sync(ctx);
     });
 }
 

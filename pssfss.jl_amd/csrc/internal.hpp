@@ -171,6 +171,12 @@ void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int6
                    int64_t sC, int batch, hipStream_t st);
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
                        hipStream_t st);
+// Scratch entries per system of lu_factor_batched_ws, and the same factorisation
+// with that scratch passed in (ws holds batch * lu_factor_ws_elems(n) entries;
+// nothing is allocated).
+int64_t lu_factor_ws_elems(int64_t n);
+void lu_factor_batched_ws(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                          cd *ws, hipStream_t st);
 // perm from lu_make_perm; scratch holds batch * sB entries.  sel (optional): the
 // batch lists system indices, one per entry.  tr: solve Z^T x = b instead.
 void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,

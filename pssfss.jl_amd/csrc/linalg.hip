@@ -1186,25 +1186,25 @@ void panel_leaves(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv,
     HIPCHECK(hipGetLastError());
 }
 
+// Blocked LU of the batch with caller-provided scratch: Pt (n x NB per system, the
+// column-major panel copy), T (SB x n per system, U12 block rows) and Linv
+// (2 x 64 x 64 per system, the panel's diagonal-block inverses).
 template <class C>
-void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                 hipStream_t st) {
-    // column-major panel scratch (n x NB per system), the U12 block rows (SB x n) and
-    // the panel's diagonal-block inverses (2 x 64 x 64)
+void lu_factor_ws_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP, C *Pt,
+                    C *T, C *Linv, hipStream_t st) {
     const int64_t sPt = n * NB;
-    DBuf<C> Pt((size_t)sPt * batch), T((size_t)SB * n * batch), Linv((size_t)2 * 4096 * batch);
     for (int64_t k = 0; k < n; k += NB) {
         const int kb = (int)std::min<int64_t>(NB, n - k);
         stage_begin(3, st);
         if (n - k <= LEAF_MAX_ROWS) {
-            panel_leaves<C>(n, k, kb, A, sA, ipiv, sP, Pt.p, sPt, batch, st);
+            panel_leaves<C>(n, k, kb, A, sA, ipiv, sP, Pt, sPt, batch, st);
         } else {
             hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(1024), 0, st, n, k, kb, A,
                                sA, ipiv, sP);
             HIPCHECK(hipGetLastError());
         }
         rowswap_t<C>(n, k, kb, A, sA, ipiv, sP, 0, n, k, k + kb, batch, st);
-        trsm_u12_t<C>(n, k, kb, A, sA, batch, Linv.p, T.p, st);
+        trsm_u12_t<C>(n, k, kb, A, sA, batch, Linv, T, st);
         stage_end(3, st);
         const int64_t rest = n - k - kb;
         if (rest > 0) {
@@ -1216,6 +1216,13 @@ void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t 
         }
     }
     stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
+}
+
+template <class C>
+void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                 hipStream_t st) {
+    DBuf<C> Pt((size_t)n * NB * batch), T((size_t)SB * n * batch), Linv((size_t)2 * 4096 * batch);
+    lu_factor_ws_t<C>(n, batch, A, sA, ipiv, sP, Pt.p, T.p, Linv.p, st);
     // the scratch is released once the factorisation has run
     HIPCHECK(hipStreamSynchronize(st));
 }
@@ -1291,6 +1298,14 @@ void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int6
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
                        hipStream_t st) {
     lu_factor_t<cd>(n, batch, A, sA, ipiv, sP, st);
+}
+
+int64_t lu_factor_ws_elems(int64_t n) { return n * NB + (int64_t)SB * n + 2 * 4096; }
+
+void lu_factor_batched_ws(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
+                          cd *ws, hipStream_t st) {
+    cd *Pt = ws, *T = ws + n * NB * batch, *Linv = T + (int64_t)SB * n * batch;
+    lu_factor_ws_t<cd>(n, batch, A, sA, ipiv, sP, Pt, T, Linv, st);
 }
 
 void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,

@@ -104,13 +104,15 @@ def lib():
         L.pssfss_lu_solve_batch.argtypes = [vp, C.c_int64, C.c_int, C.c_int, _dp, _dp, C.c_int,
                                             C.POINTER(C.c_int), _dp]
         L.pssfss_cascade_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.pssfss_lu_solve_batch_noalloc.argtypes = [vp, C.c_int64, C.c_int, C.c_int, _dp, _dp]
         L.pssfss_stats_enable.argtypes = [vp, C.c_int]
         L.pssfss_stats_read.argtypes = [vp, C.POINTER(Stats), C.c_int]
         for name in ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_device_info",
                      "pssfss_sheet_upload", "pssfss_sheet_free", "pssfss_sheet_gsm",
                      "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_fill_dk0", "pssfss_filljk",
                      "pssfss_modal_tables", "pssfss_rwgbfft", "pssfss_lu_solve_batch",
-                     "pssfss_cascade_batch", "pssfss_stats_enable", "pssfss_stats_read"):
+                     "pssfss_cascade_batch", "pssfss_stats_enable", "pssfss_stats_read",
+                     "pssfss_lu_solve_batch_noalloc"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -121,7 +123,7 @@ EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssf
            "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk", "pssfss_modal_tables",
            "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
            "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint",
-           "pssfss_face_pair_classes", "pssfss_fill_dk0")
+           "pssfss_face_pair_classes", "pssfss_fill_dk0", "pssfss_lu_solve_batch_noalloc")
 
 
 def _d(a):
