@@ -2,24 +2,26 @@
 """Frequency points per second of the per-point MoM path (spatial sums, spectral
 tables, fill, LU, RHS/solve, extraction, cascade) on one MI355X.
 
-Workload (default): PEC annular ring (a = 3.0 mm, b = 3.6 mm) in a 9 mm cell on a
-0.787 mm eps_r = 2.2 / tan_delta = 0.0009 slab, 2016 faces / 2856 unknowns,
-theta = 30 deg, phi in {0, 45} deg, 51 frequencies 5..25 GHz: 102 points.
-One step = the whole sweep (all 102 points) with the sheet resident on the
-device; meshing, setup_rwg and mode selection are outside the timed region.
+Headline workload (default, --workload two_layer): a corner-parameterised 8 mm PEC
+patch (39 x 39 bilinear grid: 3042 faces, 4485 unknowns, every face pair its own
+class) in a 12 mm cell on eps_r 3.0 (1.0 mm) + eps_r 1.1 foam (3.0 mm), 12
+frequencies 8..12 GHz at normal incidence: 12 points.  One step = the whole sweep
+from a freshly uploaded sheet, so the frequency-independent spatial sums (one
+phasing key at normal incidence) run once per step, as once per analyze call in
+the reference; the sheet upload (mesh and maps) is outside the timed region.
 
 value = points/s of the mixed path (FP32 LU and smooth spatial sums, FP64
-refinement, singular terms and tables); the plain FP64 path is timed beside it
-and max|dS| between them is reported, as is max|dS| of both against the oracle
-on the CPU-baseline sample point.  detail.two_layer_patch holds the
-differentiation workload: forward points/s in FP64 and mixed, and adjoint
-gradient vectors per second; detail.meander_polarizer a sample of the cascaded
-4-sheet meanderline polarizer at full mesh size (about 12 k unknowns per sheet);
-detail.cross_batch a sample of the batched Jerusalem / loaded cross problems.
+refinement, singular terms and tables), all 12 points batched.  Beside it: the
+plain FP64 path batched the same way, the plain FP64 path one point per library
+call (the reference's calling pattern), max|dS| between the paths, per-stage
+times, the LU's whole-step flop rate F = (8/3) N^3 + 8 N^2 nrhs per point, and
+the adjoint gradient rate of |S21|^2 co-polar TE and TM with respect to the 8
+corner coordinates.  detail.ring_on_slab times the oblique ring sweep (spatial
+sums on every point); detail.meander_polarizer and detail.cross_batch sample the
+cascaded and batched workloads.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -28,12 +30,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 for _sub in ("pssfss.jl_amd", "oracle"):
     sys.path.insert(0, os.path.join(ROOT, _sub))
 
-# HBM bytes per launch pair of the mixed spatial-sum kernels on this workload
+# HBM bytes per launch pair of the mixed spatial-sum kernels on the ring workload
 # (k_jk_mfma<1,2> and the half-width k_jk_mfma<1,1>), from the FETCH_SIZE /
 # WRITE_SIZE passes in profiles/ring_on_slab_traffic_pmc.md (counters calibrated
 # with the 256 MiB copy of --calibrate copy: FETCH x2.00, WRITE x1.00).
 JK_MFMA_TRAFFIC_BYTES = ((2860.8 + 558.3) * 1024 * 2.0
                          + (40685465.2 + 2554272.0) * 1024 * 1.0)
+# HBM bytes per launch of the mixed LU trailing update k_gemm_mfma<cf,2,false,1> on the
+# two-layer patch (mean over its 35 launches per sweep), from the FETCH_SIZE / WRITE_SIZE
+# passes in profiles/two_layer_patch_traffic_pmc.md, calibrated the same way.
+TRAIL_GEMM_TRAFFIC_BYTES = 1.104e6 * 1024 * 2.0 + 8.291e5 * 1024 * 1.0
 PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 (vector = matrix) / FP32 matrix
 
 
@@ -46,20 +52,56 @@ def max_dS(a, b):
     return m
 
 
+def two_layer_strata(pk):
+    from pssfss_amd import adjoint as A
+    import numpy as np
+    corners = [[2.0, 2.0], [10.0, 2.0], [10.0, 10.0], [2.0, 10.0]]
+    sh, vt = A.bilinear_patch(corners, 39, 39, 12.0, 12.0)
+    L = pk.Layer.make
+    strata = [L(), sh, L(width=1.0, units="mm", epsr=3.0), L(width=3.0, units="mm", epsr=1.1),
+              L()]
+    return strata, np.linspace(8.0, 12.0, 12), {"theta": [0.0], "phi": [0.0]}, vt
+
+
+def timed_sweeps(hip, ctx, setup, prec, steps, warmup, label, max_batch=0):
+    """`warmup` untimed then `steps` timed sweeps, each from freshly uploaded sheets
+    (uploaded before the clock starts).  Returns (per-step seconds, last results,
+    last infos, stage stats summed over the timed steps)."""
+    for w in range(warmup):
+        with hip.SheetSolver(ctx, setup, prec, max_batch) as s:
+            s.upload()
+            t0 = time.perf_counter()
+            s.run()
+            print(f"[bench] {label} {prec} warmup {w}: {time.perf_counter() - t0:.3f} s",
+                  flush=True)
+    ctx.stats_read(reset=True)
+    times, res, info = [], None, None
+    for k in range(steps):
+        with hip.SheetSolver(ctx, setup, prec, max_batch) as s:
+            s.upload()
+            ctx.stats_enable(True)
+            t0 = time.perf_counter()
+            res, info = s.run()
+            times.append(time.perf_counter() - t0)
+            ctx.stats_enable(False)
+        print(f"[bench] {label} {prec} step {k}: {times[-1]:.3f} s", flush=True)
+    return times, res, info, ctx.stats_read(reset=True)
+
+
+def stage_summary(st, steps):
+    return {k: round(v["ms"] / steps, 3) for k, v in st.items() if k != "lu_total_flops"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="ring", choices=["ring", "patch"])
+    ap.add_argument("--workload", default="two_layer", choices=["two_layer", "ring", "patch"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--precisions", default="fp64,mixed")
-    ap.add_argument("--no-two-layer", action="store_true",
-                    help="skip the two-layer patch forward / adjoint measurements")
-    ap.add_argument("--no-meander", action="store_true",
-                    help="skip the 4-sheet meanderline polarizer sample")
-    ap.add_argument("--no-cross", action="store_true",
-                    help="skip the batched cross-problem sample")
+    ap.add_argument("--no-detail", action="store_true",
+                    help="headline workload only (no per-point FP64 sweep, adjoint, ring, "
+                         "meander or cross samples)")
     ap.add_argument("--calibrate", choices=["copy", "fma"],
                     help="run only a counter-calibration kernel of known work (under rocprofv3): "
                          "copy = 256 MiB device copy x 3, fma = FP64 and FP32 FMA loops of "
@@ -83,96 +125,82 @@ def main():
     import pssfss_amd as pk
     from pssfss_amd import hip
 
-    if args.workload == "ring":
+    vt = None
+    if args.workload == "two_layer":
+        strata, freqs, steer, vt = two_layer_strata(pk)
+        wname = ("two_layer_patch: 8 mm PEC patch (39x39 bilinear grid), 12 mm cell, eps_r 3.0 "
+                 "1.0 mm + eps_r 1.1 3.0 mm, 12 freqs 8-12 GHz, normal incidence")
+    elif args.workload == "ring":
         strata, freqs, steer = pk.workloads.ring_on_slab()
         wname = ("ring_on_slab: PEC annulus a=3.0 b=3.6 mm, 9 mm cell, 0.787 mm eps_r=2.2 "
                  "tan_d=0.0009 slab, theta=30, phi={0,45}, 51 freqs 5-25 GHz")
     else:
         strata, freqs, steer = pk.workloads.patch_free_space()
         wname = "patch_free_space: 5 mm PEC patch, 10 mm cell, 21 freqs 2-30 GHz, normal incidence"
-    npts, nface, nbf = pk.workloads.sheet_points(strata, freqs, steer)
     ctx = hip.get_context()
     dev = ctx.device_info()
-    print(f"[bench] device {dev}; workload {args.workload}: {npts} points, {nface} faces, "
-          f"{nbf} unknowns", flush=True)
     setup = pk.analyze.prepare(strata, freqs, steer)
-    precs = [p for p in args.precisions.split(",") if p]
-    solvers = {}
-    shared = None
-    for p in precs:
-        s = hip.SheetSolver(ctx, setup, p)
-        if shared is None:
-            shared = s.dsheets
-        s.dsheets = shared
-        solvers[p] = s
+    npts = len(list(setup.points()))
+    nface, nbf = setup.sheets[0].nface, int(setup.rwgdat[0].nbf)
+    nrhs = len(setup.layers[0].P) + len(setup.layers[-1].P)
+    print(f"[bench] device {dev}; workload {args.workload}: {npts} points, {nface} faces, "
+          f"{nbf} unknowns, {nrhs} right-hand sides", flush=True)
 
-    timing, results, stages = {}, {}, {}
-    for p in precs:
-        for w in range(args.warmup):
-            t0 = time.perf_counter()
-            solvers[p].run()
-            print(f"[bench] warmup {p} step {w}: {time.perf_counter() - t0:.3f} s", flush=True)
-        ctx.stats_read(reset=True)
-        ctx.stats_enable(True)
-        times = []
-        for k in range(args.steps):
-            t0 = time.perf_counter()
-            res, info = solvers[p].run()
-            times.append(time.perf_counter() - t0)
-            print(f"[bench] {p} step {k}: {times[-1]:.3f} s", flush=True)
-        ctx.stats_enable(False)
-        stages[p] = ctx.stats_read(reset=True)
-        results[p] = res
-        timing[p] = times
-        iters = [i.refine_iters for i in info["infos"]]
-        fb = sum(i.fallback_fp64 for i in info["infos"])
+    timing, results, stages, extra = {}, {}, {}, {}
+    for p in ("fp64", "mixed"):
+        times, res, info, st = timed_sweeps(hip, ctx, setup, p, args.steps, args.warmup,
+                                            args.workload)
+        timing[p], results[p], stages[p] = times, res, st
+        extra[f"points_per_s_{p}"] = round(npts / (sum(times) / len(times)), 3)
+        extra[f"stage_ms_per_step_{p}"] = stage_summary(st, args.steps)
         if p == "mixed":
-            print(f"[bench] mixed refinement sweeps min/max {min(iters)}/{max(iters)}, "
-                  f"FP64 fallbacks {fb}", flush=True)
-
-    head = "mixed" if "mixed" in timing else precs[0]
-    step_s = sum(timing[head]) / len(timing[head])
+            it = [i.refine_iters for i in info["infos"]]
+            extra["mixed_refinement_sweeps_min_max"] = [min(it), max(it)]
+            extra["mixed_fp64_fallbacks"] = int(sum(i.fallback_fp64 for i in info["infos"]))
+    step_s = sum(timing["mixed"]) / len(timing["mixed"])
     value = npts / step_s
-    extra = {}
-    for p in precs:
-        extra[f"points_per_s_{p}"] = npts / (sum(timing[p]) / len(timing[p]))
-    if "mixed" in results and "fp64" in results:
-        extra["max_dS_mixed_vs_fp64"] = max_dS(results["mixed"], results["fp64"])
-        extra["speedup_mixed_vs_fp64"] = extra["points_per_s_mixed"] / extra["points_per_s_fp64"]
+    extra["max_dS_mixed_vs_fp64"] = max_dS(results["mixed"], results["fp64"])
+    extra["speedup_mixed_vs_fp64_batched"] = round(value / extra["points_per_s_fp64"], 3)
+    if not args.no_detail:
+        # the reference's calling pattern: one point per library call, FP64
+        t, res1, _, st1 = timed_sweeps(hip, ctx, setup, "fp64", 1, 1, args.workload + " per-point",
+                                       max_batch=1)
+        extra["points_per_s_fp64_per_point"] = round(npts / t[0], 3)
+        extra["stage_ms_per_step_fp64_per_point"] = stage_summary(st1, 1)
+        extra["speedup_mixed_vs_fp64_per_point"] = round(value / extra["points_per_s_fp64_per_point"], 3)
+        extra["max_dS_fp64_per_point_vs_batched"] = max_dS(res1, results["fp64"])
 
-    # Roofline of the dominant kernel: the spatial sums (the stage that takes the
-    # most time), algorithmic flops = 8 per (class, quadrature point, phasing,
-    # lattice point) that the reference's ring rule sums, counted on the device.
-    st = stages[head]
-    jk = st["spatial_sums"]
-    jk_kernel = ("k_jk_mfma<1,2> + k_jk_mfma<1,1> (spatial sums, FP32 MFMA; the second "
-                 "launch is the last phasing block at half width)" if head == "mixed"
-                 else "k_jk_mfma64<2> + k_jk_mfma64<1> (spatial sums, FP64 MFMA)")
-    achieved = jk["flops"] / (jk["ms"] * 1e-3) / 1e12 if jk["ms"] > 0 else 0.0
-    roof = {"bound": "mfma", "kernel": jk_kernel, "achieved": round(achieved, 3),
-            "peak": PEAK[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK[head], 4),
-            "avg_launch_ms": round(jk["ms"] / max(1, jk["launches"]), 3),
-            "flops_per_launch": jk["flops"] / max(1, jk["launches"]),
-            "traffic": (JK_MFMA_TRAFFIC_BYTES if head == "mixed" and args.workload == "ring"
-                        else None)}
+    # Roofline of the dominant kernel of the mixed step: the LU's trailing-update
+    # GEMM (k_gemm_mfma, FP32 matrix cores), algorithmic flops 8 m n k per complex
+    # update, time from HIP events on the library's stream around every launch.
+    st = stages["mixed"]
     g = st["lu_gemm"]
     g_ach = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
-    extra["roofline_lu_gemm"] = {"kernel": "k_gemm_mfma (LU trailing update)",
-                                 "achieved": round(g_ach, 3), "peak": PEAK[head],
-                                 "unit": "TFLOP/s", "frac": round(g_ach / PEAK[head], 4),
-                                 "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4)}
-    tot_ms = sum(v["ms"] for k, v in st.items() if k != "lu_total_flops")
-    stage_ms = {k: round(v["ms"] / args.steps, 2) for k, v in st.items() if k != "lu_total_flops"}
-    lu_flops = st["lu_total_flops"]["flops"] / args.steps
-    extra["stage_ms_per_step"] = stage_ms
-    extra["dominant_stage"] = max(stage_ms, key=stage_ms.get)
-    extra["lu_tflops_per_s_whole_step"] = round(lu_flops / step_s / 1e12, 3)
+    roof = {"bound": "mfma", "kernel": "k_gemm_mfma<cf,2,false,1> (LU trailing updates, FP32 "
+            "matrix cores, three real products per complex k)",
+            "achieved": round(g_ach, 3), "peak": PEAK["mixed"], "unit": "TFLOP/s",
+            "frac": round(g_ach / PEAK["mixed"], 4),
+            "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4),
+            "flops_per_launch": g["flops"] / max(1, g["launches"]),
+            "launches_per_step": g["launches"] / args.steps,
+            "traffic": TRAIL_GEMM_TRAFFIC_BYTES if args.workload == "two_layer" else None}
+    # the LU's whole-step rate of the metric's definition: F per point x points/s
+    F = 8.0 / 3.0 * nbf ** 3 + 8.0 * nbf ** 2 * nrhs
+    extra["F_flops_per_point"] = F
+    extra["roofline_step_F"] = {
+        p: {"achieved": round(F * extra[f"points_per_s_{p}"] / 1e12, 3), "peak": PEAK[p],
+            "unit": "TFLOP/s",
+            "frac": round(F * extra[f"points_per_s_{p}"] / 1e12 / PEAK[p], 4)}
+        for p in ("fp64", "mixed")}
+    stm = extra["stage_ms_per_step_mixed"]
+    extra["dominant_stage_mixed"] = max(stm, key=stm.get)
 
-    if not args.no_two_layer:
-        extra["two_layer_patch"] = two_layer_patch(pk, hip, ctx)
-    if not args.no_meander:
+    if not args.no_detail:
+        if vt is not None:
+            extra["adjoint"] = adjoint_rates(pk, ctx, strata, freqs, steer, vt)
+        if args.workload != "ring":
+            extra["ring_on_slab"] = ring_detail(pk, hip, ctx)
         extra["meander_polarizer"] = meander_polarizer(pk, hip, ctx)
-    if not args.no_cross:
         extra["cross_batch"] = cross_batch(pk, hip, ctx)
 
     cpu = None
@@ -185,57 +213,55 @@ def main():
         "value": round(value, 3), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64", "data": "synthetic",
+        "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": wname, "points": npts, "faces": nface, "unknowns": nbf,
-                   "precision": head},
+                   "rhs": nrhs, "precision": "mixed"},
         "roofline": roof, "cpu_baseline": cpu, "detail": extra,
     }
     print(json.dumps(line), flush=True)
 
 
-def two_layer_patch(pk, hip, ctx):
-    """Corner-parameterised 8 mm patch (39 x 39 bilinear grid: 3042 faces, 4485
-    unknowns, every face pair its own class) in a 12 mm cell on eps_r 3.0 (1.0 mm)
-    + eps_r 1.1 foam (3.0 mm); 12 frequencies 8..12 GHz at normal incidence.
-    Forward points/s of the FP64 and mixed paths, and gradient vectors per second of
-    |S21|^2 co-polar TE and TM with respect to the 8 corner coordinates (one
-    gradient vector = 8 derivatives of one objective at one point)."""
+def adjoint_rates(pk, ctx, strata, freqs, steer, vt):
+    """Gradient vectors per second of |S21|^2 co-polar TE and TM with respect to the 8
+    corner coordinates (one gradient vector = 8 derivatives of one objective at one
+    point), FP64 and mixed forward/adjoint solves."""
     import numpy as np
     from pssfss_amd import adjoint as A
-    corners = [[2.0, 2.0], [10.0, 2.0], [10.0, 10.0], [2.0, 10.0]]
-    sh, vt = A.bilinear_patch(corners, 39, 39, 12.0, 12.0)
-    L = pk.Layer.make
-    strata = [L(), sh, L(width=1.0, units="mm", epsr=3.0), L(width=3.0, units="mm", epsr=1.1),
-              L()]
-    freqs = np.linspace(8.0, 12.0, 12)
-    steer = {"theta": [0.0], "phi": [0.0]}
-    setup = pk.analyze.prepare(strata, freqs, steer)
-    out = {"faces": sh.nface, "unknowns": int(setup.rwgdat[0].nbf), "points": len(freqs)}
-    res = {}
-    for p in ("fp64", "mixed"):
-        s = hip.SheetSolver(ctx, setup, p)
-        s.run()                                   # warm-up (J/K integrals, allocations)
-        t0 = time.perf_counter()
-        res[p], _ = s.run()
-        dt = time.perf_counter() - t0
-        out[f"points_per_s_{p}"] = round(len(freqs) / dt, 3)
-        s.close()
-        print(f"[bench] two-layer patch {p}: {dt:.3f} s for {len(freqs)} points", flush=True)
-    out["speedup_mixed_vs_fp64"] = round(out["points_per_s_mixed"] / out["points_per_s_fp64"], 3)
-    out["max_dS_mixed_vs_fp64"] = max_dS(res["mixed"], res["fp64"])
     obj = [(2, 1, 1, 1), (2, 1, 2, 2)]
-    grads = {}
+    out, grads = {}, {}
     for p in ("fp64", "mixed"):
+        A.gradients(strata, freqs[:1], steer, vt, obj, ctx=ctx, precision=p)   # warm-up
         t0 = time.perf_counter()
         grads[p], _ = A.gradients(strata, freqs, steer, vt, obj, ctx=ctx, precision=p)
         dt = time.perf_counter() - t0
         g = grads[p]
-        out[f"adjoint_grads_per_s_{p}"] = round(g.shape[0] * g.shape[1] / dt, 3)
-        out[f"adjoint_s_{p}"] = round(dt, 3)
-        print(f"[bench] two-layer patch adjoint {p}: {dt:.3f} s for {g.shape[0]} points x "
-              f"{g.shape[1]} objectives x {g.shape[2]} parameters", flush=True)
+        out[f"grads_per_s_{p}"] = round(g.shape[0] * g.shape[1] / dt, 3)
+        print(f"[bench] adjoint {p}: {dt:.3f} s for {g.shape[0]} points x {g.shape[1]} "
+              f"objectives x {g.shape[2]} parameters", flush=True)
     d = np.abs(grads["mixed"] - grads["fp64"]) / np.abs(grads["fp64"]).max(axis=-1, keepdims=True)
-    out["adjoint_max_rel_mixed_vs_fp64"] = float(d.max())
+    out["max_rel_mixed_vs_fp64"] = float(d.max())
+    return out
+
+
+def ring_detail(pk, hip, ctx, steps=2):
+    """The oblique ring sweep (2016 faces, 2856 unknowns, 102 points, a new phasing
+    key on every point): points/s and stages of both precisions, and the spatial
+    sums' rate (8 flops per lattice point of the reference's ring rule)."""
+    strata, freqs, steer = pk.workloads.ring_on_slab()
+    setup = pk.analyze.prepare(strata, freqs, steer)
+    npts = len(list(setup.points()))
+    out, res = {"points": npts, "unknowns": int(setup.rwgdat[0].nbf)}, {}
+    for p in ("fp64", "mixed"):
+        t, res[p], _, st = timed_sweeps(hip, ctx, setup, p, steps, 1, "ring")
+        out[f"points_per_s_{p}"] = round(npts / (sum(t) / len(t)), 3)
+        out[f"stage_ms_per_step_{p}"] = stage_summary(st, steps)
+        jk = st["spatial_sums"]
+        ach = jk["flops"] / (jk["ms"] * 1e-3) / 1e12 if jk["ms"] > 0 else 0.0
+        out[f"spatial_sums_{p}"] = {"achieved": round(ach, 3), "peak": PEAK[p],
+                                    "unit": "TFLOP/s", "frac": round(ach / PEAK[p], 4)}
+    out["spatial_sums_mixed"]["traffic"] = JK_MFMA_TRAFFIC_BYTES
+    out["speedup_mixed_vs_fp64"] = round(out["points_per_s_mixed"] / out["points_per_s_fp64"], 3)
+    out["max_dS_mixed_vs_fp64"] = max_dS(res["mixed"], res["fp64"])
     return out
 
 
@@ -244,20 +270,14 @@ def _sweep(hip, ctx, setup, label):
     are those analysed (the sweep skips cut-off points as the reference does)."""
     out, res = {}, {}
     for p in ("fp64", "mixed"):
-        s = hip.SheetSolver(ctx, setup, p)
-        s.run()
-        t0 = time.perf_counter()
-        res[p], info = s.run()
-        dt = time.perf_counter() - t0
-        s.close()
+        t, res[p], info, _ = timed_sweeps(hip, ctx, setup, p, 1, 1, label)
         npts = len(res[p])
         out["points"] = npts
-        out[f"points_per_s_{p}"] = round(npts / dt, 4)
+        out[f"points_per_s_{p}"] = round(npts / t[0], 4)
         if p == "mixed":
             it = [i.refine_iters for i in info["infos"]]
             out["mixed_refinement_sweeps_min_max"] = [min(it), max(it)]
             out["mixed_fp64_fallbacks"] = int(sum(i.fallback_fp64 for i in info["infos"]))
-        print(f"[bench] {label} {p}: {dt:.3f} s for {npts} points", flush=True)
     out["speedup_mixed_vs_fp64"] = round(out["points_per_s_mixed"] / out["points_per_s_fp64"], 3)
     out["max_dS_mixed_vs_fp64"] = max_dS(res["mixed"], res["fp64"])
     return out
@@ -308,8 +328,9 @@ def cross_batch(pk, hip, ctx):
 
 
 def cpu_baseline(pk, strata, freqs, steer, gpu_results):
-    """Oracle (FP64 CPU restatement) on the first point of the sweep: points/s on
-    the host's cores, and max|dS| of each GPU path against it at that point."""
+    """Oracle (FP64 CPU restatement) on the first point of the sweep, which carries
+    the sweep's spatial sums as the reference's first point does: points/s on the
+    host's cores, and max|dS| of each GPU path against it at that point."""
     import oracle
     f0 = [float(freqs[0])]
     st0 = {k: [list(v)[0]] for k, v in steer.items()}
@@ -321,10 +342,19 @@ def cpu_baseline(pk, strata, freqs, steer, gpu_results):
     for p, res in gpu_results.items():
         errs[f"max_dS_{p}_vs_oracle_point0"] = max_dS(res[:1], ref)
     keys = list(steer.keys())
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     sample = (f"1 point ({f0[0]} GHz, {keys[0]}={st0[keys[0]][0]}, {keys[1]}={st0[keys[1]][0]}) "
-              f"of the same workload, full mesh, {dt:.1f} s")
+              f"of the same workload, full mesh, spatial sums included, {dt:.1f} s")
     return ({"value": round(1.0 / dt, 5), "unit": "points/s", "cores": cores, "kind": "port",
-             "sample": sample}, errs)
+             "cpu": cpu_model, "sample": sample}, errs)
 
 
 if __name__ == "__main__":

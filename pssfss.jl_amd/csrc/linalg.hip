@@ -34,7 +34,9 @@ constexpr int SB = 64;   // sub-panel width: triangular blocks held in LDS
 // operand reads.  Its error bound is normwise (Higham, "Stability of a method for
 // multiplying complex matrices with three real matrix multiplications", 1992);
 // the FP32 factors it builds are only used under FP64 refinement.
-template <class C, int WT, bool TA>
+// TAG only names instantiations apart in kernel traces (1: the LU's trailing
+// updates); the code is the same.
+template <class C, int WT, bool TA, int TAG = 0>
 __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                                                    const C *__restrict__ A, int64_t lda,
                                                    int64_t sA, const C *__restrict__ B,
@@ -1015,13 +1017,13 @@ unsigned grid1(int64_t n) {
     return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
 }
 
-template <class C, bool TA = false>
+template <class C, bool TA = false, int TAG = 0>
 void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, const C *B,
             int64_t ldb, int64_t sB, C beta, C *Cm, int64_t ldc, int64_t sC, int batch,
             hipStream_t st, const int32_t *sel = nullptr) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
     dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
-    hipLaunchKernelGGL((k_gemm_mfma<C, 2, TA>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA,
+    hipLaunchKernelGGL((k_gemm_mfma<C, 2, TA, TAG>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA,
                        B, ldb, sB, beta, Cm, ldc, sC, sel);
     HIPCHECK(hipGetLastError());
 }
@@ -1140,8 +1142,8 @@ void update_t(int64_t n, int64_t kk, int kw, int64_t c1, C *A, int64_t sA, int b
     C mone, one;
     mone.x = -1; mone.y = 0;
     one.x = 1; one.y = 0;
-    gemm_t<C>((int)ncols, (int)nrows, kw, mone, A + kk * n + r0, n, sA, A + r0 * n + kk, n, sA,
-              one, A + r0 * n + r0, n, sA, batch, st);
+    gemm_t<C, false, 1>((int)ncols, (int)nrows, kw, mone, A + kk * n + r0, n, sA,
+                        A + r0 * n + kk, n, sA, one, A + r0 * n + r0, n, sA, batch, st);
 }
 
 // Panel [k, k + kb) of rows [k, n): into the scratch, then per sub-panel of SB

@@ -405,11 +405,18 @@ class SheetSolver:
     """Runs the `_analyze` sweep with every sheet GSM computed on the device,
     all points of one sheet Gblock in one batched call."""
 
-    def __init__(self, ctx: Context, setup, precision: str = "fp64"):
+    def __init__(self, ctx: Context, setup, precision: str = "fp64", max_batch: int = 0):
         self.ctx = ctx
         self.setup = setup
         self.precision = precision
+        self.max_batch = max_batch      # points per library call; 0 = all points of a sheet
         self.dsheets = {}
+
+    def upload(self):
+        """Upload every analysed sheet now (otherwise the first run() does it)."""
+        for i, sh in enumerate(self.setup.sheets):
+            if sh.style != "NULL" and sh.sheet_class in ("J", "M"):
+                self._dsheet(i)
 
     def _dsheet(self, isheet):
         if isheet not in self.dsheets:
@@ -447,10 +454,13 @@ class SheetSolver:
         for ig, reqs in requests.items():
             ish = su.junc[su.gbls[ig].j - 1]
             ds = self._dsheet(ish - 1)
-            gsms, infos = sheet_gsm_batch(self.ctx, ds, [b for _, b in reqs], self.precision)
-            for (ip, _), g, inf in zip(reqs, gsms, infos):
-                solved[(ip, ig)] = g
-            infos_all += infos
+            step = self.max_batch if self.max_batch > 0 else len(reqs)
+            for r0 in range(0, len(reqs), step):
+                part = reqs[r0:r0 + step]
+                gsms, infos = sheet_gsm_batch(self.ctx, ds, [b for _, b in part], self.precision)
+                for (ip, _), g, inf in zip(part, gsms, infos):
+                    solved[(ip, ig)] = g
+                infos_all += infos
         results = []
         for ip in live:
             pt = points[ip]
