@@ -7,9 +7,13 @@ FP64 CPU restatement of PSSFSS.jl's per-frequency MoM path:
     src/FillZY.jl, src/RWG.jl:428-529);
   * LAPACK zgetrf/zgetrs through scipy for lu! / ldiv! (src/PSSFSS.jl:401,439);
   * RHS, solve and GSM extraction of calculate_jtype_gsm / calculate_mtype_gsm
-    (src/PSSFSS.jl:362-587) and the _analyze loop (src/PSSFSS.jl:199-334),
-    using the host-side TL/mode helpers of the package (pinned separately by
-    the reference's GSM/mode tests).
+    (src/PSSFSS.jl:362-587);
+  * tl_oracle.py: modal constants, the Gblock transmission-line partial GSMs and
+    Green's functions, junction GSMs, the star product, the steering points and
+    the Gblock walk of _analyze (src/Modes.jl:298-322, src/GSMs.jl:16-750,
+    src/PSSFSS.jl:230-305), restated independently of the package.
+Only mode and Gblock selection (pinned by the reference's own goldens) and the
+Result record come from the package.
 """
 import ctypes as C
 import math
@@ -162,24 +166,24 @@ class SheetCache:
         self.val = None
 
 
-def sheet_point(pk, layers, sheet, u, rwgdat, s, k0, kinc, cache=None, want_z=False):
+_TOL_U = 1e-4    # src/Constants.jl: tolerance of the J/K cache key on u
+
+
+def sheet_point(region, sheet, u, rwgdat, s, k0, kinc, cache=None, want_z=False):
     """calculate_jtype_gsm / calculate_mtype_gsm (src/PSSFSS.jl:362-587) in FP64.
 
-    `pk` is the package module (pssfss_amd) providing the host TL helpers."""
-    from pssfss_amd.constants import TOL
-    gsmmod = pk.gsm
+    region: tl_oracle.ModalLayer list, set up at the point.  Returns the Gblock's
+    GSM as [s11, s12, s21, s22] and a dict of extras."""
+    import tl_oracle as T
     upm = sheet.upm
     area = abs(sheet.s1[0] * sheet.s2[1] - sheet.s1[1] * sheet.s2[0]) / upm ** 2
-    ends = (layers[0], layers[-1])
-    acf = []
-    for l in ends:
-        ai = (2 * math.pi) * (2 * math.pi) / abs(l.beta1[0] * l.beta2[1] - l.beta1[1] * l.beta2[0])
-        acf.append(math.sqrt(ai / area))
+    ends = (region[0], region[-1])
+    acf = [math.sqrt(((2 * math.pi) * (2 * math.pi) / l.cell_area_recip()) / area) for l in ends]
     magnetic = sheet.sheet_class == "M"
     if magnetic:
-        gsm, tlgf, vinc = gsmmod.gsm_magnetic_gblock(layers, s, k0)
+        gsm, tlgf, vinc = T.gblock_magnetic(region, s, k0)
     else:
-        gsm, tlgf, vinc = gsmmod.gsm_electric_gblock(layers, s, k0)
+        gsm, tlgf, vinc = T.gblock_electric(region, s, k0)
     psi1 = float(kinc @ sheet.s1) / upm
     psi2 = float(kinc @ sheet.s2) / upm
     b1m, b2m = sheet.beta1 * upm, sheet.beta2 * upm
@@ -187,12 +191,12 @@ def sheet_point(pk, layers, sheet, u, rwgdat, s, k0, kinc, cache=None, want_z=Fa
     ulocal = u / upm
     cache = cache if cache is not None else SheetCache()
     if (cache.key is None or psi1 != cache.key[0] or psi2 != cache.key[1] or cache.key[2] == 0
-            or abs((ulocal - cache.key[2]) / cache.key[2]) > TOL):
+            or abs((ulocal - cache.key[2]) / cache.key[2]) > _TOL_U):
         cache.val = filljk(sheet, rwgdat, ulocal, psi1, psi2)
         cache.key = (psi1, psi2, ulocal)
     JK, rho_r, rinv = cache.val
-    tabs = modal_tables(magnetic, k0, u, psi1, psi2, layers, s, b1m, b2m, b00, 1e-7)
-    Z = fill(sheet, rwgdat, JK, rho_r, rinv, k0, u, layers, s, tabs[:3], b1m, b2m, psi1, psi2)
+    tabs = modal_tables(magnetic, k0, u, psi1, psi2, region, s, b1m, b2m, b00, 1e-7)
+    Z = fill(sheet, rwgdat, JK, rho_r, rinv, k0, u, region, s, tabs[:3], b1m, b2m, psi1, psi2)
     Zkeep = Z.copy() if want_z else None
     lu = sla.lu_factor(Z, overwrite_a=True, check_finite=False)
     nbf = rwgdat.nbf
@@ -204,10 +208,11 @@ def sheet_point(pk, layers, sheet, u, rwgdat, s, k0, kinc, cache=None, want_z=Fa
             if qp > 0 and _approx(kv, l.beta[qp - 1]):
                 bfft[:, sr, qp] = bfft[:, sr, qp - 1]
                 continue
-            if sr == 1 and len(layers[0].beta) > qp and _approx(kv, layers[0].beta[qp]):
+            if sr == 1 and len(region[0].beta) > qp and _approx(kv, region[0].beta[qp]):
                 bfft[:, 1, qp] = bfft[:, 0, qp]
                 continue
             bfft[:, sr, qp] = rwgbfft(sheet, rwgdat, kv, psi1, psi2)
+    blk = {(1, 1): 0, (1, 2): 1, (2, 1): 2, (2, 2): 3}
     sigma_ = -1
     for sr, ls in enumerate(ends):
         sigma_ *= -1
@@ -222,15 +227,14 @@ def sheet_point(pk, layers, sheet, u, rwgdat, s, k0, kinc, cache=None, want_z=Fa
             rhs = np.conj(b[:, 0]) * src[0] + np.conj(b[:, 1]) * src[1]
             x = sla.lu_solve(lu, rhs, check_finite=False)
             for orr, lo in enumerate(ends):
-                smat = gsm[(orr + 1, sr + 1)]
+                smat = gsm[blk[(orr + 1, sr + 1)]]
                 for q in range(len(lo.P)):
                     FT = x @ bfft[:, orr, q]
+                    t = lo.tvec[q]
                     if magnetic:
-                        t = lo.tvec[q]
                         zt = np.array([-t[1], t[0]])
                         smat[q, qp] += (zt[0] * FT[0] + zt[1] * FT[1]) * (sigma_ * tlgf[q, orr] * lo.c[q])
                     else:
-                        t = lo.tvec[q]
                         smat[q, qp] -= (t[0] * FT[0] + t[1] * FT[1]) * (
                             tlgf[q, orr] / (lo.c[q] * acf[orr] * area))
     return gsm, {"Z": Zkeep, "mmax": tabs[0], "converged": tabs[3], "psi": (psi1, psi2)}
@@ -243,29 +247,32 @@ def _approx(a, b):
 
 # ------------------------------------------------------------------- analyze
 def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False, dbmin=None):
-    """The `_analyze` loop (src/PSSFSS.jl:199-334) with the oracle sheet solver.
+    """The `_analyze` loop (src/PSSFSS.jl:199-334) with the oracle's sheet solver and
+    transmission-line side (tl_oracle); mode and Gblock selection from the package's
+    prepare().
 
     steering: dict with keys 'theta','phi' (degrees) or 'psi1','psi2' (degrees);
     the first key is the outer loop.  Returns a list of result dicts."""
+    import tl_oracle as T
     kw = {} if dbmin is None else {"dbmin": dbmin}
     setup = pk.analyze.prepare(strata, freqs, steering, mode_override=mode_override, **kw)
+    ml = [T.ModalLayer(l) for l in setup.layers]
     caches = [SheetCache() for _ in setup.sheets]
     results = []
-    for point in setup.points():
-        k0, b00 = point.k0, point.beta00
-        for l in setup.layers:
-            pk.modes.setup_modes(l, k0, b00)
-        if not setup.principal_ok():
+    for fghz, k0, b00, steer in T.points(setup):
+        for l in ml:
+            l.setup(k0, b00)
+        if not T.principal_propagates(ml[0], ml[-1]):
             continue
         extras = []
 
         def sheet_solver(region, sheet, u, rwgdat, s, isheet):
-            g, ex = sheet_point(pk, region, sheet, u, rwgdat, s, k0, b00, caches[isheet],
+            g, ex = sheet_point(region, sheet, u, rwgdat, s, k0, b00, caches[isheet],
                                 want_z=want_z)
             extras.append(ex)
             return g
-        gsm = setup.cascade_point(k0, sheet_solver)
-        res = pk.outputs.make_result(gsm, point.steer, b00, point.fghz, setup.layers)
+        g = T.walk(setup, ml, k0, sheet_solver)
+        res = pk.outputs.make_result(pk.GSM(*g), steer, b00, fghz, setup.layers)
         res.extras = extras
         results.append(res)
     return results

@@ -469,7 +469,7 @@ def test_resistive_sheet_closed_form(ctx, pk, orc):
     polarisations.  S against the oracle to 1e-9 and against the closed form to
     1e-4 (the oracle itself meets it to about 1e-5 on this 8 x 8 mesh)."""
     _say("full-cell resistive sheet vs the transmission-line closed form")
-    eta0 = 376.730313412
+    eta0 = math.sqrt(1.25663706212e-6 / (1 / (1.25663706212e-6 * 299792458.0 ** 2)))
     R = 120.0
     sh = pk.rectstrip(Lx=10, Ly=10, Px=10, Py=10, Nx=8, Ny=8, units="mm", Rsheet=R)
     strata = [pk.Layer.make(), sh, pk.Layer.make()]
