@@ -55,6 +55,7 @@ def lib():
         L.orc_rwgbfft.argtypes = [C.c_int64, _d, C.c_double, _i, _i, C.c_int64, _i, _i, _i,
                                   _d, C.c_double, C.c_double, _d]
         L.orc_fft2.argtypes = [_d, C.c_int]
+        L.orc_decisions.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_num_threads.restype = C.c_int
         _LIB = L
     return _LIB
@@ -156,6 +157,35 @@ def rwgbfft(sheet, rwgdat, k, psi1, psi2):
     return ft.view(np.complex128).reshape(nbf, 2)
 
 
+class FrozenDecisions:
+    """The oracle's discrete choices (rings summed, table cells and Floquet shifts,
+    interpolation snaps, singular-integral branches), recorded per analysed point at
+    one geometry and replayed at nearby geometries of the same mesh topology, so
+    that central differences of the restatement see a smooth function:
+
+        fz = FrozenDecisions(nufp, npoints)
+        analyze(pk, base, freqs, steer, decisions=(fz, "record"))
+        analyze(pk, moved, freqs, steer, decisions=(fz, "replay"))
+
+    (the same frequencies and steering each time: the Gblock partition and the
+    table sizes depend on them)."""
+
+    MODES = {"record": 1, "replay": 2}
+
+    def __init__(self, nufp, npoints=1):
+        self.rings = np.zeros((npoints, 7 * nufp), np.int8)
+        self.cells = np.zeros((npoints, 42 * nufp), np.int16)
+        self.sing = np.zeros((npoints, nufp), np.uint8)
+
+    def select(self, mode, ipoint):
+        lib().orc_decisions(self.MODES[mode], self.rings[ipoint].ctypes.data,
+                            self.cells[ipoint].ctypes.data, self.sing[ipoint].ctypes.data)
+
+    @staticmethod
+    def off():
+        lib().orc_decisions(0, None, None, None)
+
+
 # ---------------------------------------------------------- sheet GSM (one point)
 class SheetCache:
     """The reference keeps the J/K integrals on the sheet keyed by (psi1, psi2, u)
@@ -246,13 +276,15 @@ def _approx(a, b):
 
 
 # ------------------------------------------------------------------- analyze
-def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False, dbmin=None):
+def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False, dbmin=None,
+            decisions=None):
     """The `_analyze` loop (src/PSSFSS.jl:199-334) with the oracle's sheet solver and
     transmission-line side (tl_oracle); mode and Gblock selection from the package's
     prepare().
 
     steering: dict with keys 'theta','phi' (degrees) or 'psi1','psi2' (degrees);
-    the first key is the outer loop.  Returns a list of result dicts."""
+    the first key is the outer loop.  decisions: (FrozenDecisions, "record" or
+    "replay"), one record per analysed point.  Returns a list of result dicts."""
     import tl_oracle as T
     kw = {} if dbmin is None else {"dbmin": dbmin}
     setup = pk.analyze.prepare(strata, freqs, steering, mode_override=mode_override, **kw)
@@ -265,6 +297,8 @@ def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False, dbmin
         if not T.principal_propagates(ml[0], ml[-1]):
             continue
         extras = []
+        if decisions is not None:
+            decisions[0].select(decisions[1], len(results))
 
         def sheet_solver(region, sheet, u, rwgdat, s, isheet):
             g, ex = sheet_point(region, sheet, u, rwgdat, s, k0, b00, caches[isheet],
@@ -275,4 +309,6 @@ def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False, dbmin
         res = pk.outputs.make_result(pk.GSM(*g), steer, b00, fghz, setup.layers)
         res.extras = extras
         results.append(res)
+    if decisions is not None:
+        FrozenDecisions.off()
     return results

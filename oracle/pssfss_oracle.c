@@ -66,9 +66,28 @@ static inline void ring_mn(int r, int k, int *m, int *n) {
 
 /* ---------------------------------------------------------------- jksums */
 /* src/PGF.jl:67-106. urho00, us1, us2 are u-scaled 2-vectors (sheet units). */
-void orc_jksums(const double *urho00, double psi1, double psi2, const double *us1,
-                const double *us2, int extract, double convtest, double *jsum_out,
-                double *ksum_out, int *rings_out) {
+/* Frozen decisions (finite-difference references, test infrastructure only): in
+ * mode 1 every discrete choice of the spatial sums (rings summed), the table
+ * interpolation (cell, Floquet shift, the |xi| < 1e-8 snap) and the singular
+ * integrals (the |p0/l| < 1e-4 branch) is recorded per class and quadrature point;
+ * in mode 2 the recorded choices are replayed instead of re-deciding, so that
+ * the restatement is a smooth function of the node positions around the point
+ * where they were recorded.  Mode 0: the reference's rules. */
+static int g_dmode = 0;
+static int8_t *g_rings = NULL;      /* 7 per class */
+static int16_t *g_cells = NULL;     /* 6 x 7 per class: m, n, msh, nsh, snap1, snap2 */
+static uint8_t *g_sing = NULL;      /* 1 per class: bit i = small-p0 branch of edge i */
+
+void orc_decisions(int mode, int8_t *rings, int16_t *cells, uint8_t *sing) {
+    g_dmode = mode;
+    g_rings = mode ? rings : NULL;
+    g_cells = mode ? cells : NULL;
+    g_sing = mode ? sing : NULL;
+}
+
+static void jksums_impl(const double *urho00, double psi1, double psi2, const double *us1,
+                        const double *us2, int extract, double convtest, double *jsum_out,
+                        double *ksum_out, int *rings_out, int8_t *rslot) {
     const int jkringmax = 65, rmin = 5;
     const double small = 1e-5;
     double arg = sqrt(urho00[0] * urho00[0] + urho00[1] * urho00[1]);
@@ -99,15 +118,27 @@ void orc_jksums(const double *urho00, double psi1, double psi2, const double *us
         }
         jsum += jring;
         ksum += kring;
+        if (rslot && g_dmode == 2) {
+            if (r >= *rslot) break;
+            continue;
+        }
         if (r > rmin) {
             double c1 = cabs(jring / jsum), c2 = cabs(kring / ksum);
             double conv = c1 > c2 ? c1 : c2;
             if (conv < convtest) break;
         }
     }
+    if (rslot && g_dmode == 1) *rslot = (int8_t)rsave;
     jsum_out[0] = creal(jsum); jsum_out[1] = cimag(jsum);
     ksum_out[0] = creal(ksum); ksum_out[1] = cimag(ksum);
     if (rings_out) *rings_out = rsave;
+}
+
+void orc_jksums(const double *urho00, double psi1, double psi2, const double *us1,
+                const double *us2, int extract, double convtest, double *jsum_out,
+                double *ksum_out, int *rings_out) {
+    jksums_impl(urho00, psi1, psi2, us1, us2, extract, convtest, jsum_out, ksum_out, rings_out,
+                NULL);
 }
 
 /* --------------------------------------------------------------- filljk! */
@@ -146,7 +177,8 @@ void orc_filljk(int64_t nface, const double *rho, const int64_t *fv, int64_t nuf
                             (rs[0][1] + rs21[1] * QXI[i]) + rs31[1] * QETA[i]};
             double ur[2] = {ulocal * (rmc[0] - rt[0]), ulocal * (rmc[1] - rt[1])};
             double js[2], ks[2];
-            orc_jksums(ur, psi1, psi2, us1, us2, 1, 1e-8, js, ks, NULL);
+            jksums_impl(ur, psi1, psi2, us1, us2, 1, 1e-8, js, ks, NULL,
+                        g_rings ? g_rings + 7 * iu + i : NULL);
             cplx jw = mk(js[0], js[1]) * QW[i];
             Jx += jw * QXI[i]; Je += jw * QETA[i]; J += jw;
             cplx kw = mk(ks[0], ks[1]) * QW[i];
@@ -166,7 +198,13 @@ void orc_filljk(int64_t nface, const double *rho, const int64_t *fv, int64_t nuf
             double ledge = lp - lm;
             double pplus = sqrt(p0sq + lp * lp), pminus = sqrt(p0sq + lm * lm);
             double factor;
-            if (fabs(p0 / ledge) < 1e-4) {
+            int small_p0 = fabs(p0 / ledge) < 1e-4;
+            if (g_sing) {
+                if (g_dmode == 2) small_p0 = (g_sing[iu] >> i) & 1;
+                else if (small_p0) g_sing[iu] |= (uint8_t)(1u << i);
+                else g_sing[iu] &= (uint8_t)~(1u << i);
+            }
+            if (small_p0) {
                 factor = lp * pplus - lm * pminus;
             } else {
                 factor = p0 * log((pplus + lp) / (pminus + lm));
@@ -484,23 +522,41 @@ int orc_modal_tables(int magnetic, double k0, double u, double psi1, double psi2
 
 /* Sigma_m interpolation, src/PGF.jl:392-443.  beta1/beta2 in 1/m, rho in m. */
 static cplx sigma_interp(const cplx *tab, int M, const double *b1, const double *b2,
-                         double psi1, double psi2, double rx, double ry) {
+                         double psi1, double psi2, double rx, double ry, int16_t *cslot) {
     double x1o = (b1[0] * rx + b1[1] * ry) / TWOPI;
-    if (fabs(x1o) < 1e-8) x1o = 0.0;
     double x2o = (b2[0] * rx + b2[1] * ry) / TWOPI;
-    if (fabs(x2o) < 1e-8) x2o = 0.0;
-    /* Julia mod(x, 1) */
-    double x1 = fmod(x1o, 1.0);
-    if (x1 == 0) x1 = 0.0; else if (x1 < 0) x1 = x1 + 1.0;
-    double x2 = fmod(x2o, 1.0);
-    if (x2 == 0) x2 = 0.0; else if (x2 < 0) x2 = x2 + 1.0;
-    double msh = rint(x1o - x1), nsh = rint(x2o - x2);
-    int m = (int)(M * x1);
-    double p = M * x1 - m;
-    int n = (int)(M * x2);
-    double q = M * x2 - n;
-    if (m > M - 1) m = M - 1;   /* guard: M*x rounding up to M */
-    if (n > M - 1) n = M - 1;
+    double x1, x2, msh, nsh, p, q;
+    int m, n;
+    if (cslot && g_dmode == 2) {            /* the recorded cell, shift and snaps */
+        if (cslot[4]) x1o = 0.0;
+        if (cslot[5]) x2o = 0.0;
+        m = cslot[0]; n = cslot[1];
+        msh = cslot[2]; nsh = cslot[3];
+        x1 = x1o - msh; x2 = x2o - nsh;
+        p = M * x1 - m;
+        q = M * x2 - n;
+    } else {
+        int snap1 = fabs(x1o) < 1e-8, snap2 = fabs(x2o) < 1e-8;
+        if (snap1) x1o = 0.0;
+        if (snap2) x2o = 0.0;
+        /* Julia mod(x, 1) */
+        x1 = fmod(x1o, 1.0);
+        if (x1 == 0) x1 = 0.0; else if (x1 < 0) x1 = x1 + 1.0;
+        x2 = fmod(x2o, 1.0);
+        if (x2 == 0) x2 = 0.0; else if (x2 < 0) x2 = x2 + 1.0;
+        msh = rint(x1o - x1); nsh = rint(x2o - x2);
+        m = (int)(M * x1);
+        p = M * x1 - m;
+        n = (int)(M * x2);
+        q = M * x2 - n;
+        if (m > M - 1) m = M - 1;   /* guard: M*x rounding up to M */
+        if (n > M - 1) n = M - 1;
+        if (cslot && g_dmode == 1) {
+            cslot[0] = (int16_t)m; cslot[1] = (int16_t)n;
+            cslot[2] = (int16_t)msh; cslot[3] = (int16_t)nsh;
+            cslot[4] = (int16_t)snap1; cslot[5] = (int16_t)snap2;
+        }
+    }
     double p2 = p * p, q2 = q * q, pq = p * q;
     size_t E = (size_t)M + 2;
 #define T(a, b) tab[(size_t)((a) + 1) + E * (size_t)((b) + 1)]
@@ -517,7 +573,7 @@ static cplx sigma_interp(const cplx *tab, int M, const double *b1, const double 
 
 void orc_sigma(const double *tab, int M, const double *b1, const double *b2, double psi1,
                double psi2, const double *rho, double *out) {
-    cplx v = sigma_interp((const cplx *)tab, M, b1, b2, psi1, psi2, rho[0], rho[1]);
+    cplx v = sigma_interp((const cplx *)tab, M, b1, b2, psi1, psi2, rho[0], rho[1], NULL);
     out[0] = creal(v); out[1] = cimag(v);
 }
 
@@ -600,8 +656,9 @@ void orc_fill(int magnetic, int64_t nface, const double *rho, double upm, const 
                             (rs[0][1] + (rs[1][1] - rs[0][1]) * QXI[i]) +
                                 (rs[2][1] - rs[0][1]) * QETA[i]};
             double rd[2] = {rmc[0] - rt[0], rmc[1] - rt[1]};
-            cplx sg1 = sigma_interp(t1, M, beta1, beta2, psi1, psi2, rd[0], rd[1]);
-            cplx sg2 = sigma_interp(t2, M, beta1, beta2, psi1, psi2, rd[0], rd[1]);
+            int16_t *cs = g_cells ? g_cells + 6 * (7 * iu + i) : NULL;
+            cplx sg1 = sigma_interp(t1, M, beta1, beta2, psi1, psi2, rd[0], rd[1], cs);
+            cplx sg2 = sigma_interp(t2, M, beta1, beta2, psi1, psi2, rd[0], rd[1], cs);
             cplx sw = sg1 * QW[i];
             I1x += sw * QXI[i];
             I1e += sw * QETA[i];

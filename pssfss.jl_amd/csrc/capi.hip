@@ -138,6 +138,7 @@ struct pssfss_ctx {
     DBuf<cd> wG;                 // fill class terms
     DBuf<double> wanorm;         // infinity norms of the systems (refinement)
     std::set<pssfss_sheet *> sheets;   // sheets uploaded and not yet freed
+    LuOverlap lu_ov;             // look-ahead LU streams (panel / update CU subsets)
     const cd *dft_matrix(int M) {
         auto it = dft.find(M);
         if (it != dft.end()) return it->second.p;
@@ -788,7 +789,7 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wperm.ensure((size_t)n * np);
     if (precision == PSSFSS_FP64) {
         ctx->wT.ensure((size_t)sB * np);
-        lu_factor_batched(n, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, st);
+        lu_factor_batched(n, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, st, &ctx->lu_ov);
         stage_begin(5, st);
         lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
         lu_solve_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wperm.p, n, ctx->wB.p, sB, ctx->wT.p, st);
@@ -804,7 +805,7 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wanorm.ensure((size_t)np + 2);
     convert_z_to_f32_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wZf.p, ctx->wanorm.p, st);
     stage_end(3, st);
-    lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st);
+    lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st, &ctx->lu_ov);
     stage_begin(5, st);
     lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
     ctx->wDinv.ensure((size_t)np * 2 * ((n + 63) / 64) * 4096);
@@ -1333,6 +1334,47 @@ void upload_sheet(pssfss_ctx *ctx, const pssfss_sheet_desc *d, pssfss_sheet *sh)
 // ===================================================================== C ABI
 extern "C" {
 
+namespace {
+// CUs given to the look-ahead LU's panel stream; the update stream gets the rest.
+// Batches of more systems than this keep the single-stream factorisation (each
+// system's leaf takes a CU of its own).
+constexpr int LU_PANEL_CUS = 32;
+
+void destroy_overlap(LuOverlap &ov) {
+    for (hipEvent_t *e : {&ov.e_start, &ov.e_u12, &ov.e_panel, &ov.e_done})
+        if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+    if (ov.panel) (void)hipStreamDestroy(ov.panel), ov.panel = nullptr;
+    if (ov.update) (void)hipStreamDestroy(ov.update), ov.update = nullptr;
+    ov.max_batch = 0;
+}
+
+// Two streams for the look-ahead LU (default priority): the update stream is kept
+// off the first LU_PANEL_CUS CUs; on any failure the context keeps the
+// single-stream factorisation.
+void init_overlap(pssfss_ctx *c) {
+    LuOverlap &ov = c->lu_ov;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, c->device) != hipSuccess) { (void)hipGetLastError(); return; }
+    const int ncu = p.multiProcessorCount;
+    if (ncu < 4 * LU_PANEL_CUS) return;
+    const int words = (ncu + 31) / 32;
+    std::vector<uint32_t> mu(words, 0);
+    for (int cu = LU_PANEL_CUS; cu < ncu; ++cu) mu[cu / 32] |= 1u << (cu % 32);
+    // the panel stream may use every CU; the update stream leaves the first
+    // LU_PANEL_CUS to the panel's leaves
+    bool ok = hipStreamCreate(&ov.panel) == hipSuccess &&
+              hipExtStreamCreateWithCUMask(&ov.update, words, mu.data()) == hipSuccess;
+    for (hipEvent_t *e : {&ov.e_start, &ov.e_u12, &ov.e_panel, &ov.e_done})
+        ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        destroy_overlap(ov);
+        return;
+    }
+    ov.max_batch = LU_PANEL_CUS;
+}
+}  // namespace
+
 int pssfss_ctx_create(int device, pssfss_ctx **out) {
     if (!out) return PSSFSS_EINVAL;
     *out = nullptr;
@@ -1350,6 +1392,7 @@ int pssfss_ctx_create(int device, pssfss_ctx **out) {
         delete c;
         return PSSFSS_EHIP;
     }
+    init_overlap(c);
     *out = c;
     return PSSFSS_OK;
 }
@@ -1362,6 +1405,7 @@ int pssfss_ctx_destroy(pssfss_ctx *ctx) {
     for (pssfss_sheet *sh : ctx->sheets) delete sh;
     ctx->sheets.clear();
     ctx->dft.clear();
+    destroy_overlap(ctx->lu_ov);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return PSSFSS_OK;

@@ -169,8 +169,17 @@ void zgemm_batched(int m, int n, int k, cd alpha, const cd *A, int64_t lda, int6
 void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int64_t sA,
                    const cf *B, int64_t ldb, int64_t sB, cf beta, cf *C, int64_t ldc,
                    int64_t sC, int batch, hipStream_t st);
+// Streams and events of the look-ahead factorisation (owned by the caller):
+// `update` is kept off a few CUs, where the panel's one-workgroup-per-system leaves
+// on `panel` find room; batches above max_batch (more leaves than those CUs) use
+// the single-stream factorisation.
+struct LuOverlap {
+    hipStream_t panel = nullptr, update = nullptr;
+    hipEvent_t e_start = nullptr, e_u12 = nullptr, e_panel = nullptr, e_done = nullptr;
+    int max_batch = 0;
+};
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                       hipStream_t st);
+                       hipStream_t st, const LuOverlap *ov = nullptr);
 // Scratch entries per system of lu_factor_batched_ws, and the same factorisation
 // with that scratch passed in (ws holds batch * lu_factor_ws_elems(n) entries;
 // nothing is allocated).
@@ -185,7 +194,7 @@ void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
 void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t *perm,
                   hipStream_t st);
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv,
-                           int64_t sP, hipStream_t st);
+                           int64_t sP, hipStream_t st, const LuOverlap *ov = nullptr);
 // sel (optional): the batch lists system indices, one per entry.  tr: Z^T x = b.
 // dinv (optional, not with tr): the diagonal-block inverses of lu_diag_inverses_f32,
 // used in place of the sequential diagonal solves.

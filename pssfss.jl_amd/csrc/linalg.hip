@@ -23,8 +23,8 @@ constexpr int SB = 64;   // sub-panel width: triangular blocks held in LDS
 // transposed tile (B^T A^T), so the 16 lanes that share an output row of the
 // instruction hold 16 consecutive rows of the column-major C: the epilogue's
 // loads and stores are coalesced.  Four wavefronts in 2 x 2, each owning
-// (16 WT) x (16 WT) of the (32 WT) x (32 WT) workgroup tile; BK = 16 complex k
-// per LDS stage, double-buffered.  WT = 2 (64 x 64) is what the library
+// (16 WT) x (16 WT) of the (32 WT) x (32 WT) workgroup tile; BK complex k per
+// LDS stage, double-buffered: 8 in FP64, 16 in FP32.  WT = 2 (64 x 64) is what the library
 // launches: the 128 x 128 tile (WT = 4) runs one wavefront per SIMD and measured
 // slower on MI355X.  TA: A is given transposed (C = A^T B, A stored k x m), which
 // is a row-major m x k operand such as Z.  sel (optional) lists the batch entries.
@@ -1101,14 +1101,22 @@ void rowswap_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, const int32_t *ip
 // rank-SB update of the second block's rows.  Linv holds 2 x 4096 and T SB x n
 // entries per system.
 template <class C>
-void trsm_u12_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, int batch, C *Linv, C *T,
-                hipStream_t st) {
-    const int64_t c0 = k + kb, cols = n - c0;
-    if (cols <= 0) return;
+void panel_linv_t(int64_t n, int64_t k, int kb, const C *A, int64_t sA, int batch, C *Linv,
+                  hipStream_t st) {
     const int nh = (kb + SB - 1) / SB;
     hipLaunchKernelGGL(k_panel_linv<C>, dim3((unsigned)nh, (unsigned)batch), dim3(64), 0, st, n, A,
                        sA, k, kb, Linv);
     HIPCHECK(hipGetLastError());
+}
+
+// have_linv: Linv already holds the panel's diagonal-block inverses.
+template <class C>
+void trsm_u12_t(int64_t n, int64_t k, int kb, C *A, int64_t sA, int batch, C *Linv, C *T,
+                hipStream_t st, bool have_linv = false) {
+    const int64_t c0 = k + kb, cols = n - c0;
+    if (cols <= 0) return;
+    const int nh = (kb + SB - 1) / SB;
+    if (!have_linv) panel_linv_t<C>(n, k, kb, A, sA, batch, Linv, st);
     C mone, one, zero;
     mone.x = -1; mone.y = 0;
     one.x = 1; one.y = 0;
@@ -1144,6 +1152,21 @@ void update_t(int64_t n, int64_t kk, int kw, int64_t c1, C *A, int64_t sA, int b
     one.x = 1; one.y = 0;
     gemm_t<C, false, 1>((int)ncols, (int)nrows, kw, mone, A + kk * n + r0, n, sA,
                         A + r0 * n + kk, n, sA, one, A + r0 * n + r0, n, sA, batch, st);
+}
+
+// A[r0:n, c0:c1] -= A[r0:n, kk:kk+kw] A[kk:kk+kw, c0:c1] (row-major, r0 = kk + kw), any
+// column range to the right of the panel; TAG names the launches apart in traces.
+template <class C, int TAG>
+void update_cols_t(int64_t n, int64_t kk, int kw, int64_t c0, int64_t c1, C *A, int64_t sA,
+                   int batch, hipStream_t st) {
+    const int64_t r0 = kk + kw;
+    const int64_t ncols = c1 - c0, nrows = n - r0;
+    if (ncols <= 0 || nrows <= 0) return;
+    C mone, one;
+    mone.x = -1; mone.y = 0;
+    one.x = 1; one.y = 0;
+    gemm_t<C, false, TAG>((int)ncols, (int)nrows, kw, mone, A + kk * n + c0, n, sA,
+                          A + r0 * n + kk, n, sA, one, A + r0 * n + c0, n, sA, batch, st);
 }
 
 // Panel [k, k + kb) of rows [k, n): into the scratch, then per sub-panel of SB
@@ -1220,11 +1243,82 @@ void lu_factor_ws_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64
     stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
 }
 
+// The same factorisation with one panel of look-ahead: the panel stream ov.panel
+// factors panel k + 1 while the update stream ov.update (kept off the first few
+// CUs, where the panel's one-workgroup-per-system leaves run) applies panel k's
+// rank-NB update to the columns beyond panel k + 1.  Per step k, on the update
+// stream: panel k's interchanges on the columns to its right, U12 of panel k, the
+// update of panel k + 1's columns; then on the panel stream: panel k + 1's
+// factorisation and its diagonal-block inverses; on the update stream, at the same
+// time, the update of the remaining columns.  The interchanges of each panel on
+// the columns to its left (the L factor) are applied after the last step, in panel
+// order, which leaves the factors LAPACK's getrf produces with the same pivots.
+template <class C>
+void lu_factor_la_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP, C *Pt,
+                    C *T, C *Linv, hipStream_t st, const LuOverlap &ov) {
+    const int64_t sPt = n * NB;
+    hipStream_t P = ov.panel, G = ov.update;
+    auto record = [](hipEvent_t e, hipStream_t s) { HIPCHECK(hipEventRecord(e, s)); };
+    auto wait = [](hipStream_t s, hipEvent_t e) { HIPCHECK(hipStreamWaitEvent(s, e, 0)); };
+    auto factor_panel = [&](int64_t k) {
+        const int kb = (int)std::min<int64_t>(NB, n - k);
+        stage_begin(3, P);
+        if (n - k <= LEAF_MAX_ROWS) {
+            panel_leaves<C>(n, k, kb, A, sA, ipiv, sP, Pt, sPt, batch, P);
+        } else {
+            hipLaunchKernelGGL(k_panel<C>, dim3((unsigned)batch), dim3(1024), 0, P, n, k, kb, A,
+                               sA, ipiv, sP);
+            HIPCHECK(hipGetLastError());
+        }
+        if (k + kb < n) panel_linv_t<C>(n, k, kb, A, sA, batch, Linv, P);
+        stage_end(3, P);
+    };
+    record(ov.e_start, st);
+    wait(P, ov.e_start);
+    wait(G, ov.e_start);
+    factor_panel(0);
+    record(ov.e_panel, P);
+    for (int64_t k = 0; k < n; k += NB) {
+        const int kb = (int)std::min<int64_t>(NB, n - k);
+        const int64_t k1 = k + kb;
+        wait(G, ov.e_panel);                         // panel k factored, its inverses ready
+        if (k1 >= n) break;
+        const int kb1 = (int)std::min<int64_t>(NB, n - k1);
+        const int64_t rest = n - k1;
+        stage_begin(3, G);
+        rowswap_t<C>(n, k, kb, A, sA, ipiv, sP, k1, n, k1, k1, batch, G);
+        trsm_u12_t<C>(n, k, kb, A, sA, batch, Linv, T, G, true);
+        update_cols_t<C, 2>(n, k, kb, k1, k1 + kb1, A, sA, batch, G);   // panel k + 1's columns
+        stage_end(3, G);
+        stage_flops(3, 8.0 * (double)rest * kb1 * kb * batch);
+        record(ov.e_u12, G);
+        wait(P, ov.e_u12);
+        factor_panel(k1);
+        record(ov.e_panel, P);
+        if (k1 + kb1 < n) {                          // meanwhile the rest of the trailing matrix
+            stage_begin(4, G);
+            update_cols_t<C, 1>(n, k, kb, k1 + kb1, n, A, sA, batch, G);
+            stage_end(4, G, 8.0 * (double)rest * (double)(rest - kb1) * kb * batch);
+        }
+    }
+    // every panel's interchanges on the L columns to its left
+    for (int64_t k = NB; k < n; k += NB) {
+        const int kb = (int)std::min<int64_t>(NB, n - k);
+        rowswap_t<C>(n, k, kb, A, sA, ipiv, sP, 0, k, k, k, batch, G);
+    }
+    record(ov.e_done, G);
+    wait(st, ov.e_done);
+    stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
+}
+
 template <class C>
 void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                 hipStream_t st) {
+                 hipStream_t st, const LuOverlap *ov = nullptr) {
     DBuf<C> Pt((size_t)n * NB * batch), T((size_t)SB * n * batch), Linv((size_t)2 * 4096 * batch);
-    lu_factor_ws_t<C>(n, batch, A, sA, ipiv, sP, Pt.p, T.p, Linv.p, st);
+    if (ov && ov->panel && batch <= ov->max_batch && n > NB)
+        lu_factor_la_t<C>(n, batch, A, sA, ipiv, sP, Pt.p, T.p, Linv.p, st, *ov);
+    else
+        lu_factor_ws_t<C>(n, batch, A, sA, ipiv, sP, Pt.p, T.p, Linv.p, st);
     // the scratch is released once the factorisation has run
     HIPCHECK(hipStreamSynchronize(st));
 }
@@ -1298,8 +1392,8 @@ void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int6
 }
 
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                       hipStream_t st) {
-    lu_factor_t<cd>(n, batch, A, sA, ipiv, sP, st);
+                       hipStream_t st, const LuOverlap *ov) {
+    lu_factor_t<cd>(n, batch, A, sA, ipiv, sP, st, ov);
 }
 
 int64_t lu_factor_ws_elems(int64_t n) { return n * NB + (int64_t)SB * n + 2 * 4096; }
@@ -1326,8 +1420,8 @@ void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t
 }
 
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                           hipStream_t st) {
-    lu_factor_t<cf>(n, batch, A, sA, ipiv, sP, st);
+                           hipStream_t st, const LuOverlap *ov) {
+    lu_factor_t<cf>(n, batch, A, sA, ipiv, sP, st, ov);
 }
 
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,

@@ -131,6 +131,13 @@ def gradients(strata, freqs, steering, vtan, objectives: Sequence[Tuple[int, int
     if vt.shape[1:] != (sheet.rho.shape[0], 2):
         raise ValueError("vtan must be (nparam, nnode, 2)")
     ds = hip.DeviceSheet(ctx, sheet, su.rwgdat[0])
+    try:
+        return _gradients_on(ctx, ds, su, sheet, vt, nparam, objectives, precision)
+    finally:
+        ds.free()
+
+
+def _gradients_on(ctx, ds, su, sheet, vt, nparam, objectives, precision):
     sheet_gbl = [ig for ig, g in enumerate(su.gbls) if g.j != 0]
     if len(sheet_gbl) != 1 or su.gbldup[sheet_gbl[0]] > 0:
         raise ValueError("gradients: one sheet Gblock")
@@ -179,5 +186,4 @@ def gradients(strata, freqs, steering, vtan, objectives: Sequence[Tuple[int, int
     ctx.check(L.pssfss_sheet_adjoint(ctx.h, ds.h, npts, pts, nparam, hip._d(vt), nobj,
                                      hip._d(wbar), hip._d(grad.view(np.float64)),
                                      hip.PRECISION[precision]))
-    ds.free()
     return grad.real, np.array(values)

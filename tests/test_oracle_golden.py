@@ -302,3 +302,38 @@ def test_power_conservation_lossless(pk, orc):
         for c in range(2):
             p = np.sum(np.abs(g.s11[:, c]) ** 2) + np.sum(np.abs(g.s21[:, c]) ** 2)
             assert abs(p - 1) < 1e-3
+
+
+def test_frozen_decisions_make_fd_smooth(pk, orc):
+    """With its discrete choices recorded at a base geometry and replayed, the
+    oracle is smooth in the node positions: central differences at 1e-6 and 1e-4 of
+    the period agree to 1e-7 relative (unfrozen differences at 1e-6 carry the
+    ~1e-9 jumps of cell crossings, ~1e-3 relative)."""
+    from pssfss_amd.adjoint import bilinear_patch
+    P = 12.0
+    corners = np.array([[2.1, 2.4], [9.7, 2.0], [10.1, 9.6], [2.3, 9.9]])
+    L = pk.Layer.make
+
+    def strata(c):
+        sh, _ = bilinear_patch(c, 5, 5, P, P)
+        return [L(), sh, L(width=1.0, units="mm", epsr=3.0), L(width=3.0, units="mm", epsr=1.1),
+                L()]
+
+    def f(res):
+        g = res[0]["gsm"]
+        return np.array([abs(g.s21[0, 0]) ** 2, abs(g.s21[1, 1]) ** 2])
+    freqs, steer = [10.0], {"theta": [0.0], "phi": [0.0]}
+    nufp = pk.analyze.prepare(strata(corners), freqs, steer).rwgdat[0].nufp
+    fz = orc.FrozenDecisions(nufp, 1)
+    orc.analyze(pk, strata(corners), freqs, steer, decisions=(fz, "record"))
+    for k in (0, 5):
+        d = []
+        for h in (1e-6 * P, 1e-4 * P):
+            c1, c2 = corners.copy().reshape(-1), corners.copy().reshape(-1)
+            c1[k] += h
+            c2[k] -= h
+            d.append((f(orc.analyze(pk, strata(c1.reshape(4, 2)), freqs, steer,
+                                    decisions=(fz, "replay")))
+                      - f(orc.analyze(pk, strata(c2.reshape(4, 2)), freqs, steer,
+                                      decisions=(fz, "replay")))) / (2 * h))
+        assert np.abs(d[0] - d[1]).max() < 1e-7 * np.abs(d[0]).max()
