@@ -159,11 +159,11 @@ def rwgbfft(sheet, rwgdat, k, psi1, psi2):
 
 class FrozenDecisions:
     """The oracle's discrete choices (rings summed, table cells and Floquet shifts,
-    interpolation snaps, singular-integral branches), recorded per analysed point at
-    one geometry and replayed at nearby geometries of the same mesh topology, so
-    that central differences of the restatement see a smooth function:
+    interpolation snaps, singular-integral branches), recorded per analysed point
+    and sheet at one geometry and replayed at nearby geometries of the same mesh
+    topology, so that central differences of the restatement see a smooth function:
 
-        fz = FrozenDecisions(nufp, npoints)
+        fz = FrozenDecisions(nufp, npoints)        # nufp: one int per sheet, or an int
         analyze(pk, base, freqs, steer, decisions=(fz, "record"))
         analyze(pk, moved, freqs, steer, decisions=(fz, "replay"))
 
@@ -173,13 +173,13 @@ class FrozenDecisions:
     MODES = {"record": 1, "replay": 2}
 
     def __init__(self, nufp, npoints=1):
-        self.rings = np.zeros((npoints, 7 * nufp), np.int8)
-        self.cells = np.zeros((npoints, 42 * nufp), np.int16)
-        self.sing = np.zeros((npoints, nufp), np.uint8)
+        nufps = [int(nufp)] if np.ndim(nufp) == 0 else [int(v) for v in nufp]
+        self.slots = [[(np.zeros(7 * n, np.int8), np.zeros(42 * n, np.int16),
+                        np.zeros(n, np.uint8)) for n in nufps] for _ in range(npoints)]
 
-    def select(self, mode, ipoint):
-        lib().orc_decisions(self.MODES[mode], self.rings[ipoint].ctypes.data,
-                            self.cells[ipoint].ctypes.data, self.sing[ipoint].ctypes.data)
+    def select(self, mode, ipoint, isheet=0):
+        r, c, g = self.slots[ipoint][isheet]
+        lib().orc_decisions(self.MODES[mode], r.ctypes.data, c.ctypes.data, g.ctypes.data)
 
     @staticmethod
     def off():
@@ -297,10 +297,11 @@ def analyze(pk, strata, freqs, steering, mode_override=None, want_z=False, dbmin
         if not T.principal_propagates(ml[0], ml[-1]):
             continue
         extras = []
-        if decisions is not None:
-            decisions[0].select(decisions[1], len(results))
+        ipoint = len(results)
 
         def sheet_solver(region, sheet, u, rwgdat, s, isheet):
+            if decisions is not None:
+                decisions[0].select(decisions[1], ipoint, isheet)
             g, ex = sheet_point(region, sheet, u, rwgdat, s, k0, b00, caches[isheet],
                                 want_z=want_z)
             extras.append(ex)

@@ -8,6 +8,7 @@ non-zero status with the library's own error text.
 import ctypes as C
 import math
 import os
+import time
 from typing import List
 
 import numpy as np
@@ -405,11 +406,16 @@ class SheetSolver:
     """Runs the `_analyze` sweep with every sheet GSM computed on the device,
     all points of one sheet Gblock in one batched call."""
 
-    def __init__(self, ctx: Context, setup, precision: str = "fp64", max_batch: int = 0):
+    # largest GSM port the device cascade kernel takes
+    CASCADE_MAX_PORT = 34
+
+    def __init__(self, ctx: Context, setup, precision: str = "fp64", max_batch: int = 0,
+                 cascade: str = "device"):
         self.ctx = ctx
         self.setup = setup
         self.precision = precision
         self.max_batch = max_batch      # points per library call; 0 = all points of a sheet
+        self.cascade = cascade          # "device": batched star products on the GPU; "host"
         self.dsheets = {}
 
     def upload(self):
@@ -461,7 +467,22 @@ class SheetSolver:
                 for (ip, _), g, inf in zip(part, gsms, infos):
                     solved[(ip, ig)] = g
                 infos_all += infos
-        results = []
+        t0 = time.perf_counter()
+        ports = [len(l.P) for l in su.layers]
+        if self.cascade == "device" and live and max(ports) <= self.CASCADE_MAX_PORT:
+            totals = self._cascade_device(points, live, solved)
+        else:
+            totals = self._cascade_host(points, live, solved)
+        results = [O.make_result(g, points[ip].steer, points[ip].beta00, points[ip].fghz,
+                                 su.layers) for ip, g in zip(live, totals)]
+        stats = {"npoints": len(live), "infos": infos_all,
+                 "cascade_s": time.perf_counter() - t0}
+        return results, stats
+
+    def _cascade_host(self, points, live, solved):
+        """The reference's walk (src/PSSFSS.jl:256-305) point by point on the host."""
+        su = self.setup
+        out = []
         for ip in live:
             pt = points[ip]
             for l in su.layers:
@@ -475,10 +496,67 @@ class SheetSolver:
                         gbl_of[(_ip, ig)] = True
                         return solved[(_ip, ig)].copy()
                 raise RuntimeError("sheet GSM missing from the batched solve")
-            g = su.cascade_point(pt.k0, solver)
-            results.append(O.make_result(g, pt.steer, pt.beta00, pt.fghz, su.layers))
-        stats = {"npoints": len(live), "infos": infos_all}
-        return results, stats
+            out.append(su.cascade_point(pt.k0, solver))
+        return out
+
+    def _cascade_device(self, points, live, solved):
+        """The same walk with every point advanced together: per Gblock, one batched
+        star product (src/GSMs.jl:125-141) of all points on the device
+        (pssfss_cascade_batch), then the propagation through the next layer
+        (src/GSMs.jl:155-176) as diagonal scalings."""
+        su = self.setup
+        layers = su.layers
+        nb = len(live)
+        blocks = [[None] * nb for _ in su.gbls]
+        gammas = []
+        for k, ip in enumerate(live):
+            pt = points[ip]
+            for l in layers:
+                Md.setup_modes(l, pt.k0, pt.beta00)
+            gammas.append([l.gamma.copy() for l in layers])
+            saved = {}
+            for ig, gbl in enumerate(su.gbls):
+                i1, i2 = gbl.rng[0], 1 + gbl.rng[-1]
+                if gbl.j == 0:
+                    blocks[ig][k] = G.gsm_slab_interface(layers[i1 - 1], layers[i2 - 1], pt.k0)
+                    continue
+                sheet = su.sheets[su.junc[gbl.j - 1] - 1]
+                region = layers[i1 - 1:i2]
+                if su.gbldup[ig] > 0:
+                    gb = saved[su.gbldup[ig] - 1].copy()
+                else:
+                    if sheet.sheet_class in ("J", "M"):
+                        gb = solved[(ip, ig)].copy()
+                    elif sheet.sheet_class == "E":
+                        gb = G.pecgsm(len(region[0].P), len(region[-1].P))
+                    elif sheet.sheet_class == "H":
+                        gb = G.pmcgsm(len(region[0].P), len(region[-1].P))
+                    else:
+                        raise ValueError(f"Illegal sheet class: {sheet.sheet_class}")
+                    if su.gbldup[ig] < 0:
+                        saved[ig] = gb.copy()
+                if sheet.dx != 0 or sheet.dy != 0:
+                    G.translate_gsm(gb, sheet.dx / sheet.upm, sheet.dy / sheet.upm, region[0],
+                                    region[-1])
+                blocks[ig][k] = gb
+
+        def propagate(gs, il):
+            w = layers[il].width
+            if w == 0:
+                return
+            for k, g in enumerate(gs):
+                p = np.exp(-gammas[k][il] * w)
+                g.s12 *= p[None, :]
+                g.s21 *= p[:, None]
+                g.s22 *= p[None, :]
+                g.s22 *= p[:, None]
+        n1 = len(layers[0].P)
+        cur = [G.GSM.new(n1, n1) for _ in live]
+        propagate(cur, 0)
+        for ig, gbl in enumerate(su.gbls):
+            cur = self.ctx.cascade_batch(cur, blocks[ig])
+            propagate(cur, gbl.rng[-1])
+        return cur
 
     def close(self):
         for d in self.dsheets.values():

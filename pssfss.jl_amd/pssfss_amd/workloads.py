@@ -11,7 +11,8 @@ patch_two_layer: 12 mm cell, square patch on eps_r = 3.0 (1.0 mm) + eps_r = 1.1
 meander_polarizer: four PEC meanderlines (6 x 10 mm cell, 0.3 mm strips, meander
     heights drawn from a seeded U(2, 3) mm), eps_r = 1.07 foam spacers 5 mm thick,
     eps_r = 2.8 skins 0.25 mm thick; 101 frequencies 8..14 GHz at theta 0 and 40
-    deg, phi = 0.
+    deg, phi = 0.  Analysed with dbmin = 68 the foam layers carry 34 modes (the
+    reference's selection keeps symmetric mode groups: 30 or 34 around 32).
 cross_batch: independent single-sheet problems in free space, alternately a solid
     Jerusalem cross and a loaded cross drawn from a seed: period U(8, 15) mm, arm
     length (half the tip-to-tip span) U(0.3, 0.45) P, strip width U(0.03, 0.08) P,

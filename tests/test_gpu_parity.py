@@ -488,3 +488,26 @@ def test_resistive_sheet_closed_form(ctx, pk, orc):
     print(f"  max |dS| vs oracle {d:.3e}; vs closed form {dc:.3e} (Gamma {G:.4f}, T {T:.4f})")
     assert d <= 1e-9
     assert dc <= 1e-4
+
+
+def test_device_cascade_matches_host(ctx, pk, orc):
+    """The product path cascades every point's Gblocks with batched star products on
+    the device (pssfss_cascade_batch, src/GSMs.jl:125-141): on the 4-sheet meander
+    polarizer at 34 modes per foam layer (dbmin 68) its S equals the host walk's to
+    1e-12 and the oracle's to 1e-9."""
+    from pssfss_amd.hip import SheetSolver
+    _say("device vs host cascade, 4-sheet meander polarizer (ntri=120), 34 modes, 2 freqs x 2 angles")
+    strata, freqs, steer = pk.workloads.meander_polarizer(ntri=120, nfreq=2)
+    setup = pk.analyze.prepare(strata, freqs, steer, dbmin=68.0)
+    assert max(len(l.P) for l in setup.layers) == 34
+    with SheetSolver(ctx, setup, "fp64", cascade="device") as s:
+        dev, st = s.run()
+        s.cascade = "host"
+        host, _ = s.run()
+    d = _max_dS(dev, host)
+    ref = orc.analyze(pk, strata, freqs, steer, dbmin=68.0)
+    do = _max_dS(dev, ref)
+    print(f"  max |dS| device vs host cascade {d:.3e}, vs oracle {do:.3e}; "
+          f"cascade {1e3 * st['cascade_s']:.1f} ms")
+    assert d <= 1e-12
+    assert do <= 1e-9
