@@ -150,3 +150,55 @@ def test_adjoint_mixed_vs_fp64(ctx, pk):
           f"max |f_mixed - f_fp64| = {np.abs(v32 - v64).max():.2e}")
     assert rel.max() < 1e-6
     assert np.abs(v32 - v64).max() < 1e-6
+
+
+def test_stack_gradient_meander_axial_ratio(ctx, pk, orc):
+    """The cascaded 4-sheet meanderline polarizer (rotated 45 deg, so that it turns a
+    linear wave circular): gradient of the mean axial ratio AR21(H) over 10 GHz at
+    theta 0 and 40 deg with respect to the 4 meander amplitudes, through the reverse
+    pass of the cascade and one adjoint call per sheet, against frozen-decision
+    oracle central differences at 1e-6 of the amplitude (bar 1e-5)."""
+    from pssfss_amd import adjoint as A
+    from pssfss_amd import outputs as O
+    print("\n[case] stack gradient: mean AR21(H) of the 4-sheet meander w.r.t. amplitudes",
+          flush=True)
+    hs = [2.2, 2.5, 2.7, 2.4]
+    L = pk.Layer.make
+
+    def build(hv):
+        sheets, vts = [], []
+        for h in hv:
+            sh, vt = A.meander_amplitude_tangent(dict(a=6.0, b=10.0, h=h, w1=0.3, w2=0.3,
+                                                      ntri=120, units="mm", fufp=False,
+                                                      rot=45.0))
+            sheets.append(sh)
+            vts.append(vt)
+        foam = lambda: L(width=5.0, units="mm", epsr=1.07)
+        skin = lambda: L(width=0.25, units="mm", epsr=2.8)
+        return [L(), skin(), sheets[0], foam(), sheets[1], foam(), sheets[2], foam(),
+                sheets[3], skin(), L()], vts
+    freqs, steer = [10.0], {"phi": [0.0], "theta": [0.0, 40.0]}
+    obj = O.ardb(2, 1, O.H)
+    strata, vts = build(hs)
+    grad, val = A.gradients_stack(strata, freqs, steer, [(i, vts[i]) for i in range(4)], obj,
+                                  ctx=ctx, dbmin=30.0)
+    su = pk.analyze.prepare(strata, freqs, steer, dbmin=30.0)
+    fz = orc.FrozenDecisions([r.nufp for r in su.rwgdat], 2)
+    base = np.mean([obj(r) for r in orc.analyze(pk, strata, freqs, steer, dbmin=30.0,
+                                                 decisions=(fz, "record"))])
+    assert abs(val - base) < 1e-9 * abs(base)
+    fd = []
+    for i in range(4):
+        d = 1e-6 * hs[i]
+        vals = []
+        for sgn in (1, -1):
+            hv = list(hs)
+            hv[i] += sgn * d
+            s2, _ = build(hv)
+            res = orc.analyze(pk, s2, freqs, steer, dbmin=30.0, decisions=(fz, "replay"))
+            vals.append(np.mean([obj(r) for r in res]))
+        fd.append((vals[0] - vals[1]) / (2 * d))
+    fd = np.array(fd)
+    rel = np.abs(grad - fd).max() / np.abs(fd).max()
+    print(f"  mean AR {val:.4f} dB; grad {grad}, FD {fd}; max rel {rel:.2e}")
+    assert rel < 1e-5
