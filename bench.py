@@ -201,6 +201,7 @@ def main():
         if args.workload != "ring":
             extra["ring_on_slab"] = ring_detail(pk, hip, ctx)
         extra["meander_polarizer"] = meander_polarizer(pk, hip, ctx)
+        extra["meander_polarizer_gradient"] = meander_gradient(pk, ctx)
         extra["cross_batch"] = cross_batch(pk, hip, ctx)
 
     cpu = None
@@ -303,6 +304,32 @@ def meander_polarizer(pk, hip, ctx):
            "prepare_s": round(prep, 3)}
     out.update(_sweep(hip, ctx, setup, "meander polarizer"))
     return out
+
+
+def meander_gradient(pk, ctx):
+    """Gradient of the mean axial ratio AR21(H) over two points (11 GHz, theta 0
+    and 40 deg) of the full-size 4-sheet meander polarizer (about 8000 faces per
+    sheet, every face pair its own class) with respect to the 4 meander amplitudes:
+    device forward solves of all sheets, reverse pass through the cascade on the
+    host, one adjoint call per sheet (FP64)."""
+    from pssfss_amd import adjoint as A
+    from pssfss_amd import outputs as O
+    hs = pk.workloads.meander_amplitudes()
+    vts = []
+    for h in hs:
+        _, vt = A.meander_amplitude_tangent(dict(a=6.0, b=10.0, h=float(h), w1=0.3, w2=0.3,
+                                                 ntri=8000, units="mm", fufp=False))
+        vts.append(vt)
+    strata, _, _ = pk.workloads.meander_polarizer(ntri=8000, nfreq=2, fufp=False, hs=hs)
+    freqs, steer = [11.0], {"phi": [0.0], "theta": [0.0, 40.0]}
+    t0 = time.perf_counter()
+    grad, val = A.gradients_stack(strata, freqs, steer, [(i, vts[i]) for i in range(4)],
+                                  O.ardb(2, 1, O.H), ctx=ctx, dbmin=30.0)
+    dt = time.perf_counter() - t0
+    print(f"[bench] meander gradient: {dt:.3f} s, mean AR {val:.3f} dB, grad {grad}", flush=True)
+    return {"points": 2, "parameters": 4, "seconds": round(dt, 3),
+            "grads_per_s": round(2 / dt, 4), "mean_ar21_h_db": val,
+            "d_mean_ar_d_h_db_per_mm": [float(g) for g in grad]}
 
 
 def cross_batch(pk, hip, ctx):

@@ -52,11 +52,17 @@ def patch_two_layer(n: int = 39, nfreq: int = 12, L: float = 8.0):
     return strata, freqs, steering
 
 
-def meander_polarizer(ntri: int = 8000, nfreq: int = 101, seed: int = 0, thetas=(0.0, 40.0)):
-    rng = np.random.default_rng(seed)
-    hs = rng.uniform(2.0, 3.0, 4)
-    sheets = [meander(a=6.0, b=10.0, h=float(h), w1=0.3, w2=0.3, ntri=ntri, units="mm")
-              for h in hs]
+def meander_amplitudes(seed: int = 0):
+    return np.random.default_rng(seed).uniform(2.0, 3.0, 4)
+
+
+def meander_polarizer(ntri: int = 8000, nfreq: int = 101, seed: int = 0, thetas=(0.0, 40.0),
+                      fufp: bool = True, hs=None):
+    """fufp=False keeps every face pair its own class (needed for geometry
+    gradients); hs overrides the seeded amplitudes (mm)."""
+    hs = meander_amplitudes(seed) if hs is None else hs
+    sheets = [meander(a=6.0, b=10.0, h=float(h), w1=0.3, w2=0.3, ntri=ntri, units="mm",
+                      fufp=fufp) for h in hs]
     L = Layer.make
     foam = lambda: L(width=5.0, units="mm", epsr=1.07)
     skin = lambda: L(width=0.25, units="mm", epsr=2.8)
