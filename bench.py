@@ -333,27 +333,37 @@ def meander_gradient(pk, ctx):
 
 
 def cross_batch(pk, hip, ctx):
-    """The first Jerusalem cross and the first loaded cross of the seeded 16-problem
-    batch (full meshes), each at 8 of its 64 frequencies (every 8th from 1 GHz) and
-    the 4 scan angles: 64 points of two uneven problem sizes, run one problem after
-    the other."""
-    probs = pk.workloads.cross_batch()[:2]
-    out = {"problems": [], "points": 0}
-    tot = {"fp64": 0.0, "mixed": 0.0}
-    for strata, freqs, steer, par in probs:
-        f = list(freqs[::8])
-        setup = pk.analyze.prepare(strata, f, steer)
-        r = _sweep(hip, ctx, setup, par["kind"])
-        npts = r["points"]
-        r.update({"kind": par["kind"], "period_mm": round(par["P"], 3),
-                  "faces": int(strata[1].nface), "unknowns": int(setup.rwgdat[0].nbf)})
-        out["problems"].append(r)
-        out["points"] += npts
-        for p in tot:
-            tot[p] += npts / r[f"points_per_s_{p}"]
-    for p in tot:
-        out[f"points_per_s_{p}"] = round(out["points"] / tot[p], 4)
-    out["speedup_mixed_vs_fp64"] = round(tot["fp64"] / tot["mixed"], 3)
+    """The 16 seeded Jerusalem / loaded cross problems (full meshes, 800-4000
+    requested triangles, uneven unknown counts), each at 16 of its 64 frequencies
+    (every 4th from 1 GHz) and the 4 scan angles: 1024 points.  Timed three ways per
+    precision after one untimed pass over two problems (kernels loaded): one problem
+    after another with the library's default chunking; one after another with the
+    tuner's points per chunk for each size class (tuned before the timed region);
+    four problems at a time (four contexts on one GPU, tuned chunks)."""
+    from pssfss_amd.batch import analyze_batch, ChunkTuner
+    probs = pk.workloads.cross_batch()
+    problems = [(s, list(f[::4]), st) for s, f, st, _ in probs]
+    t0 = time.perf_counter()
+    setups = [pk.analyze.prepare(s, f, st, classify=lambda sh: hip.face_pair_classes(ctx, sh))
+              for s, f, st in problems]
+    out = {"problems": len(problems), "unknowns": [int(su.rwgdat[0].nbf) for su in setups],
+           "prepare_s": round(time.perf_counter() - t0, 3)}
+    tuner = ChunkTuner()
+    analyze_batch(problems[:2], "fp64", workers=1, setups=setups[:2])
+    for prec in ("fp64", "mixed"):
+        r1, s1 = analyze_batch(problems, prec, workers=1, setups=setups)
+        rt, st = analyze_batch(problems, prec, workers=1, tuner=tuner, setups=setups)
+        r4, s4 = analyze_batch(problems, prec, workers=4, tuner=tuner, setups=setups)
+        out["points"] = s1["points"]
+        out[f"points_per_s_{prec}_one_at_a_time"] = round(s1["points_per_s"], 3)
+        out[f"points_per_s_{prec}_one_at_a_time_tuned"] = round(st["points_per_s"], 3)
+        out[f"points_per_s_{prec}_four_at_a_time"] = round(s4["points_per_s"], 3)
+        out[f"max_dS_{prec}_tuned_vs_default"] = max(max_dS(a, b) for a, b in zip(r1, rt))
+        out[f"max_dS_{prec}_four_vs_one"] = max(max_dS(a, b) for a, b in zip(r1, r4))
+        print(f"[bench] cross batch {prec}: one at a time {s1['points_per_s']:.1f} (tuned "
+              f"{st['points_per_s']:.1f}), four at a time {s4['points_per_s']:.1f} points/s",
+              flush=True)
+    out["tuned_chunks"] = tuner.table
     return out
 
 

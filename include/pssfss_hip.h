@@ -104,6 +104,12 @@ const char *pssfss_last_error(const pssfss_ctx *ctx);
 int pssfss_device_info(pssfss_ctx *ctx, char *name, int name_len, int *num_cu,
                        double *hbm_gib);
 
+/* Chunking of the batched sheet calls (no counterpart in the reference): at most
+ * max_points points per chunk (0: 128), and a chunk's workspaces within
+ * mem_fraction of the free HBM (0: 0.45).  Several contexts driven concurrently
+ * on one GPU (uneven problems side by side) each take a share. */
+int pssfss_ctx_set_chunk(pssfss_ctx *ctx, int max_points, double mem_fraction);
+
 /* ---- sheets ----------------------------------------------------------------- */
 /* Upload a sheet; replaces setup_rwg's in-memory RWGData for the device
  * (src/RWG.jl:90-324 builds the maps; src/PSSFSS.jl:220 calls it). */

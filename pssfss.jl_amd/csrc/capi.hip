@@ -59,8 +59,9 @@ cd hcis(double t) { return cd{cos(t), sin(t)}; }
 // The mmax sequence of the table convergence loop (src/PGF.jl:223-233):
 // 0, then successive even 5-smooth numbers, ending at 2048.
 const std::vector<int> &mmax_sequence() {
-    static std::vector<int> seq;
-    if (seq.empty()) {
+    // built once, under the language's thread-safe initialisation of local statics
+    static const std::vector<int> seq = [] {
+        std::vector<int> q;
         auto next235 = [](int n) {
             for (int v = n;; ++v) {
                 int x = v;
@@ -71,13 +72,14 @@ const std::vector<int> &mmax_sequence() {
             }
         };
         int M = 0;
-        seq.push_back(0);
+        q.push_back(0);
         while (M < MMAX_LAST) {
             M = next235(1 + M);
             while (M % 2 != 0) M = next235(1 + M);
-            seq.push_back(M);
+            q.push_back(M);
         }
-    }
+        return q;
+    }();
     return seq;
 }
 
@@ -139,6 +141,8 @@ struct pssfss_ctx {
     DBuf<double> wanorm;         // infinity norms of the systems (refinement)
     std::set<pssfss_sheet *> sheets;   // sheets uploaded and not yet freed
     LuOverlap lu_ov;             // look-ahead LU streams (panel / update CU subsets)
+    int chunk_max = 128;         // points per chunk of a batched sheet call
+    double mem_frac = 0.45;      // share of free HBM a chunk's workspaces may take
     const cd *dft_matrix(int M) {
         auto it = dft.find(M);
         if (it != dft.end()) return it->second.p;
@@ -1218,10 +1222,9 @@ int chunk_size(pssfss_ctx *ctx, const pssfss_sheet *sh, int npts, int precision)
     }
     double per = 16.0 * sh->nbf * sh->nbf * (precision == PSSFSS_MIXED ? 1.5 : 1.0) +
                  96.0 * sh->nufp + 4.0e6;
-    double budget = 0.45 * (double)freeb;
+    double budget = ctx->mem_frac * (double)freeb;
     int c = (int)std::max(1.0, std::floor(budget / per));
-    (void)ctx;
-    return std::min(std::min(c, 128), npts);
+    return std::min(std::min(c, ctx->chunk_max), npts);
 }
 
 void upload_sheet(pssfss_ctx *ctx, const pssfss_sheet_desc *d, pssfss_sheet *sh) {
@@ -1769,6 +1772,15 @@ int pssfss_cascade_batch(pssfss_ctx *ctx, int nbatch, int n1, int n2, int n3, co
         cascade_batched_dev(nbatch, n1, n2, n3, da.p, db.p, dc.p, ctx->st);
         d2h(ctx, (cd *)c, dc.p, (size_t)lc * nbatch);
         sync(ctx);
+    });
+}
+
+int pssfss_ctx_set_chunk(pssfss_ctx *ctx, int max_points, double mem_fraction) {
+    return guarded(ctx, [&] {
+        if (max_points < 0 || max_points > 4096 || !(mem_fraction >= 0) || mem_fraction > 0.9)
+            throw ArgError("max_points must be in 0..4096 and mem_fraction in 0..0.9");
+        ctx->chunk_max = max_points > 0 ? max_points : 128;
+        ctx->mem_frac = mem_fraction > 0 ? mem_fraction : 0.45;
     });
 }
 

@@ -1,0 +1,137 @@
+"""Independent problems of uneven size analysed side by side on one GPU.
+
+The reference analyses one structure at a time, point by point (the sweep of
+src/PSSFSS.jl:230-305).  Here each problem's points go through the batched sheet
+path (fill, LU and solves of a chunk of points per launch sequence), problems are
+dealt largest first (unknowns^3 x points) to the first free worker, and each worker
+drives a library context of its own (its own stream, workspaces and look-ahead LU
+streams) from a host thread.  One worker is the default: with several, every
+release of a per-call device temporary (hipFree) waits for the whole device, so
+side-by-side contexts stall each other and four workers run at about half the rate
+of one.
+
+ChunkTuner picks the points per chunk for each class of unknown count (rounded up
+to a multiple of 512) and precision by timing candidate chunk sizes on the first
+points of the first problem of that class, and records the choice (optionally in
+a JSON file that later runs read back).
+"""
+import json
+import os
+import queue
+import threading
+import time
+from typing import List, Optional
+
+from . import hip
+from .analyze import prepare
+
+
+def _cost(setup) -> float:
+    n = max((r.nbf for r in setup.rwgdat if r is not None), default=1)
+    return float(n) ** 3 * len(list(setup.points()))
+
+
+class ChunkTuner:
+    """Points per chunk by (precision, unknown-count class), measured once."""
+
+    def __init__(self, candidates=(16, 32, 64), path: Optional[str] = None):
+        self.candidates = tuple(int(c) for c in candidates)
+        self.path = path
+        self.table = {}
+        self.lock = threading.Lock()
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.table = json.load(f)
+
+    @staticmethod
+    def key(nbf: int, precision: str) -> str:
+        return f"{precision}:{512 * ((int(nbf) + 511) // 512)}"
+
+    def chunk(self, ctx, setup, precision: str, mem_fraction: float) -> int:
+        nbf = max(r.nbf for r in setup.rwgdat if r is not None)
+        k = self.key(nbf, precision)
+        with self.lock:
+            if k in self.table:
+                return int(self.table[k]["chunk"])
+        # the same first points of the sweep under every candidate chunking, each on
+        # freshly uploaded sheets (every spatial sum computed, none from a cache)
+        limit = max(self.candidates)
+        rates = {}
+        for k_run, c in enumerate((self.candidates[0],) + self.candidates):
+            ctx.set_chunk(c, mem_fraction)
+            with hip.SheetSolver(ctx, setup, precision) as s:
+                s.upload()
+                t0 = time.perf_counter()
+                res, _ = s.run(limit=limit)
+                if k_run > 0:                       # the first run is the warm-up
+                    rates[c] = len(res) / (time.perf_counter() - t0)
+        best = max(rates, key=rates.get)
+        with self.lock:
+            self.table[k] = {"chunk": best, "points_per_s": {str(c): round(r, 3)
+                                                             for c, r in rates.items()}}
+            if self.path:
+                with open(self.path, "w") as f:
+                    json.dump(self.table, f, indent=1, sort_keys=True)
+        return best
+
+
+def analyze_batch(problems: List[tuple], precision: str = "mixed", workers: int = 1,
+                  tuner: Optional[ChunkTuner] = None, chunk: int = 0, setups=None,
+                  device: int = 0):
+    """Analyze independent problems, `workers` at a time on one GPU.
+
+    problems: (strata, freqs, steering) tuples as analyze takes them; setups: their
+    prepare() results if already made.  chunk: points per chunk for every problem
+    (0: the library default, unless a tuner chooses).  Returns (results per problem
+    in input order, stats: wall seconds, points, per-problem seconds and chunks)."""
+    ctx0 = hip.get_context()
+    if setups is None:
+        setups = [prepare(st, fr, sr, classify=lambda sh: hip.face_pair_classes(ctx0, sh))
+                  for st, fr, sr in problems]
+    order = sorted(range(len(setups)), key=lambda i: -_cost(setups[i]))
+    share = min(0.45, 0.8 / max(1, workers))
+    chunks = [chunk] * len(setups)
+    if tuner is not None:       # measured alone, before the workers share the GPU
+        for i in order:
+            chunks[i] = tuner.chunk(ctx0, setups[i], precision, share)
+        ctx0.set_chunk(0, 0.0)
+    work = queue.Queue()
+    for i in order:
+        work.put(i)
+    results = [None] * len(setups)
+    per = [None] * len(setups)
+    errors = []
+
+    def worker():
+        ctx = hip.Context(device)
+        try:
+            while True:
+                try:
+                    i = work.get_nowait()
+                except queue.Empty:
+                    return
+                c = chunks[i]
+                ctx.set_chunk(c, share)
+                t0 = time.perf_counter()
+                with hip.SheetSolver(ctx, setups[i], precision) as s:
+                    res, _ = s.run()
+                results[i] = res
+                per[i] = {"seconds": time.perf_counter() - t0, "chunk": c,
+                          "points": len(res)}
+        except Exception as e:          # reported after the join
+            errors.append(e)
+        finally:
+            ctx.close()
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker) for _ in range(max(1, workers))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    npts = sum(len(r) for r in results)
+    return results, {"seconds": wall, "points": npts, "points_per_s": npts / wall,
+                     "problems": per}

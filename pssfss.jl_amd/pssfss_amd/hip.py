@@ -106,6 +106,7 @@ def lib():
                                             C.POINTER(C.c_int), _dp]
         L.pssfss_cascade_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.pssfss_lu_solve_batch_noalloc.argtypes = [vp, C.c_int64, C.c_int, C.c_int, _dp, _dp]
+        L.pssfss_ctx_set_chunk.argtypes = [vp, C.c_int, C.c_double]
         L.pssfss_stats_enable.argtypes = [vp, C.c_int]
         L.pssfss_stats_read.argtypes = [vp, C.POINTER(Stats), C.c_int]
         for name in ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_device_info",
@@ -113,7 +114,7 @@ def lib():
                      "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_fill_dk0", "pssfss_filljk",
                      "pssfss_modal_tables", "pssfss_rwgbfft", "pssfss_lu_solve_batch",
                      "pssfss_cascade_batch", "pssfss_stats_enable", "pssfss_stats_read",
-                     "pssfss_lu_solve_batch_noalloc"):
+                     "pssfss_lu_solve_batch_noalloc", "pssfss_ctx_set_chunk"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -124,7 +125,8 @@ EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssf
            "pssfss_sheet_gsm_batch", "pssfss_fill", "pssfss_filljk", "pssfss_modal_tables",
            "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
            "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint",
-           "pssfss_face_pair_classes", "pssfss_fill_dk0", "pssfss_lu_solve_batch_noalloc")
+           "pssfss_face_pair_classes", "pssfss_fill_dk0", "pssfss_lu_solve_batch_noalloc",
+           "pssfss_ctx_set_chunk")
 
 
 def _d(a):
@@ -154,6 +156,11 @@ class Context:
         gib = C.c_double()
         self.check(lib().pssfss_device_info(self.h, name, 256, C.byref(cu), C.byref(gib)))
         return name.value.decode(), cu.value, gib.value
+
+    def set_chunk(self, max_points: int = 0, mem_fraction: float = 0.0):
+        """Points per chunk of the batched sheet calls and their share of free HBM
+        (0: the defaults, 128 points and 0.45)."""
+        self.check(lib().pssfss_ctx_set_chunk(self.h, int(max_points), float(mem_fraction)))
 
     def stats_enable(self, on: bool = True):
         self.check(lib().pssfss_stats_enable(self.h, 1 if on else 0))
@@ -430,12 +437,15 @@ class SheetSolver:
             self.dsheets[isheet] = DeviceSheet(self.ctx, su.sheets[isheet], su.rwgdat[isheet])
         return self.dsheets[isheet]
 
-    def run(self):
+    def run(self, limit: int = 0):
+        """The sweep (its first `limit` analysed points when limit > 0)."""
         su = self.setup
         points = list(su.points())
         live = []
         requests = {}    # (gbl index) -> list of (point index, builder)
         for ip, pt in enumerate(points):
+            if limit > 0 and len(live) >= limit:
+                break
             for l in su.layers:
                 Md.setup_modes(l, pt.k0, pt.beta00)
             if not su.principal_ok():

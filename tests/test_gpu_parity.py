@@ -511,3 +511,26 @@ def test_device_cascade_matches_host(ctx, pk, orc):
           f"cascade {1e3 * st['cascade_s']:.1f} ms")
     assert d <= 1e-12
     assert do <= 1e-9
+
+
+def test_analyze_batch_uneven_problems(ctx, pk, orc, tmp_path):
+    """Three seeded cross problems of different unknown counts analysed side by side
+    (three workers, each on its own context, chunk sizes from the tuner): every
+    problem's S equals its own one-at-a-time analysis to 1e-12 and the oracle to
+    1e-9 (FP64), and the tuner records a choice per size class."""
+    from pssfss_amd.batch import analyze_batch, ChunkTuner
+    probs = pk.workloads.cross_batch(nprob=3, nfreq=2, seed=5, ntri_range=(120, 260))
+    problems = [(s, [9.0, 21.0], st) for s, _, st, _ in probs]
+    _say(f"analyze_batch: 3 cross problems ({[p[0][1].nface for p in problems]} faces), "
+         f"2 freqs x 4 angles, 3 workers")
+    tuner = ChunkTuner(candidates=(4, 8), path=str(tmp_path / "chunks.json"))
+    res, st = analyze_batch(problems, precision="fp64", workers=3, tuner=tuner)
+    assert len(tuner.table) >= 1 and (tmp_path / "chunks.json").exists()
+    for (strata, freqs, steer), r in zip(problems, res):
+        one = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+        ref = orc.analyze(pk, strata, freqs, steer)
+        d1, d2 = _max_dS(r, one), _max_dS(r, ref)
+        print(f"  vs one-at-a-time {d1:.2e}, vs oracle {d2:.2e}")
+        assert len(r) == len(one) == len(ref) == 8
+        assert d1 <= 1e-12 and d2 <= 1e-9
+    print(f"  {st['points']} points in {st['seconds']:.3f} s; tuner {tuner.table}")
