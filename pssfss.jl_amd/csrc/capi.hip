@@ -818,19 +818,34 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     for (int p = 0; p < np; ++p) all[p] = p;
     Refined rf = refine_mixed(ctx, n, ncol, np, false, ctx->wB.p, all, true);
     stage_end(5, st);
-    // Points whose refinement did not reach an FP64-level backward error: FP64 LU
-    // (in place: Z of those points is replaced by its factors).
+    // Points whose refinement did not reach an FP64-level backward error: one
+    // batched FP64 LU and solve of those systems.  After the refinement wZ (the FP64
+    // matrices), wR (residuals) and the FP32 pivots are needed for these systems
+    // only, so their Z and right-hand sides move to the front slots of wZ and wR:
+    // slot i <= fb[i] and the moves go in increasing order, so every source is read
+    // before its slot is overwritten.  The solutions go back to their points' slots
+    // of wX.
     std::vector<int> fb;
     for (int p = 0; p < np; ++p)
         if (!rf.conv[p]) fb.push_back(p);
-    ctx->wT.ensure((size_t)sB * np);
-    for (int p : fb) {
-        lu_factor_batched(n, 1, ctx->wZ.p + p * nb2, nb2, ctx->wpiv.p + p * n, n, st);
-        lu_make_perm(n, 1, ctx->wpiv.p + p * n, n, ctx->wperm.p + p * n, st);
-        HIPCHECK(hipMemcpyAsync(ctx->wX.p + p * sB, ctx->wB.p + p * sB, sB * sizeof(cd),
-                                hipMemcpyDeviceToDevice, st));
-        lu_solve_batched(n, ncol, 1, ctx->wZ.p + p * nb2, nb2, ctx->wperm.p + p * n, n,
-                         ctx->wX.p + p * sB, sB, ctx->wT.p + p * sB, st);
+    if (!fb.empty()) {
+        const int nf = (int)fb.size();
+        ctx->wT.ensure((size_t)sB * nf);
+        for (int i = 0; i < nf; ++i) {
+            const int64_t p = fb[i];
+            if (p != i)
+                HIPCHECK(hipMemcpyAsync(ctx->wZ.p + i * nb2, ctx->wZ.p + p * nb2,
+                                        (size_t)nb2 * sizeof(cd), hipMemcpyDeviceToDevice, st));
+            HIPCHECK(hipMemcpyAsync(ctx->wR.p + i * sB, ctx->wB.p + p * sB, sB * sizeof(cd),
+                                    hipMemcpyDeviceToDevice, st));
+        }
+        lu_factor_batched(n, nf, ctx->wZ.p, nb2, ctx->wpiv.p, n, st, &ctx->lu_ov);
+        lu_make_perm(n, nf, ctx->wpiv.p, n, ctx->wperm.p, st);
+        lu_solve_batched(n, ncol, nf, ctx->wZ.p, nb2, ctx->wperm.p, n, ctx->wR.p, sB, ctx->wT.p,
+                         st);
+        for (int i = 0; i < nf; ++i)
+            HIPCHECK(hipMemcpyAsync(ctx->wX.p + (int64_t)fb[i] * sB, ctx->wR.p + i * sB,
+                                    sB * sizeof(cd), hipMemcpyDeviceToDevice, st));
     }
     HIPCHECK(hipMemcpyAsync(ctx->wB.p, ctx->wX.p, (size_t)sB * np * sizeof(cd),
                             hipMemcpyDeviceToDevice, st));

@@ -82,8 +82,11 @@ def analyze_batch(problems: List[tuple], precision: str = "mixed", workers: int 
 
     problems: (strata, freqs, steering) tuples as analyze takes them; setups: their
     prepare() results if already made.  chunk: points per chunk for every problem
-    (0: the library default, unless a tuner chooses).  Returns (results per problem
-    in input order, stats: wall seconds, points, per-problem seconds and chunks)."""
+    (0: the library default, unless a tuner chooses).  One worker runs in the calling
+    thread on the process context (device 0), whose workspaces, grown once, serve
+    every problem; several workers each open a context on `device`.  Returns
+    (results per problem in input order, stats: wall seconds, points, per-problem
+    seconds and chunks)."""
     ctx0 = hip.get_context()
     if setups is None:
         setups = [prepare(st, fr, sr, classify=lambda sh: hip.face_pair_classes(ctx0, sh))
@@ -102,6 +105,15 @@ def analyze_batch(problems: List[tuple], precision: str = "mixed", workers: int 
     per = [None] * len(setups)
     errors = []
 
+    def run_one(ctx, i):
+        c = chunks[i]
+        ctx.set_chunk(c, share)
+        t1 = time.perf_counter()
+        with hip.SheetSolver(ctx, setups[i], precision) as s:
+            res, _ = s.run()
+        results[i] = res
+        per[i] = {"seconds": time.perf_counter() - t1, "chunk": c, "points": len(res)}
+
     def worker():
         ctx = hip.Context(device)
         try:
@@ -110,25 +122,25 @@ def analyze_batch(problems: List[tuple], precision: str = "mixed", workers: int 
                     i = work.get_nowait()
                 except queue.Empty:
                     return
-                c = chunks[i]
-                ctx.set_chunk(c, share)
-                t0 = time.perf_counter()
-                with hip.SheetSolver(ctx, setups[i], precision) as s:
-                    res, _ = s.run()
-                results[i] = res
-                per[i] = {"seconds": time.perf_counter() - t0, "chunk": c,
-                          "points": len(res)}
+                run_one(ctx, i)
         except Exception as e:          # reported after the join
             errors.append(e)
         finally:
             ctx.close()
 
     t0 = time.perf_counter()
-    threads = [threading.Thread(target=worker) for _ in range(max(1, workers))]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
+    if workers <= 1:
+        try:
+            for i in order:
+                run_one(ctx0, i)
+        finally:
+            ctx0.set_chunk(0, 0.0)
+    else:
+        threads = [threading.Thread(target=worker) for _ in range(workers)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
     wall = time.perf_counter() - t0
     if errors:
         raise errors[0]

@@ -54,6 +54,39 @@ def test_lu_solve(ctx, n, nrhs, nb, prec):
         assert all(r < 1 for r in rs) and max(it) <= 6, (it, list(rs))
 
 
+def test_lu_mixed_fallback_batch(ctx):
+    """Mixed precision on a batch of 6 systems of which 3 (not adjacent: 1, 3, 4)
+    have condition number 1e10, beyond what refinement on FP32 factors can reach
+    (kappa u32 >> 1): those go through the batched FP64 re-solve after their
+    refinement stops without converging (reported backward error > 1 in units of
+    sqrt(n) eps), the others converge.  Every solution must have an FP64-level
+    normwise backward error, max|b - A x| / (||A||_inf max|x|) < 1e-13."""
+    _say("lu_solve mixed, batch of 6 with 3 ill-conditioned systems (FP64 fallback)")
+    rng = np.random.default_rng(11)
+    n, nrhs, nb = 300, 3, 6
+    bad = {1, 3, 4}
+    A = np.empty((nb, n, n), np.complex128)
+    for b in range(nb):
+        G = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        if b in bad:
+            U, _ = np.linalg.qr(G)
+            V, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+            A[b] = (U * np.logspace(0, -10, n)) @ V.conj().T
+        else:
+            A[b] = G + np.eye(n) * (2 * math.sqrt(n))
+    B = rng.standard_normal((nb, n, nrhs)) + 1j * rng.standard_normal((nb, n, nrhs))
+    X, it, rs = ctx.lu_solve(A, B, "mixed")
+    print(f"  refinement sweeps {it}, backward error / (sqrt(n) eps) {list(rs)}")
+    for b in range(nb):
+        be = (np.abs(B[b] - A[b] @ X[b]).max()
+              / (np.abs(A[b]).sum(axis=1).max() * np.abs(X[b]).max()))
+        assert be < 1e-13, (b, be)
+        if b in bad:
+            assert rs[b] > 1, (b, rs[b])        # refinement alone did not converge
+        else:
+            assert rs[b] < 1, (b, rs[b])
+
+
 def test_lu_pivoting(ctx):
     _say("lu_solve with forced row interchanges")
     rng = np.random.default_rng(3)
