@@ -306,12 +306,65 @@ __global__ __launch_bounds__(256) void k_panel_copy(int64_t n, int64_t k, int kb
     }
 }
 
+// Argmax over a wavefront (largest value, first index on ties: izamax's rule)
+// with DPP lane moves inside each row of 16 lanes (xor 1, xor 2, half mirror,
+// mirror), then the four row results read out by readlane; no LDS round trips
+// and no branches.  Every lane must be active; every lane gets the result.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xf, 0xf,
+                                               false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float read_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <class Rl>
+__device__ __forceinline__ void amax_pick(Rl &v, int &i, Rl o, int oi) {
+    const bool t = (o > v) | ((o == v) & (oi < i));
+    v = t ? o : v;
+    i = t ? oi : i;
+}
+template <int CTRL, class Rl>
+__device__ __forceinline__ void amax_dpp(Rl &v, int &i) {
+    amax_pick(v, i, dpp_mov<CTRL>(v), __builtin_amdgcn_update_dpp(i, i, CTRL, 0xf, 0xf, false));
+}
+template <class Rl>
+__device__ __forceinline__ void wave_argmax(Rl &v, int &i) {
+    amax_dpp<0xB1>(v, i);    // quad_perm [1, 0, 3, 2]
+    amax_dpp<0x4E>(v, i);    // quad_perm [2, 3, 0, 1]
+    amax_dpp<0x141>(v, i);   // row_half_mirror
+    amax_dpp<0x140>(v, i);   // row_mirror: every lane holds its row's best
+    Rl b = read_lane(v, 0);
+    int bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int r = 1; r < 4; ++r)
+        amax_pick(b, bi, read_lane(v, 16 * r), __builtin_amdgcn_readlane(i, 16 * r));
+    v = b;
+    i = bi;
+}
+
 // Factor leaf columns [c0, c0 + lw) of the scratch panel over rows [c0, m) with
 // partial pivoting (largest |re| + |im|, first index on ties: izamax), the rows
 // in registers, R per thread.  An interchange swaps two register rows, so the
 // rows go back to memory in the order LAPACK's sequential interchanges give.
 // Two barriers per column: after the wavefront winners of the pivot search, and
-// after the pivot row (and, on an interchange, row j) is staged in LDS.
+// after the pivot row (and, on an interchange, row j) is staged in LDS.  The
+// pivots are kept in LDS and written once after the column loop: a global store
+// inside it makes the next LDS staging of its wavefront wait for the store.
 // ipiv receives absolute pivot rows (k is the panel's first row).
 template <class C, int R, int LW>
 __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0, int64_t sPt,
@@ -324,7 +377,8 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
     __shared__ C xch[2][LW];
     __shared__ Rl rv[16];
     __shared__ int ri[16];
-    const int tid = threadIdx.x;
+    __shared__ int pv[LW];
+    const int tid = threadIdx.x, lane = tid & 63;
     const C zero = CTraits<C>::zero();
     C x[R][LW];
 #pragma unroll
@@ -349,24 +403,19 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
                 if (v > best) { best = v; bi = i; }
             }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const Rl ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
+        wave_argmax(best, bi);
+        if (lane == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bi; }
         __syncthreads();
-        // every thread reduces the 16 wavefront winners (no second barrier)
+        // every wavefront reduces the 16 wavefront winners (no second barrier)
         int p;
         {
-            Rl b = rv[0];
-            int bx = ri[0];
-            for (int w = 1; w < NT / 64; ++w)
-                if (rv[w] > b || (rv[w] == b && ri[w] < bx)) { b = rv[w]; bx = ri[w]; }
+            Rl b = lane < NT / 64 ? rv[lane] : (Rl)-1;
+            int bx = lane < NT / 64 ? ri[lane] : 0x7fffffff;
+            wave_argmax(b, bx);
             if (bx >= mr) bx = j;   // no comparable entry (NaN column): keep the diagonal
             p = bx;
         }
-        if (tid == 0) ipiv[k + c0 + j] = (int32_t)(k + c0 + p);
+        if (tid == 0) pv[j] = p;
         // the pivot row goes to xch[0] (it is also the row the update uses); on an
         // interchange row j goes to xch[1] for the pivot row's owner
         const int op = p % NT, sp = p / NT;
@@ -416,6 +465,7 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
             for (int c = 0; c < LW; ++c)
                 if (c < lw) P[(int64_t)(c0 + c) * m + c0 + i] = x[r][c];
     }
+    if (tid < lw) ipiv[k + c0 + tid] = (int32_t)(k + c0 + pv[tid]);
 }
 
 // After a leaf: its interchanges on the panel's other columns (in pivot order),
@@ -1055,12 +1105,14 @@ void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, in
         if (m <= 1024) LF(1, 16);
         else if (m <= 2048) LF(2, 16);
         else if (m <= 3072) LF(3, 16);
+        else if (m <= 5120) LF(5, 8);
         else if (m <= 6144) LF(6, 8);
         else LF(12, 4);
     } else {
         if (m <= 1024) LF(1, 16);
         else if (m <= 2048) LF(2, 8);
         else if (m <= 3072) LF(3, 8);
+        else if (m <= 5120) LF(5, 4);
         else if (m <= 6144) LF(6, 4);
         else LF(12, 2);
     }
