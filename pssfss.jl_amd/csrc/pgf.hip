@@ -392,7 +392,42 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
         for (int f = 0; f < F; ++f)
             if (f >= nf) done |= 1u << f;
         const unsigned all = (F == 32) ? 0xffffffffu : ((1u << F) - 1u);
-        for (int r = 1; r <= JK_RINGMAX && done != all; ++r) {
+        // one phasing at psi = (0, 0) (normal incidence): every lattice phase is
+        // exactly 1, so the terms are real and the phase products can go; the
+        // sums are bit-identical to the general loop's (e * 1 = e, e * -0 adds 0)
+        const bool zp = F == 1 && nf == 1 && spsi[0] == 0.0 && spsi[1] == 0.0;
+        for (int r = 1; zp && r <= JK_RINGMAX && done != all; ++r) {
+            typename CTraits<PT>::R jr = 0, kr = 0;
+            const int len = 8 * r;
+            for (int k = 0; k < len; ++k) {
+                int m, n;
+                ring_mn(r, k, m, n);
+                double dx = urx - ((double)m * us1x + (double)n * us2x);
+                double dy = ury - ((double)m * us1y + (double)n * us2y);
+                if (MIXED) {
+                    float fx = (float)dx, fy = (float)dy;
+                    float a = __builtin_amdgcn_sqrtf(fx * fx + fy * fy);
+                    float e = __expf(-a);
+                    jr = fmaf(fdiv_rcp(e, a), 1.0f, jr);
+                    kr = fmaf(e, 1.0f, kr);
+                } else {
+                    double a = sqrt(dx * dx + dy * dy);
+                    double e = exp(-a);
+                    double ia = 1.0 / a;
+                    jr += e * ia;
+                    kr += e;
+                }
+            }
+            const cd a = cd{(double)jr, 0.0}, b = cd{(double)kr, 0.0};
+            js[0] += a;
+            ks[0] += b;
+            if (r > 5) {
+                double c1 = cabs(a / js[0]), c2 = cabs(b / ks[0]);
+                double conv = c1 > c2 ? c1 : c2;
+                if (conv < 1e-8) done = all;
+            }
+        }
+        for (int r = 1; !zp && r <= JK_RINGMAX && done != all; ++r) {
             PT jr[F], kr[F];
 #pragma unroll
             for (int f = 0; f < F; ++f) {
@@ -407,7 +442,7 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                 double dy = ury - ((double)m * us1y + (double)n * us2y);
                 if (MIXED) {
                     float fx = (float)dx, fy = (float)dy;
-                    float a = __fsqrt_rn(fx * fx + fy * fy);
+                    float a = __builtin_amdgcn_sqrtf(fx * fx + fy * fy);   // 1 ulp
                     float e = __expf(-a);
                     float ea = fdiv_rcp(e, a);
                     if (r <= RT) {
