@@ -397,20 +397,25 @@ def _zip_loops(inner, outer, faces):
 
 
 def polyring(*, s1, s2, a, b, sides: int, ntri: int, units, orient=0.0, sheet_class="J",
-             dx=0.0, dy=0.0, rot=0.0, Rsheet=0.0, fufp=False) -> RWGSheet:
+             dx=0.0, dy=0.0, rot=0.0, Rsheet=0.0, fufp=False, mesher="structured",
+             seed=0) -> RWGSheet:
     """Concentric annular regions bounded by regular polygons (src/Elements.jl:368-558):
     ring i lies between the polygons of circumradius a[i] and b[i] with `sides`
     sides, the first vertex at angle `orient` (degrees) from the cell centre
-    (s1 + s2) / 2; a[0] = 0 makes the first ring a solid polygon.  The arguments
-    are checked as the reference checks them.
+    (s1 + s2) / 2; a[0] = 0 makes the first ring a solid polygon.  b[-1] < 0 makes
+    the outer ring fill the unit cell out to its boundary, whose shorter side gets
+    -b[-1] edges (the other side in proportion), as in the reference; the sheet
+    then carries basis functions across the cell boundary.  The arguments are
+    checked as the reference checks them.
 
     The reference meshes the rings with Triangle (src/Meshsub.jl:42-111), which is
-    absent here.  This is our own structured mesher, so mesh parity with the
-    reference is unpinned.  Each ring is a stack of similar polygons whose node
-    counts grow with their perimeter.  The strips between consecutive polygons are
-    zipped into triangles (a fan around the centre for a solid polygon), with a
-    target triangle area of (total metal area) / ntri.  The option of an outer ring
-    filling the cell to its boundary (b[-1] < 0) is not supported by this mesher."""
+    absent here; mesh parity with the reference is unpinned.  mesher="delaunay"
+    uses our unstructured mesher (pssfss_amd.meshsub, seeded by `seed`) on the
+    reference's outline, with the reference's target triangle area (total metal
+    area) / ntri; a cell-filling outer ring always uses it.  mesher="structured"
+    stacks similar polygons whose node counts grow with their perimeter and zips
+    the strips between them into triangles (a fan around the centre for a solid
+    polygon), with the same target area."""
     if not sides >= 3:
         raise ValueError("sides must be at least 3")
     if not ntri > 0:
@@ -422,17 +427,20 @@ def polyring(*, s1, s2, a, b, sides: int, ntri: int, units, orient=0.0, sheet_cl
     if len(a) != len(b):
         raise ValueError("length(a) !== length(b)")
     nring = len(a)
-    if b[-1] < 0:
-        raise ValueError("an outer ring filling the unit cell (b[end] < 0) is not supported "
-                         "by the structured mesher")
+    fillcell = b[-1] < 0
     for i in range(nring):
-        if a[i] >= b[i]:
+        if (i < nring - 1 or not fillcell) and a[i] >= b[i]:
             raise ValueError(f"a[{i + 1}] >= b[{i + 1}]")
         if a[i] < 0:
             raise ValueError("radii must be non-negative")
     for i in range(nring - 1):
         if b[i] >= a[i + 1]:
             raise ValueError(f"b[{i + 1}] >= a[{i + 2}]")
+    if mesher not in ("structured", "delaunay"):
+        raise ValueError("mesher must be 'structured' or 'delaunay'")
+    if fillcell or mesher == "delaunay":
+        return _polyring_delaunay(s1, s2, a, b, sides, ntri, units, orient, sheet_class, dx, dy,
+                                  rot, Rsheet, fufp, seed)
     s1 = np.asarray(s1, float)
     s2 = np.asarray(s2, float)
     c = 0.5 * (s1 + s2)
@@ -485,6 +493,71 @@ def polyring(*, s1, s2, a, b, sides: int, ntri: int, units, orient=0.0, sheet_cl
     fv[1, flip], fv[2, flip] = fv[2, flip].copy(), fv[1, flip].copy()
     return _element_sheet("polyring", rho, fv, s1, s2, units, sheet_class, dx, dy, rot, Rsheet,
                           fufp)
+
+
+def _polyring_delaunay(s1, s2, a, b, sides, ntri, units, orient, sheet_class, dx, dy, rot,
+                       Rsheet, fufp, seed):
+    """polyring's outline as the reference builds it (src/Elements.jl:393-509), meshed
+    by pssfss_amd.meshsub: each ring's outer and inner polygon (only the outer one
+    for a solid polygon); for a cell-filling outer ring the cell boundary, ns1
+    edges along s1 and ns2 along s2, kept unsplit so that its nodes on opposite
+    sides coincide under the lattice translations."""
+    from . import meshsub as MS
+    s1 = np.asarray(s1, float)
+    s2 = np.asarray(s2, float)
+    nring = len(a)
+    fillcell = b[-1] < 0
+    c = 0.5 * (s1 + s2)
+    alpha = 2.0 * math.pi / sides
+    area_factor = 0.5 * sides * math.sin(alpha)
+    area = [(b[i] ** 2 - a[i] ** 2) * area_factor for i in range(nring)]
+    pts, segs, fixed = [], [], []
+
+    def add_loop(vs, fx):
+        s0 = len(pts)
+        pts.extend(tuple(v) for v in vs)
+        n = len(vs)
+        segs.extend((s0 + k, s0 + (k + 1) % n) for k in range(n))
+        fixed.extend([fx] * n)
+
+    if fillcell:
+        ns1 = int(-b[-1])
+        n1, n2 = np.linalg.norm(s1), np.linalg.norm(s2)
+        if n1 <= n2:
+            ns2 = int(_round_half_even(ns1 * n2 / n1))
+        else:
+            ns2 = ns1
+            ns1 = int(_round_half_even(ns2 * n1 / n2))
+        area[-1] = abs(s1[0] * s2[1] - s1[1] * s2[0]) - area_factor * a[-1] ** 2
+    for i in range(nring):
+        poly = lambda r: [c + r * np.array([math.cos(math.radians(orient) + k * alpha),
+                                            math.sin(math.radians(orient) + k * alpha)])
+                          for k in range(sides)]
+        if a[i] == 0:
+            add_loop(poly(b[i]), False)
+        elif i == nring - 1 and fillcell:
+            cell = ([(k / ns1) * s1 for k in range(ns1)] +
+                    [s1 + (k / ns2) * s2 for k in range(ns2)] +
+                    [s1 + s2 - (k / ns1) * s1 for k in range(ns1)] +
+                    [s2 - (k / ns2) * s2 for k in range(ns2)])
+            add_loop(cell, True)
+            add_loop(poly(a[i]), False)
+        else:
+            add_loop(poly(b[i]), False)
+            add_loop(poly(a[i]), False)
+    areat = sum(area)
+    P = np.asarray(pts, float).T
+    S = np.asarray(segs, np.int64).T
+    m = MS.meshsub(points=P, seglist=S, area=areat / ntri, ntri=ntri, fixed=fixed, seed=seed)
+    sh = _element_sheet("polyring", m.rho, m.fv, s1, s2, units, sheet_class, dx, dy, rot, Rsheet,
+                        fufp)
+    sh.xieta_check = fillcell
+    return sh
+
+
+def _round_half_even(x: float) -> float:
+    """Julia's round(Int, x): ties to even."""
+    return float(np.round(x))
 
 
 def _rect_union_mesh(solid, hollow, ntri: int):
@@ -574,47 +647,97 @@ def _element_sheet(style, rho, fv, s1, s2, units, sheet_class, dx, dy, rot, Rshe
     return sh
 
 
+def _outline_mesh(loops, area: float, ntri: int, seed: int):
+    """Unstructured mesh (pssfss_amd.meshsub) of the region bounded by closed
+    outlines (even-odd: nested outlines alternate metal and hole)."""
+    from . import meshsub as MS
+    P, S = MS.outline_loops(loops)
+    m = MS.meshsub(points=P, seglist=S, area=area / ntri, ntri=ntri, seed=seed)
+    return m.rho, m.fv
+
+
 def loadedcross(*, s1, s2, L1, L2, w, ntri: int, units, orient=0.0, sheet_class="J", dx=0.0,
-                dy=0.0, rot=0.0, Rsheet=0.0, fufp=False) -> RWGSheet:
+                dy=0.0, rot=0.0, Rsheet=0.0, fufp=False, mesher="structured",
+                seed=0) -> RWGSheet:
     """Loaded cross (src/Elements.jl:794-903): a cross of tip-to-tip length L1 and
     arm width L2 whose metal is the strip of width w along its outline (the cross
     minus the cross of L1 - 2w, L2 - 2w); solid when w >= L2/2.  Centred at the
     origin of the lattice cell.
 
     The reference meshes the polygon with Triangle (src/Meshsub.jl:42-111), which
-    is absent here: this is our own structured mesher (_rect_union_mesh), so mesh
-    parity with the reference is unpinned.  orient rotates the element by a
-    multiple of 90 degrees (the shape is invariant under those)."""
+    is absent here, so mesh parity with the reference is unpinned.
+    mesher="structured" (default) is our rectilinear mesher (_rect_union_mesh);
+    mesher="delaunay" meshes the outline (the cross, and the inner cross as a hole
+    when w < L2/2) with our unstructured mesher, seeded by `seed`, aiming at ntri
+    triangles.  orient rotates the element by a multiple of 90 degrees (the shape is
+    invariant under those)."""
     for name, v in (("L1", L1), ("L2", L2), ("w", w), ("ntri", ntri)):
         if not v > 0:
             raise ValueError(f"{name} must be positive!")
     if len(s1) != 2 or len(s2) != 2:
         raise ValueError("s1 and s2 must be 2-vectors")
+    if mesher not in ("structured", "delaunay"):
+        raise ValueError("mesher must be 'structured' or 'delaunay'")
+    if orient % 90 != 0:
+        raise ValueError("orient must be a multiple of 90 for the structured mesher")
+    if mesher == "delaunay":
+        from . import meshsub as MS
+        loops = [MS.cross_outline(L1, L2)]
+        area = 2 * L1 * L2 - L2 * L2
+        if 2 * w < L2:
+            l1, l2 = L1 - 2 * w, L2 - 2 * w
+            loops.append(MS.cross_outline(l1, l2))
+            area -= 2 * l1 * l2 - l2 * l2
+        rho, fv = _outline_mesh(loops, area, ntri, seed)
+        return _element_sheet("loadedcross", rho, fv, s1, s2, units, sheet_class, dx, dy, rot,
+                              Rsheet, fufp)
     solid = _cross_rects(L1, L2)
     hollow = [] if 2 * w >= L2 else _cross_rects(L1 - 2 * w, L2 - 2 * w)
     rho, fv = _rect_union_mesh(solid, hollow, ntri)
-    if orient % 90 != 0:
-        raise ValueError("orient must be a multiple of 90 for the structured mesher")
     return _element_sheet("loadedcross", rho, fv, s1, s2, units, sheet_class, dx, dy, rot,
                           Rsheet, fufp)
 
 
 def jerusalemcross(*, P, L1, L2, A, B, w, ntri: int, units, sheet_class="J", dx=0.0, dy=0.0,
-                   rot=0.0, Rsheet=0.0, fufp=False) -> RWGSheet:
+                   rot=0.0, Rsheet=0.0, fufp=False, mesher="structured", seed=0) -> RWGSheet:
     """Jerusalem cross (src/Elements.jl:963-1110) in a square cell of period P: arms
     of width L2 spanning L1 between the outer edges of the end caps, caps of length
     A and width B.  Where 2w < L2 (arms) or 2w < B (caps) the metal is the strip of
     width w along the outline (the shape minus the one with every dimension less
     2w), otherwise that part is solid.  Centred at the origin of the cell.
 
-    Our own structured mesher replaces the reference's Triangle mesh, so mesh
-    parity with the reference is unpinned."""
+    Our own meshers replace the reference's Triangle mesh, so mesh parity with the
+    reference is unpinned: mesher="structured" (default) is the rectilinear one;
+    mesher="delaunay" meshes the outline with our unstructured mesher (seeded by
+    `seed`, aiming at ntri triangles), for a solid cross or one whose arms and caps
+    are both hollow (2w < L2 and 2w < B)."""
     for name, v in (("P", P), ("L1", L1), ("L2", L2), ("A", A), ("B", B), ("w", w),
                     ("ntri", ntri)):
         if not v > 0:
             raise ValueError(f"{name} must be positive!")
     if L1 >= P or A >= P:
         raise ValueError("the cross must fit inside the unit cell")
+    if mesher not in ("structured", "delaunay"):
+        raise ValueError("mesher must be 'structured' or 'delaunay'")
+    if mesher == "delaunay":
+        from . import meshsub as MS
+        if (2 * w < L2) != (2 * w < B):
+            raise ValueError("mesher='delaunay' needs a solid cross or hollow arms and caps")
+        if A <= L2 or L1 <= 2 * B:
+            raise ValueError("mesher='delaunay' needs caps wider than the arms (A > L2, L1 > 2B)")
+
+        def jarea(l1, l2, a, bb):
+            return 2 * l1 * l2 - l2 * l2 + 4 * (a - l2) * bb
+
+        loops = [MS.jcross_outline(L1, L2, A, B)]
+        area = jarea(L1, L2, A, B)
+        if 2 * w < L2:
+            dims = (L1 - 2 * w, L2 - 2 * w, A - 2 * w, B - 2 * w)
+            loops.append(MS.jcross_outline(*dims))
+            area -= jarea(*dims)
+        rho, fv = _outline_mesh(loops, area, ntri, seed)
+        return _element_sheet("jerusalemcross", rho, fv, [float(P), 0.0], [0.0, float(P)],
+                              units, sheet_class, dx, dy, rot, Rsheet, fufp)
     solid = _jcross_rects(L1, L2, A, B)
     hollow = []
     if 2 * w < L2 and 2 * w < B:

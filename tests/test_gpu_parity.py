@@ -422,6 +422,50 @@ def test_analyze_polyring(ctx, pk, orc):
     assert dm <= 1e-6
 
 
+def test_analyze_polyring_fill_cell(ctx, pk, orc):
+    """A metal sheet filling the cell with a square hole (polyring with b[end] < 0,
+    unstructured mesh whose basis functions also cross the cell boundary) on a
+    slab, normal and oblique incidence, against the oracle."""
+    _say("analyze cell-filling polyring (square hole) on a slab, theta 0/25, 2 freqs")
+    L = pk.Layer.make
+    sh = pk.polyring(s1=[8, 0], s2=[0, 8], a=[2.5], b=[-8], sides=4, ntri=260, units="mm",
+                     orient=45)
+    strata = [L(), sh, L(width=0.5, units="mm", epsr=2.5), L()]
+    freqs = [10.0, 18.0]
+    steer = {"phi": [0.0], "theta": [0.0, 25.0]}
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    assert len(res) == len(ref) == 4
+    d = _max_dS(res, ref)
+    print(f"  faces {sh.nface}: max |dS| fp64 = {d:.3e}")
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
+def test_analyze_loadedcross_delaunay(ctx, pk, orc):
+    """A loaded cross on the unstructured mesh in free space, oblique incidence,
+    against the oracle."""
+    _say("analyze unstructured loaded cross, theta 30, 2 freqs")
+    L = pk.Layer.make
+    sh = pk.loadedcross(s1=[10, 0], s2=[0, 10], L1=8, L2=3, w=0.8, ntri=220, units="mm",
+                        mesher="delaunay")
+    strata = [L(), sh, L()]
+    freqs = [9.0, 16.0]
+    steer = {"phi": [0.0], "theta": [30.0]}
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    d = _max_dS(res, ref)
+    print(f"  faces {sh.nface}: max |dS| fp64 = {d:.3e}")
+    assert d <= 1e-9
+    resm = pk.analyze.analyze(strata, freqs, steer, precision="mixed", ctx=ctx)
+    dm = _max_dS(resm, ref)
+    print(f"  max |dS| mixed = {dm:.3e}")
+    assert dm <= 1e-6
+
+
 @pytest.mark.parametrize("cls", ["J", "M"])
 def test_grating_closed_form(ctx, pk, cls):
     _say(f"symmetric strip grating vs Collin closed form, class {cls}")

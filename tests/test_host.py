@@ -250,7 +250,7 @@ def test_polyring_meshes(pk, a, b, sides, ntri):
 @pytest.mark.parametrize("kw,msg", [(dict(a=[1.0], b=[1.0]), "a\\[1\\] >= b\\[1\\]"),
                                     (dict(a=[0.0, 1.0], b=[1.5, 2.0]), "b\\[1\\] >= a\\[2\\]"),
                                     (dict(a=[0.0], b=[1.0, 2.0]), "length"),
-                                    (dict(a=[1.0], b=[-4.0]), "not supported"),
+                                    (dict(a=[1.0], b=[2.0], mesher="triangle"), "mesher"),
                                     (dict(a=[0.0], b=[1.0], sides=2), "sides")])
 def test_polyring_argument_checks(pk, kw, msg):
     args = dict(s1=[10, 0], s2=[0, 10], sides=6, ntri=100, units="mm")
