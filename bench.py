@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="two_layer", choices=["two_layer", "ring", "patch"])
+    ap.add_argument("--only-detail", choices=["cross_batch", "meander_polarizer"],
+                    help="run one detail sample alone and print its JSON")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-detail", action="store_true",
                     help="headline workload only (no per-point FP64 sweep, adjoint, ring, "
@@ -124,6 +126,11 @@ def main():
     import numpy as np
     import pssfss_amd as pk
     from pssfss_amd import hip
+    if args.only_detail:
+        ctx = hip.get_context()
+        fn = {"cross_batch": cross_batch, "meander_polarizer": meander_polarizer}[args.only_detail]
+        print(json.dumps({args.only_detail: fn(pk, hip, ctx)}), flush=True)
+        return
 
     vt = None
     if args.workload == "two_layer":
@@ -335,11 +342,13 @@ def meander_gradient(pk, ctx):
 def cross_batch(pk, hip, ctx):
     """The 16 seeded Jerusalem / loaded cross problems (full meshes, 800-4000
     requested triangles, uneven unknown counts), each at 16 of its 64 frequencies
-    (every 4th from 1 GHz) and the 4 scan angles: 1024 points.  Timed three ways per
+    (every 4th from 1 GHz) and the 4 scan angles: 1024 points.  Timed four ways per
     precision after one untimed pass over two problems (kernels loaded): one problem
     after another with the library's default chunking; one after another with the
     tuner's points per chunk for each size class (tuned before the timed region);
-    four problems at a time (four contexts on one GPU, tuned chunks)."""
+    four problems at a time (four contexts on one GPU, tuned chunks); problems of
+    similar size merged into shared batches (size classes within 1.1x, systems
+    padded to each batch's largest unknown count)."""
     from pssfss_amd.batch import analyze_batch, ChunkTuner
     probs = pk.workloads.cross_batch()
     problems = [(s, list(f[::4]), st) for s, f, st, _ in probs]
@@ -350,20 +359,43 @@ def cross_batch(pk, hip, ctx):
            "prepare_s": round(time.perf_counter() - t0, 3)}
     tuner = ChunkTuner()
     analyze_batch(problems[:2], "fp64", workers=1, setups=setups[:2])
+    one = {}
     for prec in ("fp64", "mixed"):
         r1, s1 = analyze_batch(problems, prec, workers=1, setups=setups)
+        one[prec] = r1
         rt, st = analyze_batch(problems, prec, workers=1, tuner=tuner, setups=setups)
         r4, s4 = analyze_batch(problems, prec, workers=4, tuner=tuner, setups=setups)
+        rm, sm = analyze_batch(problems, prec, setups=setups, merge=True, class_ratio=1.1)
         out["points"] = s1["points"]
         out[f"points_per_s_{prec}_one_at_a_time"] = round(s1["points_per_s"], 3)
         out[f"points_per_s_{prec}_one_at_a_time_tuned"] = round(st["points_per_s"], 3)
         out[f"points_per_s_{prec}_four_at_a_time"] = round(s4["points_per_s"], 3)
         out[f"max_dS_{prec}_tuned_vs_default"] = max(max_dS(a, b) for a, b in zip(r1, rt))
         out[f"max_dS_{prec}_four_vs_one"] = max(max_dS(a, b) for a, b in zip(r1, r4))
+        out[f"points_per_s_{prec}_merged"] = round(sm["points_per_s"], 3)
+        out[f"max_dS_{prec}_merged_vs_one"] = max(max_dS(a, b) for a, b in zip(r1, rm))
+        out["merged_size_classes"] = sm["classes"]
         print(f"[bench] cross batch {prec}: one at a time {s1['points_per_s']:.1f} (tuned "
-              f"{st['points_per_s']:.1f}), four at a time {s4['points_per_s']:.1f} points/s",
-              flush=True)
+              f"{st['points_per_s']:.1f}), four at a time {s4['points_per_s']:.1f}, merged "
+              f"{sm['points_per_s']:.1f} points/s", flush=True)
+    out["max_dS_mixed_vs_fp64"] = max(max_dS(a, b) for a, b in zip(one["mixed"], one["fp64"]))
     out["tuned_chunks"] = tuner.table
+    # few points per problem (one frequency, 10 GHz, the 4 angles): one problem's
+    # batch is too small to fill the GPU, the merged batches are not
+    few = [(s, [10.0], st) for s, _, st in problems]
+    fsets = [pk.analyze.prepare(s, f, st, classify=lambda sh: hip.face_pair_classes(ctx, sh))
+             for s, f, st in few]
+    for prec in ("fp64", "mixed"):
+        analyze_batch(few[:2], prec, workers=1, setups=fsets[:2])
+        ra, sa = analyze_batch(few, prec, workers=1, setups=fsets)
+        rb, sb = analyze_batch(few, prec, setups=fsets, merge=True, class_ratio=1.25)
+        out[f"few_points_{prec}"] = {
+            "points": sa["points"], "one_at_a_time_points_per_s": round(sa["points_per_s"], 3),
+            "merged_points_per_s": round(sb["points_per_s"], 3),
+            "merged_size_classes": sb["classes"],
+            "max_dS_merged_vs_one": max(max_dS(a, b) for a, b in zip(ra, rb))}
+        print(f"[bench] cross batch, 4 points per problem, {prec}: one at a time "
+              f"{sa['points_per_s']:.1f}, merged {sb['points_per_s']:.1f} points/s", flush=True)
     return out
 
 

@@ -146,6 +146,19 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
                            const pssfss_point *pts, pssfss_gsm *gsms, int precision,
                            pssfss_point_info *infos);
 
+/* Points of several sheets (independent problems, or several sheets of one stack)
+ * in shared batches: sheet g contributes npts[g] consecutive entries of pts / gsms /
+ * infos.  Every point must have the same end-region mode counts.  Each batch is
+ * filled per sheet, then factored and solved as one set of systems padded to the
+ * batch's largest unknown count (identity block below and right of each sheet's
+ * matrix, zero right-hand-side rows), which leaves each sheet's solution what its
+ * own batch would give.  Replaces the reference's problem-by-problem, point-by-point
+ * loop (src/PSSFSS.jl:230-305) when many structures of uneven size are analysed.
+ * infos may be NULL. */
+int pssfss_sheets_gsm_batch(pssfss_ctx *ctx, int nsheet, pssfss_sheet *const *sheets,
+                            const int *npts, const pssfss_point *pts, pssfss_gsm *gsms,
+                            int precision, pssfss_point_info *infos);
+
 /* ---- sensitivities (no counterpart in the reference) ---------------------------
  * Derivatives of the sheet GSM with respect to mesh geometry, for gradient-based
  * design through a ChainRulesCore.rrule on the S-parameter output.  For each of

@@ -638,10 +638,12 @@ struct ChunkResult {
 // Class terms of the fill are staged for at most this many bytes of points at once.
 constexpr int64_t FILL_G_BYTES = 4ll << 30;
 
-// Fill np points into ctx->wZ (row-major, one nbf^2 block per point).
+// Fill np points into ctx->wZ (row-major, one nbf^2 block per point), or, with zbase,
+// into ld x ld blocks (ld >= nbf, leading dimension ld) from zbase on.
 void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int np, int mixed,
                 ChunkResult &res, TableSet *keep_tabs = nullptr,
-                std::vector<FillParams> *keep_fp = nullptr) {
+                std::vector<FillParams> *keep_fp = nullptr, cd *zbase = nullptr,
+                int64_t ld = 0) {
     hipStream_t st = ctx->st;
     const double ul = pts[0].u / sh->upm;
     for (int p = 1; p < np; ++p)
@@ -658,8 +660,11 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     build_tables(ctx, tin, CONVTEST_FILL, tabs);
     res.M = tabs.M;
     res.conv = tabs.converged;
-    const int64_t nb2 = sh->nbf * sh->nbf;
-    ctx->wZ.ensure((size_t)nb2 * np);
+    if (!zbase) {
+        ld = sh->nbf;
+        ctx->wZ.ensure((size_t)ld * ld * np);
+        zbase = ctx->wZ.p;
+    }
     std::vector<FillParams> fp(np);
     for (int p = 0; p < np; ++p) {
         fp[p] = make_fill_params(sh, &pts[p]);
@@ -667,7 +672,8 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
         fp[p].tab1 = tabs.tab1(p);
         fp[p].tab2 = tabs.tab2(p);
         fp[p].jk = jkp[p];
-        fp[p].Z = ctx->wZ.p + p * nb2;
+        fp[p].Z = zbase + p * ld * ld;
+        fp[p].ldz = ld;
     }
     DBuf<FillParams> dfp(np);
     dfp.upload(fp.data(), np, st);
@@ -913,97 +919,162 @@ void col_source(const pssfss_point &pt, int n1, int col, bool mag, cd &s0, cd &s
     }
 }
 
-void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfss_gsm *gsms,
-               int np, int precision, pssfss_point_info *infos) {
+// The points of one sheet inside a chunk.
+struct GsmGroup {
+    pssfss_sheet *sh;
+    const pssfss_point *pts;
+    pssfss_gsm *gsms;
+    pssfss_point_info *infos;   // optional
+    int np;
+};
+
+// One chunk of the sheet GSM pipeline for the points of one or more sheets that
+// share the end regions' mode counts.  Every system is held as an ld x ld block, ld
+// the largest unknown count of the chunk; a sheet with fewer unknowns n fills the
+// leading n x n block and the rest is the identity's (launch_pad_identity), its
+// right-hand sides are zero below row n.  P [Z 0; 0 I] = [L 0; 0 I][U 0; 0 I] with
+// the same pivots as Z alone (no interchange can reach a padded row: its entries in
+// the first n columns are zero), so one batched factorisation and solve serve
+// sheets of uneven size and give each sheet's own solutions.
+void gsm_chunk_groups(pssfss_ctx *ctx, const std::vector<GsmGroup> &groups, int precision) {
     hipStream_t st = ctx->st;
-    const bool mag = sh->cls == 'M';
-    const int n1 = pts[0].nmodes[0], n2 = pts[0].nmodes[1];
-    for (int p = 1; p < np; ++p)
-        if (pts[p].nmodes[0] != n1 || pts[p].nmodes[1] != n2)
-            throw ArgError("batched points must have equal mode counts");
+    if (groups.empty()) return;
+    const int n1 = groups[0].pts[0].nmodes[0], n2 = groups[0].pts[0].nmodes[1];
+    int64_t ld = 0;
+    int npt = 0;
+    std::vector<int> off;
+    for (const GsmGroup &g : groups) {
+        for (int p = 0; p < g.np; ++p)
+            if (g.pts[p].nmodes[0] != n1 || g.pts[p].nmodes[1] != n2)
+                throw ArgError("batched points must have equal mode counts");
+        ld = std::max<int64_t>(ld, g.sh->nbf);
+        off.push_back(npt);
+        npt += g.np;
+    }
     const int ncol = n1 + n2;
-    const int64_t n = sh->nbf;
-    ChunkResult cr;
-    fill_chunk(ctx, sh, pts, np, precision == PSSFSS_MIXED, cr);
-    std::vector<BfftJob> jobs;
-    std::vector<std::vector<int>> jobof;
-    make_jobs(pts, np, n1, n2, jobs, jobof);
-    const int njobs = (int)jobs.size();
-    DBuf<BfftJob> djobs(njobs);
-    djobs.upload(jobs.data(), njobs, st);
-    ctx->wft.ensure((size_t)njobs * n * 2);
-    stage_begin(6, st);
-    launch_bfft(sh->dev(), sh->bf_face.p, sh->bf_loc.p, djobs.p, njobs, ctx->wft.p, st);
-    // Right-hand sides, eq. (7.39) / (7.64).
-    std::vector<int32_t> coljob(np * ncol);
-    std::vector<cd> colsrc(2 * np * ncol);
-    for (int p = 0; p < np; ++p) {
-        const pssfss_point &pt = pts[p];
-        for (int col = 0; col < ncol; ++col) {
-            cd s0, s1;
-            col_source(pt, n1, col, mag, s0, s1);
-            coljob[p * ncol + col] = jobof[p][col];
-            colsrc[2 * (p * ncol + col)] = s0;
-            colsrc[2 * (p * ncol + col) + 1] = s1;
+    const int ng = (int)groups.size();
+    // fill every group into its slots of one ld x ld block per point
+    ctx->wZ.ensure((size_t)ld * ld * npt);
+    std::vector<ChunkResult> cr(ng);
+    bool padded = false;
+    for (int g = 0; g < ng; ++g) {
+        const GsmGroup &G = groups[g];
+        cd *zb = ctx->wZ.p + (int64_t)off[g] * ld * ld;
+        fill_chunk(ctx, G.sh, G.pts, G.np, precision == PSSFSS_MIXED, cr[g], nullptr, nullptr, zb,
+                   ld);
+        if (G.sh->nbf < ld) {
+            launch_pad_identity(zb, G.sh->nbf, ld, ld * ld, G.np, st);
+            padded = true;
         }
     }
-    DBuf<int32_t> dcj(coljob.size());
-    DBuf<cd> dcs(colsrc.size());
-    dcj.upload(coljob.data(), coljob.size(), st);
-    dcs.upload(colsrc.data(), colsrc.size(), st);
-    ctx->wB.ensure((size_t)n * ncol * np);
-    launch_rhs(n, np * ncol, ctx->wft.p, dcj.p, dcs.p, ctx->wB.p, n, st);
-    stage_end(6, st);
-    solve_chunk(ctx, n, ncol, np, precision, infos);
-    // Projection of the solutions onto the modes' transforms.
-    ctx->wout.ensure((size_t)ncol * njobs * 2 * np);
-    std::vector<cd> hout((size_t)np * ncol * njobs * 2);
-    // one launch per point: columns of point p against all jobs
+    // transforms of the end-region modes and right-hand sides, per sheet
+    ctx->wB.ensure((size_t)ld * ncol * npt);
+    if (padded) launch_zero(ctx->wB.p, ld * ncol * npt, st);
+    std::vector<std::vector<BfftJob>> jobs(ng);
+    std::vector<std::vector<std::vector<int>>> jobof(ng);
+    std::vector<DBuf<cd>> ft(ng);
+    std::vector<DBuf<BfftJob>> djobs(ng);
+    std::vector<DBuf<int32_t>> dcj(ng);
+    std::vector<DBuf<cd>> dcs(ng);
     stage_begin(6, st);
-    for (int p = 0; p < np; ++p)
-        launch_project(n, ncol, njobs, ctx->wB.p + (int64_t)p * n * ncol, n, ctx->wft.p,
-                       ctx->wout.p + (int64_t)p * ncol * njobs * 2, st);
+    for (int g = 0; g < ng; ++g) {
+        const GsmGroup &G = groups[g];
+        const int64_t n = G.sh->nbf;
+        const bool mag = G.sh->cls == 'M';
+        make_jobs(G.pts, G.np, n1, n2, jobs[g], jobof[g]);
+        const int nj = (int)jobs[g].size();
+        djobs[g].upload(jobs[g].data(), nj, st);
+        ft[g].ensure((size_t)nj * n * 2);
+        launch_bfft(G.sh->dev(), G.sh->bf_face.p, G.sh->bf_loc.p, djobs[g].p, nj, ft[g].p, st);
+        // Right-hand sides, eq. (7.39) / (7.64).
+        std::vector<int32_t> coljob(G.np * ncol);
+        std::vector<cd> colsrc(2 * G.np * ncol);
+        for (int p = 0; p < G.np; ++p)
+            for (int col = 0; col < ncol; ++col) {
+                cd s0, s1;
+                col_source(G.pts[p], n1, col, mag, s0, s1);
+                coljob[p * ncol + col] = jobof[g][p][col];
+                colsrc[2 * (p * ncol + col)] = s0;
+                colsrc[2 * (p * ncol + col) + 1] = s1;
+            }
+        dcj[g].upload(coljob.data(), coljob.size(), st);
+        dcs[g].upload(colsrc.data(), colsrc.size(), st);
+        launch_rhs(n, G.np * ncol, ft[g].p, dcj[g].p, dcs[g].p,
+                   ctx->wB.p + (int64_t)off[g] * ld * ncol, ld, st);
+    }
+    stage_end(6, st);
+    std::vector<pssfss_point_info> infos(npt);
+    memset(infos.data(), 0, sizeof(pssfss_point_info) * npt);
+    solve_chunk(ctx, ld, ncol, npt, precision, infos.data());
+    // Projection of the solutions onto the modes' transforms: one launch per point,
+    // its columns against all jobs of its sheet.
+    std::vector<size_t> oout(ng + 1, 0);
+    for (int g = 0; g < ng; ++g)
+        oout[g + 1] = oout[g] + (size_t)groups[g].np * ncol * jobs[g].size() * 2;
+    ctx->wout.ensure(oout[ng]);
+    std::vector<cd> hout(oout[ng]);
+    stage_begin(6, st);
+    for (int g = 0; g < ng; ++g) {
+        const GsmGroup &G = groups[g];
+        const int nj = (int)jobs[g].size();
+        for (int p = 0; p < G.np; ++p)
+            launch_project(G.sh->nbf, ncol, nj, ctx->wB.p + (int64_t)(off[g] + p) * ld * ncol, ld,
+                           ft[g].p, ctx->wout.p + oout[g] + (size_t)p * ncol * nj * 2, st);
+    }
     stage_end(6, st);
     d2h(ctx, hout.data(), ctx->wout.p, hout.size());
     sync(ctx);
     // Scattered-field GSM entries, eqs. (6.18) / (6.37).
-    const double area = sh->s1[0] * sh->s2[1] - sh->s1[1] * sh->s2[0];
-    const double area_m2 = fabs(area) / (sh->upm * sh->upm);
-    for (int p = 0; p < np; ++p) {
-        const pssfss_point &pt = pts[p];
-        pssfss_gsm &g = gsms[p];
-        const int nm[2] = {n1, n2};
-        double *blk[2][2] = {{g.s11, g.s12}, {g.s21, g.s22}};
-        for (int col = 0; col < ncol; ++col) {
-            int sr = col < n1 ? 0 : 1, qp = col < n1 ? col : col - n1;
-            double sigma = sr == 0 ? 1.0 : -1.0;
-            for (int orr = 0; orr < 2; ++orr) {
-                double *sm = blk[orr][sr];
-                if (!sm) continue;
-                for (int q = 0; q < nm[orr]; ++q) {
-                    int ocol = orr == 0 ? q : n1 + q;
-                    int job = jobof[p][ocol];
-                    const cd *FT = &hout[(((size_t)p * ncol + col) * njobs + job) * 2];
-                    double tx = pt.tvec[orr][2 * q], ty = pt.tvec[orr][2 * q + 1];
-                    cd tl = lay(pt.tlgf, q + pt.nmax * orr);
-                    cd cq = lay(pt.c[orr], q);
-                    cd *e = (cd *)(sm + 2 * (q + (int64_t)nm[orr] * qp));
-                    if (!mag) {
-                        cd dot = tx * FT[0] + ty * FT[1];
-                        *e = *e - dot * (tl / ((cq * pt.acf[orr]) * area_m2));
-                    } else {
-                        cd dot = (-ty) * FT[0] + tx * FT[1];
-                        *e = *e + dot * ((sigma * tl) * cq);
+    for (int g = 0; g < ng; ++g) {
+        const GsmGroup &G = groups[g];
+        const pssfss_sheet *sh = G.sh;
+        const bool mag = sh->cls == 'M';
+        const int njobs = (int)jobs[g].size();
+        const cd *hg = hout.data() + oout[g];
+        const double area = sh->s1[0] * sh->s2[1] - sh->s1[1] * sh->s2[0];
+        const double area_m2 = fabs(area) / (sh->upm * sh->upm);
+        for (int p = 0; p < G.np; ++p) {
+            const pssfss_point &pt = G.pts[p];
+            pssfss_gsm &gs = G.gsms[p];
+            const int nm[2] = {n1, n2};
+            double *blk[2][2] = {{gs.s11, gs.s12}, {gs.s21, gs.s22}};
+            for (int col = 0; col < ncol; ++col) {
+                int sr = col < n1 ? 0 : 1, qp = col < n1 ? col : col - n1;
+                double sigma = sr == 0 ? 1.0 : -1.0;
+                for (int orr = 0; orr < 2; ++orr) {
+                    double *sm = blk[orr][sr];
+                    if (!sm) continue;
+                    for (int q = 0; q < nm[orr]; ++q) {
+                        int ocol = orr == 0 ? q : n1 + q;
+                        int job = jobof[g][p][ocol];
+                        const cd *FT = &hg[(((size_t)p * ncol + col) * njobs + job) * 2];
+                        double tx = pt.tvec[orr][2 * q], ty = pt.tvec[orr][2 * q + 1];
+                        cd tl = lay(pt.tlgf, q + pt.nmax * orr);
+                        cd cq = lay(pt.c[orr], q);
+                        cd *e = (cd *)(sm + 2 * (q + (int64_t)nm[orr] * qp));
+                        if (!mag) {
+                            cd dot = tx * FT[0] + ty * FT[1];
+                            *e = *e - dot * (tl / ((cq * pt.acf[orr]) * area_m2));
+                        } else {
+                            cd dot = (-ty) * FT[0] + tx * FT[1];
+                            *e = *e + dot * ((sigma * tl) * cq);
+                        }
                     }
                 }
             }
-        }
-        if (infos) {
-            infos[p].mmax = cr.M[p];
-            infos[p].table_converged = cr.conv[p];
-            infos[p].jk_rings_max = 0;
+            if (G.infos) {
+                G.infos[p] = infos[off[g] + p];
+                G.infos[p].mmax = cr[g].M[p];
+                G.infos[p].table_converged = cr[g].conv[p];
+                G.infos[p].jk_rings_max = 0;
+            }
         }
     }
+}
+
+void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfss_gsm *gsms,
+               int np, int precision, pssfss_point_info *infos) {
+    gsm_chunk_groups(ctx, {GsmGroup{sh, pts, gsms, infos, np}}, precision);
 }
 
 // Sensitivities of the sheet GSM along nparam geometry directions, FP64, 'J'
@@ -1486,6 +1557,46 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
         for (int p0 = 0; p0 < npts; p0 += c) {
             int np = std::min(c, npts - p0);
             gsm_chunk(ctx, sheet, pts + p0, gsms + p0, np, precision, infos ? infos + p0 : nullptr);
+        }
+    });
+}
+
+int pssfss_sheets_gsm_batch(pssfss_ctx *ctx, int nsheet, pssfss_sheet *const *sheets,
+                            const int *npts, const pssfss_point *pts, pssfss_gsm *gsms,
+                            int precision, pssfss_point_info *infos) {
+    return guarded(ctx, [&] {
+        if (nsheet < 0 || (nsheet > 0 && (!sheets || !npts)) || !gsms || !pts)
+            throw ArgError("null argument");
+        if (precision != PSSFSS_FP64 && precision != PSSFSS_MIXED)
+            throw ArgError("precision must be PSSFSS_FP64 or PSSFSS_MIXED");
+        HIPCHECK(hipSetDevice(ctx->device));
+        int total = 0;
+        pssfss_sheet *big = nullptr;
+        for (int g = 0; g < nsheet; ++g) {
+            if (!sheets[g] || npts[g] < 0) throw ArgError("null sheet or negative point count");
+            for (int p = 0; p < npts[g]; ++p) check_point(sheets[g], &pts[total + p], true);
+            total += npts[g];
+            if (npts[g] > 0 && (!big || sheets[g]->nbf > big->nbf)) big = sheets[g];
+        }
+        if (infos) memset(infos, 0, sizeof(pssfss_point_info) * total);
+        if (total == 0) return;
+        // batches sized for the largest sheet (every system is held at its size)
+        const int c = chunk_size(ctx, big, total, precision);
+        int g = 0, p = 0;        // next point: sheet g, its point p
+        int done = 0;
+        while (done < total) {
+            std::vector<GsmGroup> groups;
+            int take = std::min(c, total - done);
+            while (take > 0) {
+                while (p >= npts[g]) { ++g; p = 0; }
+                const int k = std::min(take, npts[g] - p);
+                groups.push_back(GsmGroup{sheets[g], pts + done, gsms + done,
+                                          infos ? infos + done : nullptr, k});
+                p += k;
+                done += k;
+                take -= k;
+            }
+            gsm_chunk_groups(ctx, groups, precision);
         }
     });
 }

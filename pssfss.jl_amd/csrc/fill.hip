@@ -4,6 +4,8 @@
 // the analytic singular terms, and stores the 9 flag-free local edge-pair terms.
 // Phase 2: one lane per Z entry gathers the terms of its <= 4 face pairs.  fillz: src/FillZY.jl:50-245; filly: :273-457;
 // zint: src/Zint.jl:218-245; Sigma interpolation: src/PGF.jl:392-443.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace {
@@ -343,7 +345,26 @@ __global__ __launch_bounds__(256) void k_fill_gather(SheetDev sh, const FillPara
             acc += zc;
         }
     }
-    P.Z[e] = acc;     // row-major Z[m][s]
+    P.Z[m * (P.ldz ? P.ldz : nbf) + s] = acc;     // row-major Z[m][s]
+}
+
+__global__ void k_pad_identity(cd *Z0, int64_t n, int64_t ld, int64_t sZ) {
+    cd *Z = Z0 + (int64_t)blockIdx.y * sZ;
+    const int64_t pad = ld - n;
+    // rows [0, n): columns [n, ld); rows [n, ld): every column
+    const int64_t cnt = n * pad + pad * ld;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r, c;
+        if (t < n * pad) {
+            r = t / pad;
+            c = n + t % pad;
+        } else {
+            r = n + (t - n * pad) / ld;
+            c = (t - n * pad) % ld;
+        }
+        Z[r * ld + c] = r == c ? cd{1.0, 0.0} : cd{0.0, 0.0};
+    }
 }
 
 __global__ void k_zero(cd *p, int64_t n) {
@@ -359,6 +380,15 @@ void launch_zero(cd *p, int64_t n, hipStream_t st) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+    HIPCHECK(hipGetLastError());
+}
+
+void launch_pad_identity(cd *Z, int64_t n, int64_t ld, int64_t sZ, int npts, hipStream_t st) {
+    if (n >= ld || npts <= 0) return;
+    const int64_t cnt = (ld - n) * (n + ld);
+    const int64_t blocks = std::min<int64_t>((cnt + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)blocks, (unsigned)npts), dim3(256), 0, st, Z,
+                       n, ld, sZ);
     HIPCHECK(hipGetLastError());
 }
 

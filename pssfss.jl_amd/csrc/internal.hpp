@@ -119,7 +119,8 @@ struct FillParams {
     int M, pad;
     const cd *tab1, *tab2;      // (M+2)^2 each
     const cd *jk;               // 6*nufp: J, Jxi, Jeta, K, Kxi, Keta (SoA)
-    cd *Z;                      // nbf*nbf
+    cd *Z;                      // nbf x nbf, row-major with leading dimension ldz
+    int64_t ldz;                // 0: nbf
     const cd *dtab1, *dtab2;    // d/dk0 of the tables (k0-derivative fill only)
     cd dc3, dd3;                // d/dk0 of c3, d3 (electric) or KFfact, KPfact (magnetic)
 };
@@ -157,6 +158,9 @@ void launch_fill_classes(const SheetDev &sh, const FillParams *d_fp, int npts,
 // d/dk0 of the class terms of one point (P.dtab1, P.dtab2, P.dc3, P.dd3 set).
 void launch_fill_classes_dk0(const SheetDev &sh, const FillParams *d_fp, const double *rho_r,
                              const double *rinv, int magnetic, cd *G, hipStream_t st);
+// Rows and columns [n, ld) of npts row-major ld x ld blocks (stride sZ) set to the
+// identity's, so that a system of n unknowns padded to ld factors as itself.
+void launch_pad_identity(cd *Z, int64_t n, int64_t ld, int64_t sZ, int npts, hipStream_t st);
 // Phase 2: Z of each point from G (fpcls: class of face pair ifm + nface*ifs, CSR only).
 void launch_fill_gather(const SheetDev &sh, const FillParams *d_fp, int npts,
                         const int32_t *bf_face, const int32_t *bf_loc, const int32_t *fpcls,

@@ -583,7 +583,8 @@ constexpr int JK_MFMA_WH = 1;
 
 template <class T>
 struct LatPtT {
-    T ox, oy;
+    T ox, oy;      // the lattice offset rounded to T
+    T lx, ly;      // what that rounding left out (FP32 only; zero in FP64)
     int m, n;
 };
 typedef LatPtT<float> LatPt;
@@ -597,8 +598,12 @@ __global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, La
     int m, n;
     ring_mn(r, (int)(e - ring_base(r)), m, n);
     LatPtT<T> L;
-    L.ox = (T)((double)m * us1x + (double)n * us2x);
-    L.oy = (T)((double)m * us1y + (double)n * us2y);
+    const double ox = (double)m * us1x + (double)n * us2x;
+    const double oy = (double)m * us1y + (double)n * us2y;
+    L.ox = (T)ox;
+    L.oy = (T)oy;
+    L.lx = (T)(ox - (double)L.ox);
+    L.ly = (T)(oy - (double)L.oy);
     L.m = m;
     L.n = n;
     lat[e] = L;
@@ -683,7 +688,12 @@ __device__ __forceinline__ void jk_mfma_body(SheetDev sh, int64_t nufp, double u
     __syncthreads();
     // this thread's term row (its lane) relative to the lattice offsets; its
     // phase slot (point pkk, phasing pf)
+    // in FP32 both the row's offset and the lattice offsets are pairs hi + lo, so
+    // that a distance is their hi difference plus their lo difference: accurate to
+    // FP32 relative to the distance itself, also where an observation point lies
+    // close to a lattice image of the source point (the hi parts then cancel)
     const T urx = (T)urxs[lane], ury = (T)urys[lane];
+    const T urlx = (T)(urxs[lane] - (double)urx), urly = (T)(urys[lane] - (double)ury);
     const int pkk = tid / FT, pf = tid % FT;
     const int pc = lane & 15;
     cd js[2][4], ks[2][4];
@@ -712,13 +722,16 @@ __device__ __forceinline__ void jk_mfma_body(SheetDev sh, int64_t nufp, double u
     // while the MFMAs of chunk k run, so that the term arithmetic and the matrix
     // products overlap; lattice data is loaded one chunk earlier still.
     constexpr int NCH = JK_RINGMAX * (JK_RINGMAX + 1) / 2;
-    T2 lt[2 / WH];                   // term offsets of the next chunk to be formed
+    struct T4 {
+        T x, y, lx, ly;
+    };
+    T4 lt[2 / WH];                   // term offsets of the next chunk to be formed
     int2 lq;                         // phase indices of that chunk
     auto load_lat = [&](int k) {
 #pragma unroll
         for (int h = 0; h < 2 / WH; ++h) {
             const LatPtT<T> &L = lat[1 + 8 * (int64_t)k + wave + 4 * WH * h];
-            lt[h] = T2{L.ox, L.oy};
+            lt[h] = T4{L.ox, L.oy, L.lx, L.ly};
         }
         const LatPtT<T> &Q = lat[1 + 8 * (int64_t)k + pkk];
         lq = int2{Q.m, Q.n};
@@ -742,13 +755,15 @@ __device__ __forceinline__ void jk_mfma_body(SheetDev sh, int64_t nufp, double u
     // term h of this thread for the chunk in lt, into buffer b
     auto form_term = [&](int b, int h) {
         const int kk = wave + 4 * WH * h;
-        const T dx = urx - lt[h].x, dy = ury - lt[h].y;
         if constexpr (F32) {
+            const float dx = (urx - lt[h].x) + (urlx - lt[h].lx);
+            const float dy = (ury - lt[h].y) + (urly - lt[h].ly);
             const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
             const float ex = __expf(-a);
             TL.Aj[b][kk][lane] = fdiv_rcp(ex, a);
             TL.Ak[b][kk][lane] = ex;
         } else {
+            const double dx = urx - lt[h].x, dy = ury - lt[h].y;
             const double a = sqrt(dx * dx + dy * dy);
             const double ex = exp(-a);
             TL.Aj[b][kk][lane] = ex * (1.0 / a);
@@ -814,10 +829,10 @@ __device__ __forceinline__ void jk_mfma_body(SheetDev sh, int64_t nufp, double u
             }
             __syncthreads();
             if (k + 1 < kend) continue;
-            // ring r complete.  Rings 1..4 stay in the FP32 accumulators and are
-            // folded with ring 5 (the break test starts after ring 5); from ring 5
-            // on, every ring is folded into the FP64 running sums and tested.
-            if (r >= 5) {
+            // ring r complete: folded into the FP64 running sums (in FP32 every ring
+            // on its own, which keeps the FP32 accumulation short) and, after ring 5,
+            // tested; FP64 accumulates rings 1..4 with ring 5.
+            if (r >= 5 || F32) {
 #pragma unroll
                 for (int g = 0; g < NG; ++g)
 #pragma unroll

@@ -14,6 +14,13 @@ ChunkTuner picks the points per chunk for each class of unknown count (rounded u
 to a multiple of 512) and precision by timing candidate chunk sizes on the first
 points of the first problem of that class, and records the choice (optionally in
 a JSON file that later runs read back).
+
+merge=True instead puts problems of similar size into shared batches: the sheet
+Gblocks of all problems are sorted by unknown count and cut into size classes
+(largest / smallest <= class_ratio, equal end-region mode counts), and each class
+goes through pssfss_sheets_gsm_batch, whose batches are filled sheet by sheet and
+then factored and solved together, every system padded to the batch's largest
+unknown count (padding work <= class_ratio^3 - 1 of the smallest member's).
 """
 import json
 import os
@@ -75,9 +82,68 @@ class ChunkTuner:
         return best
 
 
+def size_classes(entries, ratio: float):
+    """Cut (key, unknowns, mode count) entries into classes of equal mode count whose
+    largest / smallest unknown count stays within ratio, each class in order of
+    increasing size; returns lists of keys."""
+    out = []
+    by_modes = {}
+    for key, n, nc in entries:
+        by_modes.setdefault(nc, []).append((n, key))
+    for nc in sorted(by_modes):
+        cur, lo = [], None
+        for n, key in sorted(by_modes[nc], key=lambda t: t[0]):
+            if cur and n > ratio * lo:
+                out.append(cur)
+                cur, lo = [], None
+            if lo is None:
+                lo = n
+            cur.append(key)
+        if cur:
+            out.append(cur)
+    return out
+
+
+def _analyze_merged(ctx, setups, precision: str, ratio: float):
+    solvers = [hip.SheetSolver(ctx, su, precision) for su in setups]
+    try:
+        t0 = time.perf_counter()
+        reqs = [s.requests() for s in solvers]
+        entries = []
+        for i, (_, _, rq) in enumerate(reqs):
+            for ig, lst in rq.items():
+                if not lst:
+                    continue
+                pt = lst[0][1].point
+                entries.append(((i, ig), int(solvers[i].sheet_of(ig).rwgdat.nbf),
+                                int(pt.nmodes[0] + pt.nmodes[1])))
+        solved = [dict() for _ in setups]
+        infos = [[] for _ in setups]
+        classes = size_classes(entries, ratio)
+        for cls in classes:
+            groups = [(solvers[i].sheet_of(ig), [b for _, b in reqs[i][2][ig]]) for i, ig in cls]
+            out = hip.sheets_gsm_batch(ctx, groups, precision)
+            for (i, ig), (gsms, inf) in zip(cls, out):
+                for (ip, _), g in zip(reqs[i][2][ig], gsms):
+                    solved[i][(ip, ig)] = g
+                infos[i] += inf
+        results = []
+        for i, s in enumerate(solvers):
+            points, live, _ = reqs[i]
+            results.append(s.finish(points, live, solved[i], infos[i])[0])
+        wall = time.perf_counter() - t0
+    finally:
+        for s in solvers:
+            s.close()
+    npts = sum(len(r) for r in results)
+    sizes = {k: n for k, n, _ in entries}
+    return results, {"seconds": wall, "points": npts, "points_per_s": npts / wall,
+                     "classes": [[sizes[k] for k in c] for c in classes]}
+
+
 def analyze_batch(problems: List[tuple], precision: str = "mixed", workers: int = 1,
                   tuner: Optional[ChunkTuner] = None, chunk: int = 0, setups=None,
-                  device: int = 0):
+                  device: int = 0, merge: bool = False, class_ratio: float = 1.1):
     """Analyze independent problems, `workers` at a time on one GPU.
 
     problems: (strata, freqs, steering) tuples as analyze takes them; setups: their
@@ -86,11 +152,19 @@ def analyze_batch(problems: List[tuple], precision: str = "mixed", workers: int 
     thread on the process context (device 0), whose workspaces, grown once, serve
     every problem; several workers each open a context on `device`.  Returns
     (results per problem in input order, stats: wall seconds, points, per-problem
-    seconds and chunks)."""
+    seconds and chunks).  merge: problems of similar size share batches (one
+    worker, the process context; chunk sizes from the library's memory rule unless
+    chunk is given); stats then lists the size classes."""
     ctx0 = hip.get_context()
     if setups is None:
         setups = [prepare(st, fr, sr, classify=lambda sh: hip.face_pair_classes(ctx0, sh))
                   for st, fr, sr in problems]
+    if merge:
+        ctx0.set_chunk(chunk, 0.45)
+        try:
+            return _analyze_merged(ctx0, setups, precision, class_ratio)
+        finally:
+            ctx0.set_chunk(0, 0.0)
     order = sorted(range(len(setups)), key=lambda i: -_cost(setups[i]))
     share = min(0.45, 0.8 / max(1, workers))
     chunks = [chunk] * len(setups)

@@ -89,6 +89,9 @@ def lib():
         L.pssfss_sheet_free.argtypes = [vp, vp]
         L.pssfss_sheet_gsm.argtypes = [vp, vp, C.POINTER(Point), C.POINTER(Gsm), C.c_int,
                                        C.POINTER(PointInfo)]
+        L.pssfss_sheets_gsm_batch.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int),
+                                              C.POINTER(Point), C.POINTER(Gsm), C.c_int,
+                                              C.POINTER(PointInfo)]
         L.pssfss_sheet_gsm_batch.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.POINTER(Gsm),
                                              C.c_int, C.POINTER(PointInfo)]
         L.pssfss_sheet_adjoint.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.c_int, _dp,
@@ -126,7 +129,7 @@ EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssf
            "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
            "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint",
            "pssfss_face_pair_classes", "pssfss_fill_dk0", "pssfss_lu_solve_batch_noalloc",
-           "pssfss_ctx_set_chunk")
+           "pssfss_ctx_set_chunk", "pssfss_sheets_gsm_batch")
 
 
 def _d(a):
@@ -409,6 +412,28 @@ def sheet_gsm_batch(ctx: Context, dsheet: DeviceSheet, builders: List[PointBuild
     return [b.result() for b in builders], list(infos)
 
 
+def sheets_gsm_batch(ctx: Context, groups, precision: str = "fp64"):
+    """Sheet GSMs of several sheets in shared batches (pssfss_sheets_gsm_batch):
+    groups is a list of (DeviceSheet, [PointBuilder, ...]); every point must have
+    the same end-region mode counts.  Returns per group (gsms, infos)."""
+    builders = [b for _, bs in groups for b in bs]
+    n = len(builders)
+    if n == 0:
+        return [([], []) for _ in groups]
+    pts = (Point * n)(*[b.point for b in builders])
+    gs = (Gsm * n)(*[b.gsm_struct() for b in builders])
+    infos = (PointInfo * n)()
+    handles = (C.c_void_p * len(groups))(*[d.h for d, _ in groups])
+    counts = (C.c_int * len(groups))(*[len(bs) for _, bs in groups])
+    ctx.check(lib().pssfss_sheets_gsm_batch(ctx.h, len(groups), handles, counts, pts, gs,
+                                            PRECISION[precision], infos))
+    out, k = [], 0
+    for _, bs in groups:
+        out.append(([b.result() for b in bs], list(infos[k:k + len(bs)])))
+        k += len(bs)
+    return out
+
+
 class SheetSolver:
     """Runs the `_analyze` sweep with every sheet GSM computed on the device,
     all points of one sheet Gblock in one batched call."""
@@ -439,6 +464,28 @@ class SheetSolver:
 
     def run(self, limit: int = 0):
         """The sweep (its first `limit` analysed points when limit > 0)."""
+        points, live, requests = self.requests(limit)
+        solved = {}
+        infos_all = []
+        for ig, reqs in requests.items():
+            ds = self.sheet_of(ig)
+            step = self.max_batch if self.max_batch > 0 else len(reqs)
+            for r0 in range(0, len(reqs), step):
+                part = reqs[r0:r0 + step]
+                gsms, infos = sheet_gsm_batch(self.ctx, ds, [b for _, b in part], self.precision)
+                for (ip, _), g, inf in zip(part, gsms, infos):
+                    solved[(ip, ig)] = g
+                infos_all += infos
+        return self.finish(points, live, solved, infos_all)
+
+    def sheet_of(self, ig):
+        """Device sheet of sheet Gblock ig."""
+        su = self.setup
+        return self._dsheet(su.junc[su.gbls[ig].j - 1] - 1)
+
+    def requests(self, limit: int = 0):
+        """The sweep's analysed points (the cut-off skip of src/PSSFSS.jl:252-255) and,
+        per sheet Gblock, the list of (point index, PointBuilder) to solve."""
         su = self.setup
         points = list(su.points())
         live = []
@@ -465,18 +512,12 @@ class SheetSolver:
                     continue
                 b = PointBuilder(region, sheet, su.uvec[ish - 1], s, pt.k0, pt.beta00)
                 requests.setdefault(ig, []).append((ip, b))
-        solved = {}
-        infos_all = []
-        for ig, reqs in requests.items():
-            ish = su.junc[su.gbls[ig].j - 1]
-            ds = self._dsheet(ish - 1)
-            step = self.max_batch if self.max_batch > 0 else len(reqs)
-            for r0 in range(0, len(reqs), step):
-                part = reqs[r0:r0 + step]
-                gsms, infos = sheet_gsm_batch(self.ctx, ds, [b for _, b in part], self.precision)
-                for (ip, _), g, inf in zip(part, gsms, infos):
-                    solved[(ip, ig)] = g
-                infos_all += infos
+        return points, live, requests
+
+    def finish(self, points, live, solved, infos_all):
+        """Cascade every analysed point from its solved sheet GSMs (keys (point,
+        Gblock)) and return (results, stats) as run() does."""
+        su = self.setup
         t0 = time.perf_counter()
         ports = [len(l.P) for l in su.layers]
         if self.cascade == "device" and live and max(ports) <= self.CASCADE_MAX_PORT:

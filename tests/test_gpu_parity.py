@@ -611,3 +611,30 @@ def test_analyze_batch_uneven_problems(ctx, pk, orc, tmp_path):
         assert len(r) == len(one) == len(ref) == 8
         assert d1 <= 1e-12 and d2 <= 1e-9
     print(f"  {st['points']} points in {st['seconds']:.3f} s; tuner {tuner.table}")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_analyze_batch_merged_size_classes(ctx, pk, orc, prec):
+    """Three seeded cross problems of different unknown counts in one size class
+    (class_ratio 10): each batch holds points of several sheets, every system padded
+    with the identity to the batch's largest unknown count, and one factorisation
+    serves them all.  Every problem's S equals its own one-at-a-time analysis (FP64:
+    1e-12; mixed: 1e-6, the refinement stopping where it does for each system) and
+    the oracle (FP64 1e-9, mixed 1e-6); a chunk limit of 5 points makes batches cut
+    across the sheets."""
+    from pssfss_amd.batch import analyze_batch
+    probs = pk.workloads.cross_batch(nprob=3, nfreq=2, seed=5, ntri_range=(120, 260))
+    problems = [(s, [9.0, 21.0], st) for s, _, st, _ in probs]
+    _say(f"analyze_batch merged ({prec}): 3 cross problems "
+         f"({[p[0][1].nface for p in problems]} faces), 2 freqs x 4 angles, chunks of 5")
+    res, st = analyze_batch(problems, precision=prec, merge=True, class_ratio=10.0, chunk=5)
+    assert len(st["classes"]) == 1 and len(set(st["classes"][0])) > 1, st["classes"]
+    for (strata, freqs, steer), r in zip(problems, res):
+        one = pk.analyze.analyze(strata, freqs, steer, precision=prec, ctx=ctx)
+        ref = orc.analyze(pk, strata, freqs, steer)
+        d1, d2 = _max_dS(r, one), _max_dS(r, ref)
+        print(f"  vs one-at-a-time {d1:.2e}, vs oracle {d2:.2e}")
+        assert len(r) == len(one) == len(ref) == 8
+        assert d1 <= (1e-12 if prec == "fp64" else 1e-6)
+        assert d2 <= (1e-9 if prec == "fp64" else 1e-6)
+    print(f"  classes {st['classes']}: {st['points']} points in {st['seconds']:.3f} s")
