@@ -471,7 +471,10 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
 // After a leaf: its interchanges on the panel's other columns (in pivot order),
 // and the leaf's U rows for the columns to its right up to ce, the end of its
 // sub-panel (unit-lower solve with the leaf's lw x lw block).  One thread per
-// panel column.
+// panel column.  Thread 0 first resolves the lw interchanges on row indices: the
+// rows they touch (at most 2 lw) and, for each, the row whose value ends up there;
+// every column's thread then issues all its loads (a masked slot reads row c0, in
+// bounds) before it stores any value, instead of lw dependent read-modify-writes.
 template <class C, int LW>
 __global__ __launch_bounds__(NB) void k_leaf_apply(int64_t m, int kb, int ce, int c0, int lw,
                                                    C *P0, int64_t sPt, const int32_t *ipiv0,
@@ -479,32 +482,61 @@ __global__ __launch_bounds__(NB) void k_leaf_apply(int64_t m, int kb, int ce, in
     C *P = P0 + (int64_t)blockIdx.x * sPt;
     const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
     __shared__ C L[LW][LW];
+    __shared__ int trow[2 * LW], tsrc[2 * LW];
+    __shared__ int ntouch;
     const int c = threadIdx.x;
     for (int t = c; t < lw * lw; t += blockDim.x) {
         const int a = t / lw, b = t % lw;
         L[a][b] = P[(int64_t)(c0 + b) * m + c0 + a];
     }
+    if (c == 0) {
+        int cnt = 0;
+        for (int j = 0; j < lw; ++j) {
+            const int r = c0 + j, p = (int)(ipiv[k + c0 + j] - k);
+            if (p == r) continue;
+            int a = -1, b = -1;
+            for (int t = 0; t < cnt; ++t) {
+                if (trow[t] == r) a = t;
+                if (trow[t] == p) b = t;
+            }
+            if (a < 0) { trow[cnt] = r; tsrc[cnt] = r; a = cnt++; }
+            if (b < 0) { trow[cnt] = p; tsrc[cnt] = p; b = cnt++; }
+            const int x = tsrc[a];
+            tsrc[a] = tsrc[b];
+            tsrc[b] = x;
+        }
+        ntouch = cnt;
+    }
     __syncthreads();
     if (c >= kb || (c >= c0 && c < c0 + lw)) return;
     C *col = P + (int64_t)c * m;
-    for (int j = 0; j < lw; ++j) {
-        const int64_t r = c0 + j, p = (int64_t)ipiv[k + c0 + j] - k;
-        if (p != r) {
-            const C t = col[r];
-            col[r] = col[p];
-            col[p] = t;
-        }
+    const int cnt = ntouch;
+    typedef typename CTraits<C>::R Rl;
+    Rl vr[2 * LW], vi[2 * LW];      // parts apart: a complex array went to scratch in FP64
+#pragma unroll
+    for (int t = 0; t < 2 * LW; ++t) {
+        const C x = col[t < cnt ? tsrc[t] : c0];
+        vr[t] = x.x;
+        vi[t] = x.y;
     }
+#pragma unroll
+    for (int t = 0; t < 2 * LW; ++t)
+        if (t < cnt) {
+            C x;
+            x.x = vr[t];
+            x.y = vi[t];
+            col[trow[t]] = x;
+        }
     if (c >= c0 + lw && c < ce) {
         C u[LW];
 #pragma unroll
         for (int j = 0; j < LW; ++j) {
             if (j < lw) {
-                C v = col[c0 + j];
+                C w = col[c0 + j];
 #pragma unroll
-                for (int l = 0; l < j; ++l) cfma(v, -L[j][l], u[l]);
-                u[j] = v;
-                col[c0 + j] = v;
+                for (int l = 0; l < j; ++l) cfma(w, -L[j][l], u[l]);
+                u[j] = w;
+                col[c0 + j] = w;
             }
         }
     }
