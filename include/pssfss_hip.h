@@ -173,8 +173,10 @@ int pssfss_sheets_gsm_batch(pssfss_ctx *ctx, int nsheet, pssfss_sheet *const *sh
  * solve is repeated in `precision` (PSSFSS_FP64, or PSSFSS_MIXED: FP32 LU with FP64
  * refinement); the adjoint systems Z^T lambda = g use the same factors (refined in
  * FP64 against Z in the mixed form); the fill is recomputed with derivatives and
- * contracted, Z's derivative is never stored.  'J' sheets without surface
- * resistance.  grad: npts x nobj x nparam complex. */
+ * contracted, Z's derivative is never stored.  'J' sheets (surface resistance
+ * included) and 'M' sheets uploaded without face-pair classes (every pair its own
+ * class; a sheet with ufp classes is refused).  grad: npts x nobj x nparam
+ * complex. */
 int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
                          const pssfss_point *pts, int nparam, const double *vtan, int nobj,
                          const double *wbar, double *grad, int precision);

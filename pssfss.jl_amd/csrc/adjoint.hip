@@ -16,6 +16,7 @@
 
 namespace {
 
+constexpr double PI_ = 3.141592653589793;
 constexpr double TWOPI = 6.283185307179586;
 constexpr double FOURPI = 12.566370614359172;
 constexpr double MU0 = 1.25663706212e-6;
@@ -252,14 +253,20 @@ __device__ cdd sigma_interp_d(const cd *tab, int M, const double *b1, const doub
     return sm;
 }
 
-// One lane per class: derivative of the nine flag-free terms w[iml][isl] along the
-// direction, contracted as  sum_{members} conj-flagged dw * sum_c A_o[m, c] X[s, c]
-// for each objective o.  Per-workgroup partial sums go to part[block][o].
+// One lane per face pair (every pair its own class): derivative of the nine
+// flag-free terms w[iml][isl] along the direction, contracted as
+//   sum conj-flagged dw * sum_c A_o[m, c] X[s, c]
+// for each objective o.  Per-workgroup partial sums go to part[block][o].  The
+// terms are k_fill_classes' (electric or magnetic, src/FillZY.jl:183-200 and
+// :385-400) in dual numbers; self pairs of a resistive face add the derivative of
+// the surface-resistance term of k_fill_gather (src/FillZY.jl:221-231), which moves
+// with the face's area and edge lengths.
+template <bool MAGNETIC>
 __global__ __launch_bounds__(256) void k_class_tan(SheetDev sh, const FillParams *fps,
                                                    const cdd *jkd, const dd *rho_r,
                                                    const dd *rinv_, const double *ftan_m,
                                                    const cd *A, int64_t sAo, const cd *X,
-                                                   int ncol, int nobj, cd *part) {
+                                                   int ncol, int nobj, int has_fr, cd *part) {
     __shared__ cd red[4][4];
     const int64_t iu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const FillParams &P = *fps;
@@ -300,10 +307,18 @@ __global__ __launch_bounds__(256) void k_class_tan(SheetDev sh, const FillParams
             rc2[i][1] = 0.5 * (rmcy - rm[i].y);
         }
         const double u = P.u;
-        const cdd phi_core = ((FOURPI * I2) + u * (ri + J)) + (P.d3 / u) * K;
-        const cd iw = cd{0.0, P.omega};
-        const cdd Phi = (P.psf_base * (1.0 / ((TWOPI * P.omega) * EPS0))) * phi_core;
-        const cd asf = (MU0 / FOURPI) * P.mut;
+        cdd phi_core, Phi;
+        cd asf;
+        if (!MAGNETIC) {
+            phi_core = ((FOURPI * I2) + u * (ri + J)) + (P.d3 / u) * K;
+            Phi = (P.psf_base * (1.0 / ((TWOPI * P.omega) * EPS0))) * phi_core;
+            asf = (MU0 / FOURPI) * P.mut;
+        } else {
+            phi_core = ((P.I2fact * I2) + u * (ri + J)) + P.KPfact * K;
+            Phi = P.psf_base * phi_core;
+            asf = (-EPS0 / PI_) * P.epsb;
+        }
+        const cd iw = MAGNETIC ? cd{0.0, -P.omega} : cd{0.0, P.omega};
         cd dw[3][3];
 #pragma unroll
         for (int isl = 0; isl < 3; ++isl) {
@@ -317,7 +332,10 @@ __global__ __launch_bounds__(256) void k_class_tan(SheetDev sh, const FillParams
                 cdd I1i = ((r0 * I1x + r1 * I1e) + r2 * I1z) - ri_ * I1;
                 cdd Ji = ((r0 * Jx + r1 * Je) + r2 * Jz) - ri_ * J;
                 cdd Ki = ((r0 * Kx + r1 * Ke) + r2 * Kz) - ri_ * K;
-                Ai[d] = asf * ((((FOURPI * I1i) + (d ? sy : sx)) + u * Ji) + (P.c3 / u) * Ki);
+                if (!MAGNETIC)
+                    Ai[d] = asf * ((((FOURPI * I1i) + (d ? sy : sx)) + u * Ji) + (P.c3 / u) * Ki);
+                else
+                    Ai[d] = asf * ((((P.I1fact * I1i) + (d ? sy : sx)) + u * Ji) + P.KFfact * Ki);
             }
 #pragma unroll
             for (int iml = 0; iml < 3; ++iml) {
@@ -325,31 +343,48 @@ __global__ __launch_bounds__(256) void k_class_tan(SheetDev sh, const FillParams
                 dw[iml][isl] = der(iw * dot - Phi);
             }
         }
-        const int64_t nmem = sh.identity ? 1 : sh.uoff[iu + 1] - sh.uoff[iu];
-        for (int64_t im_ = 0; im_ < nmem; ++im_) {
-            int64_t fm = ifm, fs = ifs;
-            if (!sh.identity) {
-                const int64_t fp = sh.umem[sh.uoff[iu] + im_];
-                fm = fp % sh.nface;
-                fs = fp / sh.nface;
-            }
-            for (int lm = 0; lm < 3; ++lm) {
-                const int m = sh.fbf[3 * fm + lm];
-                if (m < 0) continue;
-                const cd ffm = conj(P.ff[sh.feci[3 * fm + lm]]);
-                const cd hm = sh.fsg[3 * fm + lm] > 0 ? ffm : -ffm;
+        // surface resistance of a self pair: q = R / (48 area), edge lengths squared
+        // l2[i] of the edge opposite vertex i
+        double dr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (!MAGNETIC && has_fr && ifm == ifs && sh.fr[ifs] != 0) {
+            const dd r32x = rs[2].x - rs[1].x, r32y = rs[2].y - rs[1].y;
+            const dd r12x = rs[0].x - rs[1].x, r12y = rs[0].y - rs[1].y;
+            const dd r13x = rs[0].x - rs[2].x, r13y = rs[0].y - rs[2].y;
+            const dd area48 = 48.0 * dfabs(0.5 * (r32x * r12y - r32y * r12x));
+            const dd l2[3] = {r32x * r32x + r32y * r32y, r13x * r13x + r13y * r13y,
+                              r12x * r12x + r12y * r12y};
+            const dd q = sh.fr[ifs] / area48;
+#pragma unroll
+            for (int lm = 0; lm < 3; ++lm)
+#pragma unroll
                 for (int ls = 0; ls < 3; ++ls) {
-                    const int s = sh.fbf[3 * fs + ls];
-                    if (s < 0) continue;
-                    const cd ffs = P.ff[sh.feci[3 * fs + ls]];
-                    const cd hs = sh.fsg[3 * fs + ls] > 0 ? ffs : -ffs;
-                    const cd z = hm * (hs * dw[lm][ls]);
-                    for (int o = 0; o < nobj; ++o) {
-                        cd wms = {0, 0};
-                        for (int c = 0; c < ncol; ++c)
-                            wms += A[o * sAo + m + nbf * c] * X[s + nbf * c];
-                        acc[o] += z * wms;
+                    if (lm == ls) {
+                        const int nx = ls == 2 ? 0 : ls + 1, pv = ls == 0 ? 2 : ls - 1;
+                        dr[lm][ls] = (q * (3.0 * (l2[nx] + l2[pv]) - l2[ls])).d;
+                    } else {
+                        const int t3 = 3 - lm - ls;
+                        dr[lm][ls] = (q * ((l2[ls] + l2[lm]) - 3.0 * l2[t3])).d;
                     }
+                }
+        }
+        for (int lm = 0; lm < 3; ++lm) {
+            const int m = sh.fbf[3 * ifm + lm];
+            if (m < 0) continue;
+            const cd ffm = conj(P.ff[sh.feci[3 * ifm + lm]]);
+            const cd hm = sh.fsg[3 * ifm + lm] > 0 ? ffm : -ffm;
+            for (int ls = 0; ls < 3; ++ls) {
+                const int s = sh.fbf[3 * ifs + ls];
+                if (s < 0) continue;
+                const cd ffs = P.ff[sh.feci[3 * ifs + ls]];
+                const cd hs = sh.fsg[3 * ifs + ls] > 0 ? ffs : -ffs;
+                cd z = hm * (hs * dw[lm][ls]);
+                if (lm == ls) z = z + dr[lm][ls];
+                else z = z + (hs * hm) * dr[lm][ls];
+                for (int o = 0; o < nobj; ++o) {
+                    cd wms = {0, 0};
+                    for (int c = 0; c < ncol; ++c)
+                        wms += A[o * sAo + m + nbf * c] * X[s + nbf * c];
+                    acc[o] += z * wms;
                 }
             }
         }
@@ -504,12 +539,17 @@ void launch_singular_tan(const SheetDev &sh, double ul, const double *ftan_s, dd
 
 void launch_class_tan(const SheetDev &sh, const FillParams *d_fp, const cdd *jkd,
                       const dd *rho_r, const dd *rinv, const double *ftan_m, const cd *A,
-                      int64_t sAo, const cd *X, int ncol, int nobj, cd *part, cd *out,
-                      hipStream_t st) {
+                      int64_t sAo, const cd *X, int ncol, int nobj, int mag, int has_fr,
+                      cd *part, cd *out, hipStream_t st) {
     if (nobj < 1 || nobj > 4) throw ArgError("adjoint: 1..4 objectives");
+    if (!sh.identity) throw ArgError("adjoint: face-pair classes must be the identity");
     const int64_t nblk = (sh.nufp + 255) / 256;
-    hipLaunchKernelGGL(k_class_tan, dim3((unsigned)nblk), dim3(256), 0, st, sh, d_fp, jkd, rho_r,
-                       rinv, ftan_m, A, sAo, X, ncol, nobj, part);
+    if (mag)
+        hipLaunchKernelGGL(k_class_tan<true>, dim3((unsigned)nblk), dim3(256), 0, st, sh, d_fp,
+                           jkd, rho_r, rinv, ftan_m, A, sAo, X, ncol, nobj, 0, part);
+    else
+        hipLaunchKernelGGL(k_class_tan<false>, dim3((unsigned)nblk), dim3(256), 0, st, sh, d_fp,
+                           jkd, rho_r, rinv, ftan_m, A, sAo, X, ncol, nobj, has_fr, part);
     HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(k_sum_cd, dim3((unsigned)nobj), dim3(1024), 0, st, part, nblk, nobj, out);
     HIPCHECK(hipGetLastError());

@@ -1077,10 +1077,13 @@ void gsm_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, pssfs
     gsm_chunk_groups(ctx, {GsmGroup{sh, pts, gsms, infos, np}}, precision);
 }
 
-// Sensitivities of the sheet GSM along nparam geometry directions, FP64, 'J'
-// sheets.  With W the weight of each GSM entry (ocol = observed mode, col = source
-// mode), S_scat[q,qp] = -kappa_q g_q^T I_qp (eq. 6.18), Z^T lambda_q = g_q and
-//   sum W dS = sum W (-kappa_q) (dg_q^T I_qp + lambda_q^T dV_qp)
+// Sensitivities of the sheet GSM along nparam geometry directions, FP64, 'J' or 'M'
+// sheets with every face pair its own class.  With W the weight of each GSM entry
+// (ocol = observed mode, col = source mode), S_scat[q,qp] = -kappa_{q,qp} g_q^T I_qp
+// (eqs. 6.18 / 6.37: 'J' g_q = t_q . FT, kappa = tl_q / (c_q acf_q area); 'M'
+// g_q = (z x t_q) . FT, kappa = -sigma_qp tl_q c_q, sigma = +1 for a source in
+// region 1, -1 in region 2), Z^T lambda_q = g_q and
+//   sum W dS = sum W (-kappa) (dg_q^T I_qp + lambda_q^T dV_qp)
 //            + sum_{m,s} dZ_ms sum_qp A[m,qp] I[s,qp],  A = Lambda (W kappa).
 // The first line uses the transforms' derivatives (host sums), the second the
 // class terms' derivatives contracted on the device (dZ is never formed).
@@ -1095,6 +1098,7 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
     const int ncol = n1 + n2;
     const int64_t n = sh->nbf, nf = sh->nface;
     const double ul = pts[0].u / sh->upm;
+    const bool mag = sh->cls == 'M';
     // forward: fill (keeping tables), transforms, RHS, LU + solves in `precision`
     ChunkResult cr;
     TableSet tabs;
@@ -1113,12 +1117,13 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
     std::vector<double> colt(2 * np * ncol);
     for (int p = 0; p < np; ++p)
         for (int col = 0; col < ncol; ++col) {
-            col_source(pts[p], n1, col, false, colsrc[2 * (p * ncol + col)],
+            col_source(pts[p], n1, col, mag, colsrc[2 * (p * ncol + col)],
                        colsrc[2 * (p * ncol + col) + 1]);
             coljob[p * ncol + col] = jobof[p][col];
             int orr = col < n1 ? 0 : 1, q = col < n1 ? col : col - n1;
-            colt[2 * (p * ncol + col)] = pts[p].tvec[orr][2 * q];
-            colt[2 * (p * ncol + col) + 1] = pts[p].tvec[orr][2 * q + 1];
+            const double tx = pts[p].tvec[orr][2 * q], ty = pts[p].tvec[orr][2 * q + 1];
+            colt[2 * (p * ncol + col)] = mag ? -ty : tx;
+            colt[2 * (p * ncol + col) + 1] = mag ? tx : ty;
         }
     DBuf<int32_t> dcj(coljob.size());
     DBuf<cd> dcs(colsrc.size());
@@ -1170,11 +1175,12 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
     const int64_t wsz = (int64_t)ncol * ncol;
     for (int p = 0; p < np; ++p) {
         const pssfss_point &pt = pts[p];
+        // kappa_{oc,c} = kap[oc] ('J'), kap[oc] * sigma_c ('M')
         for (int oc = 0; oc < ncol; ++oc) {
             int orr = oc < n1 ? 0 : 1, q = oc < n1 ? oc : oc - n1;
             cd tl = lay(pt.tlgf, q + pt.nmax * orr);
             cd cq = lay(pt.c[orr], q);
-            kap[p * ncol + oc] = tl / ((cq * pt.acf[orr]) * area_m2);
+            kap[p * ncol + oc] = mag ? -(tl * cq) : tl / ((cq * pt.acf[orr]) * area_m2);
         }
         for (int o = 0; o < nobj; ++o) {
             const double *wb = wbar + 2 * ((int64_t)p * nobj + o) * wsz;
@@ -1186,8 +1192,9 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
                         for (int q = 0; q < nm[orr]; ++q) {
                             int oc = orr == 0 ? q : n1 + q, c = sr == 0 ? qp : n1 + qp;
                             cd w = lay(wb, (int)(off + q + (int64_t)nm[orr] * qp));
+                            const double sigma = mag && sr == 1 ? -1.0 : 1.0;
                             Wk[((size_t)p * nobj + o) * wsz + oc * ncol + c] =
-                                w * kap[p * ncol + oc];
+                                w * (sigma * kap[p * ncol + oc]);
                         }
                     off += (int64_t)nm[orr] * nm[sr];
                 }
@@ -1252,7 +1259,8 @@ void adjoint_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, i
                 launch_class_tan(sh->dev(), dfp.p + p, jkd.p + d * sjk, rrd.p + d * 2 * nu,
                                  rid.p + d * nu, ftm.p + d * sft,
                                  dA.p + (size_t)p * nobj * n * ncol, n * ncol,
-                                 ctx->wB.p + (size_t)p * n * ncol, ncol, nobj, part.p,
+                                 ctx->wB.p + (size_t)p * n * ncol, ncol, nobj, mag ? 1 : 0,
+                                 sh->has_fr ? 1 : 0, part.p,
                                  zout.p + ((size_t)d * np + p) * nobj, st);
         }
         for (int d = 0; d < nd; ++d)
@@ -1610,8 +1618,8 @@ int pssfss_sheet_adjoint(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
         if (precision != PSSFSS_FP64 && precision != PSSFSS_MIXED)
             throw ArgError("precision must be PSSFSS_FP64 or PSSFSS_MIXED");
         if (nobj < 1 || nobj > 4) throw ArgError("adjoint: 1..4 objectives");
-        if (sheet->cls != 'J') throw ArgError("adjoint: electric-current ('J') sheets only");
-        if (sheet->has_fr) throw ArgError("adjoint: resistive faces are not supported");
+        if (!sheet->identity)
+            throw ArgError("adjoint: every face pair must be its own class (no ufp classes)");
         HIPCHECK(hipSetDevice(ctx->device));
         for (int p = 0; p < npts; ++p) check_point(sheet, &pts[p], true);
         if (npts == 0 || nparam == 0) return;

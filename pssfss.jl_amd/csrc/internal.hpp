@@ -333,11 +333,12 @@ void launch_jk_tan_multi(const SheetDev &sh, double ul, const double *us1, const
 void launch_singular_tan(const SheetDev &sh, double ul, const double *ftan_s, dd *rho_r,
                          dd *rinv, hipStream_t st);
 // out[o] = sum_{m,s} dZ_ms sum_c A[o*sAo + m + nbf*c] X[s + nbf*c]; part: one slot per
-// (256-class block, objective).
+// (256-class block, objective).  mag selects the 'M' class terms; has_fr adds the
+// surface-resistance self terms ('J').  Identity face-pair classes only.
 void launch_class_tan(const SheetDev &sh, const FillParams *d_fp, const cdd *jkd,
                       const dd *rho_r, const dd *rinv, const double *ftan_m, const cd *A,
-                      int64_t sAo, const cd *X, int ncol, int nobj, cd *part, cd *out,
-                      hipStream_t st);
+                      int64_t sAo, const cd *X, int ncol, int nobj, int mag, int has_fr,
+                      cd *part, cd *out, hipStream_t st);
 void launch_bfft_tan(const SheetDev &sh, const int32_t *bf_face, const int32_t *bf_loc,
                      const BfftJob *d_jobs, int njobs, const double *ftan_m, cd *dft,
                      hipStream_t st);

@@ -28,15 +28,17 @@ from .sheets import rectstrip
 
 # ------------------------------------------------------------------ geometry
 def bilinear_patch(corners, Nx: int, Ny: int, Px: float, Py: float, units: str = "mm",
-                   sheet_class: str = "J"):
-    """Quadrilateral PEC patch meshed as the bilinear image of a rectstrip grid.
+                   sheet_class: str = "J", Rsheet: float = 0.0):
+    """Quadrilateral patch (or, for sheet_class 'M', aperture) meshed as the
+    bilinear image of a rectstrip grid; Rsheet is the surface resistance in ohms
+    per square ('J' only).
 
     corners: (4, 2) in the cell's units, counter-clockwise from the lower left.
     Returns (sheet, vtan) with vtan[k] = d(node xy)/d(parameter k), parameters
     ordered (x0, y0, x1, y1, x2, y2, x3, y3)."""
     c = np.asarray(corners, float).reshape(4, 2)
     sh = rectstrip(Lx=1.0, Ly=1.0, Px=Px, Py=Py, Nx=Nx, Ny=Ny, units=units,
-                   sheet_class=sheet_class, fufp=False)
+                   sheet_class=sheet_class, Rsheet=Rsheet, fufp=False)
     x0, y0 = 0.5 * (Px - 1.0), 0.5 * (Py - 1.0)
     xi = np.clip(sh.rho[:, 0] - x0, 0.0, 1.0)
     eta = np.clip(sh.rho[:, 1] - y0, 0.0, 1.0)
@@ -110,6 +112,16 @@ def _blocks(g: G.GSM):
 
 
 # ------------------------------------------------------------------ driver
+def _check_adjoint_sheet(sheet, rwgdat, who: str):
+    """The adjoint differentiates 'J' (resistive or not) and 'M' sheets whose face
+    pairs are each their own class: a geometry direction moves the members of a
+    translation class differently, so shared class terms have no shared derivative."""
+    if sheet.sheet_class not in ("J", "M"):
+        raise ValueError(f"{who}: 'J' or 'M' sheets")
+    if not rwgdat.ufp_identity:
+        raise ValueError(f"{who}: sheets meshed with fufp=False (no face-pair classes)")
+
+
 def gradients(strata, freqs, steering, vtan, objectives: Sequence[Tuple[int, int, int, int]],
               ctx=None, precision: str = "fp64"):
     """Derivatives of f_o = |S_ij(m, n)|^2 of the total GSM (1-based i, j, m, n) with
@@ -123,8 +135,7 @@ def gradients(strata, freqs, steering, vtan, objectives: Sequence[Tuple[int, int
     if len(su.sheets) != 1:
         raise ValueError("gradients: single-sheet structures")
     sheet = su.sheets[0]
-    if sheet.sheet_class != "J":
-        raise ValueError("gradients: electric-current ('J') sheets")
+    _check_adjoint_sheet(sheet, su.rwgdat[0], "gradients")
     ctx = ctx or hip.get_context()
     vt = np.ascontiguousarray(vtan, dtype=np.float64)
     nparam = vt.shape[0]
@@ -291,7 +302,7 @@ def gradients_stack(strata, freqs, steering, params, objective, ctx=None,
                     precision: str = "fp64", dbmin: float = None, mode_override=None):
     """Derivative of the mean over the sweep's points of objective(result) (a real
     output such as outputs.ardb(2, 1, H), evaluated on the Result of each point)
-    with respect to geometry parameters of any of the stack's 'J' sheets.
+    with respect to geometry parameters of any of the stack's 'J' or 'M' sheets.
 
     params: list of (sheet index among the stack's sheets, vtan (nparam_s, nnode, 2)).
     Every sheet is solved on the device (fill, LU, solves); the cotangent of each
@@ -307,8 +318,7 @@ def gradients_stack(strata, freqs, steering, params, objective, ctx=None,
         if g.j != 0:
             sheet_gbls.setdefault(su.junc[g.j - 1] - 1, []).append(ig)
     for ish, _ in params:
-        if su.sheets[ish].sheet_class != "J":
-            raise ValueError("gradients_stack: electric-current ('J') sheets")
+        _check_adjoint_sheet(su.sheets[ish], su.rwgdat[ish], "gradients_stack")
     # forward: every point's sheet GSMs, batched per Gblock on the device
     pts = []
     for pt in su.points():

@@ -27,9 +27,9 @@ P = 12.0
 OBJ = [(2, 1, 1, 1), (2, 1, 2, 2)]
 
 
-def _stack(pk, corners, n=5):
+def _stack(pk, corners, n=5, **kw):
     from pssfss_amd.adjoint import bilinear_patch
-    sh, vt = bilinear_patch(corners, n, n, P, P)
+    sh, vt = bilinear_patch(corners, n, n, P, P, **kw)
     L = pk.Layer.make
     strata = [L(), sh, L(width=1.0, units="mm", epsr=3.0), L(width=3.0, units="mm", epsr=1.1),
               L()]
@@ -82,9 +82,9 @@ def test_adjoint_vs_unfrozen_fd_of_gpu_path(ctx, pk):
     assert np.median(rel) < 5e-6
 
 
-def _frozen_fd(pk, orc, corners, n, params, freqs, steer, h):
+def _frozen_fd(pk, orc, corners, n, params, freqs, steer, h, **kw):
     """Central differences of the oracle with its decisions frozen at `corners`."""
-    strata, _ = _stack(pk, corners, n)
+    strata, _ = _stack(pk, corners, n, **kw)
     nufp = pk.analyze.prepare(strata, freqs, steer).rwgdat[0].nufp
     fz = orc.FrozenDecisions(nufp, len(freqs) * len(steer["theta"]) * len(steer["phi"]))
     base = _objectives(orc.analyze(pk, strata, freqs, steer, decisions=(fz, "record")))
@@ -94,10 +94,10 @@ def _frozen_fd(pk, orc, corners, n, params, freqs, steer, h):
         c2 = corners.copy().reshape(-1)
         c1[k] += h
         c2[k] -= h
-        f1 = _objectives(orc.analyze(pk, _stack(pk, c1.reshape(4, 2), n)[0], freqs, steer,
-                                     decisions=(fz, "replay")))
-        f2 = _objectives(orc.analyze(pk, _stack(pk, c2.reshape(4, 2), n)[0], freqs, steer,
-                                     decisions=(fz, "replay")))
+        f1 = _objectives(orc.analyze(pk, _stack(pk, c1.reshape(4, 2), n, **kw)[0], freqs,
+                                     steer, decisions=(fz, "replay")))
+        f2 = _objectives(orc.analyze(pk, _stack(pk, c2.reshape(4, 2), n, **kw)[0], freqs,
+                                     steer, decisions=(fz, "replay")))
         out.append((f1 - f2) / (2 * h))
     return np.stack(out, axis=-1), base
 
@@ -115,6 +115,28 @@ def test_adjoint_vs_frozen_oracle_fd(ctx, pk, orc, steer):
     assert np.abs(vals - base).max() < 1e-12
     rel = np.abs(grad - fd) / np.abs(fd).max(axis=-1, keepdims=True)
     print(f"  max |grad - FD| / max|FD| = {rel.max():.2e}, median {np.median(rel):.2e}")
+    assert rel.max() < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["aperture", "resistive"])
+def test_adjoint_magnetic_and_resistive_vs_frozen_oracle_fd(ctx, pk, orc, kind):
+    """The same corner parameters for a quadrilateral aperture ('M' sheet: the
+    magnetic class terms, sources z x t and extraction sigma tl c) and for a patch
+    of 20 ohm/square surface resistance (the self-pair resistance term moves with
+    each face's area and edge lengths), at oblique incidence; bar 1e-5."""
+    from pssfss_amd.adjoint import gradients
+    kw = {"sheet_class": "M"} if kind == "aperture" else {"Rsheet": 20.0}
+    freqs = [8.0, 12.0]
+    steer = {"theta": [20.0], "phi": [30.0]}
+    print(f"\n[case] adjoint vs frozen-decision oracle FD, 5x5 {kind}, 8 parameters",
+          flush=True)
+    strata, vt = _stack(pk, CORNERS, **kw)
+    grad, vals = gradients(strata, freqs, steer, vt, OBJ, ctx=ctx)
+    fd, base = _frozen_fd(pk, orc, CORNERS, 5, range(8), freqs, steer, 1e-6 * P, **kw)
+    assert np.abs(vals - base).max() < 1e-12
+    rel = np.abs(grad - fd) / np.abs(fd).max(axis=-1, keepdims=True)
+    print(f"  max |grad - FD| / max|FD| = {rel.max():.2e}, median {np.median(rel):.2e}; "
+          f"max|FD| = {np.abs(fd).max():.3e}")
     assert rel.max() < 1e-5
 
 

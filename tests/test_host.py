@@ -303,3 +303,22 @@ def test_sheet_data_round_trip(pk, tmp_path):
     np.savez(str(tmp_path / "other.npz"), x=np.zeros(3))
     with pytest.raises(ValueError, match="does not contain sheet data"):
         pk.sheets.read_sheet_data(str(tmp_path / "other.npz"))
+
+
+@pytest.mark.parametrize("kw, msg", [(dict(fufp=True), "fufp=False"),
+                                     (dict(fufp=False), None)])
+def test_adjoint_sheet_checks(pk, kw, msg):
+    """The adjoint takes 'J' and 'M' sheets whose face pairs are each their own
+    class; a sheet meshed with translation classes is refused before any device
+    work."""
+    from pssfss_amd.adjoint import _check_adjoint_sheet
+    from pssfss_amd.rwg import setup_rwg
+    for cls in ("J", "M"):
+        sh = pk.sheets.rectstrip(Lx=6.0, Ly=6.0, Px=12.0, Py=12.0, Nx=4, Ny=4, units="mm",
+                                 sheet_class=cls, **kw)
+        rd = setup_rwg(sh)
+        if msg is None:
+            _check_adjoint_sheet(sh, rd, "gradients")
+        else:
+            with pytest.raises(ValueError, match=msg):
+                _check_adjoint_sheet(sh, rd, "gradients")
