@@ -1376,14 +1376,16 @@ void lu_factor_la_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64
         stage_end(3, G);
         stage_flops(3, 8.0 * (double)rest * kb1 * kb * batch);
         record(ov.e_u12, G);
-        wait(P, ov.e_u12);
-        factor_panel(k1);
-        record(ov.e_panel, P);
-        if (k1 + kb1 < n) {                          // meanwhile the rest of the trailing matrix
+        // the rest of the trailing matrix, enqueued before the panel's few dozen
+        // launches so that the host's time issuing them does not hold it back
+        if (k1 + kb1 < n) {
             stage_begin(4, G);
             update_cols_t<C, 1>(n, k, kb, k1 + kb1, n, A, sA, batch, G);
             stage_end(4, G, 8.0 * (double)rest * (double)(rest - kb1) * kb * batch);
         }
+        wait(P, ov.e_u12);
+        factor_panel(k1);
+        record(ov.e_panel, P);
     }
     // every panel's interchanges on the L columns to its left
     for (int64_t k = NB; k < n; k += NB) {
