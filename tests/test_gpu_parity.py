@@ -310,6 +310,25 @@ def test_analyze_patch(ctx, pk, orc, cls):
     assert dm <= 1e-6
 
 
+def test_analyze_psi_steering_across_cutoff(ctx, pk, orc):
+    """Unit-cell phasing steering (psi1 = 150 deg, psi2 = 0; |beta00| = 261.8 rad/m for
+    the 10 mm cell): at 3 and 4 GHz the principal modes are cut off and the points
+    are skipped (src/PSSFSS.jl:252-255), at 15 and 25 GHz they propagate; the kept
+    points match the oracle's."""
+    _say("analyze, psi steering across the principal-mode cut-off, 4 freqs")
+    sh = pk.sheets.rectstrip(Lx=5, Ly=5, Px=10, Py=10, Nx=6, Ny=6, units="mm")
+    strata = [pk.Layer.make(), sh, pk.Layer.make(width=1.0, units="mm", epsr=2.5),
+              pk.Layer.make()]
+    freqs = [3.0, 4.0, 15.0, 25.0]
+    steer = {"psi1": [150.0], "psi2": [0.0]}
+    res = pk.analyze.analyze(strata, freqs, steer, precision="fp64", ctx=ctx)
+    ref = orc.analyze(pk, strata, freqs, steer)
+    assert [r["FGHz"] for r in res] == [r["FGHz"] for r in ref] == [15.0, 25.0]
+    d = _max_dS(res, ref)
+    print(f"  kept {[r['FGHz'] for r in res]} GHz; max |dS| fp64 = {d:.3e}")
+    assert d <= 1e-9
+
+
 def test_analyze_ring_oblique(ctx, pk, orc):
     _say("analyze ring on slab (nr=2, nphi=24), theta=30, phi=0/45, 3 freqs")
     strata, freqs, steer = pk.workloads.ring_on_slab(nr=2, nphi=24, nfreq=3)

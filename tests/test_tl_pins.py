@@ -335,3 +335,38 @@ def test_power_conservation_dielectric_oblique(pk, orc):
         for c in range(2):
             p = np.sum(np.abs(g.s11[:, c]) ** 2) + np.sum(np.abs(g.s21[:, c]) ** 2)
             assert abs(p - 1) < 2e-3, p
+
+
+@pytest.mark.parametrize("eps_end", [1.0, 2.2])
+def test_principal_cutoff_rule(pk, eps_end):
+    """The point-skip rule of src/PSSFSS.jl:252-255 (principal modes of both end media
+    propagating), in the package's sweep and in the oracle's own, against its closed
+    form: with unit-cell phasings (psi1, psi2) the principal mode's transverse
+    wavenumber is |beta00| = |(psi1 beta1 + psi2 beta2) / 2 pi|, and it propagates in
+    the end media (identical, src/PSSFSS.jl:619-633) iff |beta00| < k0 sqrt(eps_r mu_r)."""
+    import tl_oracle as T
+    from pssfss_amd import modes as Md
+    P = 10.0
+    sh = pk.sheets.rectstrip(Lx=5, Ly=5, Px=P, Py=P, Nx=3, Ny=3, units="mm")
+    strata = [pk.Layer.make(epsr=eps_end), sh, pk.Layer.make(width=1.0, units="mm", epsr=3.0),
+              pk.Layer.make(epsr=eps_end)]
+    freqs = [2.0, 5.0, 9.0, 14.0, 21.0, 29.0]
+    steer = {"psi1": [0.0, 45.0, 120.0, 170.0], "psi2": [0.0, 80.0]}
+    su = pk.analyze.prepare(strata, freqs, steer)
+    kept_pkg, kept_orc, expect = [], [], []
+    for pt in su.points():
+        for l in su.layers:
+            Md.setup_modes(l, pt.k0, pt.beta00)
+        kept_pkg.append(su.principal_ok())
+        b = np.hypot(math.radians(pt.steer["psi1"]), math.radians(pt.steer["psi2"])) / (P * 1e-3)
+        kmin = pt.k0 * math.sqrt(eps_end)
+        assert abs(b - kmin) > 1e-3 * kmin          # no point on the boundary
+        expect.append(b < kmin)
+    ml = [T.ModalLayer(l) for l in su.layers]
+    for fghz, k0, b00, st in T.points(su):
+        for l in ml:
+            l.setup(k0, b00)
+        kept_orc.append(T.principal_propagates(ml[0], ml[-1]))
+    assert kept_pkg == expect
+    assert kept_orc == expect
+    assert 0 < sum(expect) < len(expect)            # both outcomes occur
