@@ -92,7 +92,10 @@ typedef struct {
     int32_t fallback_fp64;      /* 1: refinement stalled, point re-solved in FP64     */
     int32_t pad;
     double refine_resid;        /* best normwise backward error of the mixed solves,
-                                   max_c |r_c| / (sqrt(n) |x_c| |A| eps); < 1 converged */
+                                   max_c |r_c| / (sqrt(n) |x_c| |A| eps); converged
+                                   below 1000 for the sheet GSM calls (their Z holds
+                                   FP32-summed spatial sums), below 1 for
+                                   pssfss_lu_solve_batch                               */
 } pssfss_point_info;
 
 /* ---- context ---------------------------------------------------------------- */
