@@ -139,16 +139,28 @@ struct pssfss_ctx {
     DBuf<double> wnorm;
     DBuf<cd> wG;                 // fill class terms
     DBuf<double> wanorm;         // infinity norms of the systems (refinement)
+    // Spectral tables are built on st_aux while the spatial sums run on st; their
+    // buffers are kept between calls, so that building them frees nothing (hipFree
+    // waits for the whole device, which would end the overlap).
+    hipStream_t st_aux = nullptr;
+    DBuf<TabParams> wtp;
+    DBuf<cd> wt1, wt2, wtS, wtA, wtF;
+    DBuf<int> wtflags, wtM, wtmag;
+    DBuf<double> wtpsi, wtarea;
+    DBuf<double> wjkpsi;                 // phasings of the spatial-sum launches
+    DBuf<unsigned long long> wjkcnt;     // their lattice-point count (stats on)
+    bool jk_cnt_pending = false;         // wjkcnt not read back yet
     std::set<pssfss_sheet *> sheets;   // sheets uploaded and not yet freed
     LuOverlap lu_ov;             // look-ahead LU streams (panel / update CU subsets)
     int chunk_max = 128;         // points per chunk of a batched sheet call
     double mem_frac = 0.45;      // share of free HBM a chunk's workspaces may take
-    const cd *dft_matrix(int M) {
+    // built on stream s on first use; callers on another stream order after s
+    const cd *dft_matrix(int M, hipStream_t s) {
         auto it = dft.find(M);
         if (it != dft.end()) return it->second.p;
         DBuf<cd> &w = dft[M];
         w.alloc((size_t)M * M);
-        launch_dft_matrix(M, w.p, st);
+        launch_dft_matrix(M, w.p, s);
         return w.p;
     }
 };
@@ -318,26 +330,29 @@ struct TableSet {
 };
 
 void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap,
-                   const cd *t1, const cd *t2, TableSet &out);
+                   const cd *t1, const cd *t2, TableSet &out, hipStream_t st);
 
 // Tables of several points: summands ring by ring until the convergence test of
 // src/PGF.jl:291-298 passes at an mmax of the loop's sequence (rings below 40
 // can never pass it, so the first test is at mmax = 80), then the 2-D DFT as two
-// GEMMs, the phase of eq. (5.32) and the ghost rows.
+// GEMMs, the phase of eq. (5.32) and the ghost rows.  Everything runs on stream st
+// and the host waits for st alone; the buffers are the context's (ctx->wt*).
 void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convtest,
-                  TableSet &out) {
-    hipStream_t st = ctx->st;
+                  TableSet &out, hipStream_t st) {
     const int np = (int)in.size();
     const auto &seq = mmax_sequence();
     std::vector<TabParams> tp(np);
     for (int p = 0; p < np; ++p) tp[p] = in[p].tp;
-    DBuf<TabParams> d_tp(np);
+    DBuf<TabParams> &d_tp = ctx->wtp;
     d_tp.upload(tp.data(), np, st);
     int rcap = 64;
     int64_t cap = (int64_t)(2 * rcap + 1) * (2 * rcap + 1);
-    DBuf<cd> t1((size_t)cap * np), t2((size_t)cap * np);
+    DBuf<cd> &t1 = ctx->wt1, &t2 = ctx->wt2;
+    t1.ensure((size_t)cap * np);
+    t2.ensure((size_t)cap * np);
     const int rstride = MMAX_LAST / 2 + 1;
-    DBuf<int> dflags((size_t)rstride * np);
+    DBuf<int> &dflags = ctx->wtflags;
+    dflags.ensure((size_t)rstride * np);
     std::vector<int> hflags((size_t)rstride * np, 0);
     std::vector<size_t> cand(np);
     std::vector<int> decided(np, 0);
@@ -367,7 +382,7 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
                                                 used * sizeof(cd), hipMemcpyDeviceToDevice, st));
                     }
                 }
-                sync(ctx);
+                HIPCHECK(hipStreamSynchronize(st));
                 t1 = std::move(n1);
                 t2 = std::move(n2);
                 rcap = nr;
@@ -378,8 +393,9 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
             launch_ringsums(d_tp.p, np, cap, computed + 1, need, rstride, t1.p, t2.p, convtest,
                             dflags.p, st);
             stage_end(1, st);
-            d2h(ctx, hflags.data(), dflags.p, hflags.size());
-            sync(ctx);
+            HIPCHECK(hipMemcpyAsync(hflags.data(), dflags.p, hflags.size() * sizeof(int),
+                                    hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
             computed = need;
         }
         bool all = true;
@@ -401,22 +417,24 @@ void build_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, double convt
         }
         if (all) break;
     }
-    finish_tables(ctx, in, cap, t1.p, t2.p, out);
+    finish_tables(ctx, in, cap, t1.p, t2.p, out, st);
 }
 
 // Tables from ring-major summands (rings 0..M/2 of each point; out.M set): the
 // square gather, the 2-D DFT as two GEMMs, the phase of eq. (5.32) and the ghost rows.
 void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap,
-                   const cd *t1, const cd *t2, TableSet &out) {
-    hipStream_t st = ctx->st;
+                   const cd *t1, const cd *t2, TableSet &out, hipStream_t st) {
     const int np = (int)in.size();
     int Mmax = 0;
     for (int p = 0; p < np; ++p) Mmax = std::max(Mmax, out.M[p]);
     out.npts = np;
     out.Mmax = Mmax;
     const int64_t sq = (int64_t)Mmax * 2 * Mmax;
-    DBuf<cd> S((size_t)sq * np), A((size_t)sq * np), F((size_t)sq * np);
-    DBuf<int> dM(np), dmag(np);
+    DBuf<cd> &S = ctx->wtS, &A = ctx->wtA, &F = ctx->wtF;
+    S.ensure((size_t)sq * np);
+    A.ensure((size_t)sq * np);
+    F.ensure((size_t)sq * np);
+    DBuf<int> &dM = ctx->wtM, &dmag = ctx->wtmag;
     std::vector<int> hmag(np);
     std::vector<double> hpsi(2 * np), harea(np);
     for (int p = 0; p < np; ++p) {
@@ -425,7 +443,7 @@ void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap
         hpsi[2 * p + 1] = in[p].psi2;
         harea[p] = in[p].area;
     }
-    DBuf<double> dpsi(2 * np), darea(np);
+    DBuf<double> &dpsi = ctx->wtpsi, &darea = ctx->wtarea;
     dM.upload(out.M.data(), np, st);
     dmag.upload(hmag.data(), np, st);
     dpsi.upload(hpsi.data(), 2 * np, st);
@@ -435,7 +453,7 @@ void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap
     const cd one = {1.0, 0.0}, zero = {0.0, 0.0};
     for (int p = 0; p < np; ++p) {
         const int M = out.M[p];
-        const cd *W = ctx->dft_matrix(M);
+        const cd *W = ctx->dft_matrix(M, st);
         zgemm_batched(M, 2 * M, M, one, W, M, 0, S.p + p * sq, M, 0, zero, A.p + p * sq, M, 0, 1,
                       st);
         zgemm_batched(M, M, M, one, A.p + p * sq, M, (int64_t)M * M, W, M, 0, zero, F.p + p * sq,
@@ -446,7 +464,7 @@ void finish_tables(pssfss_ctx *ctx, const std::vector<TabInput> &in, int64_t cap
     launch_phase_ghost(np, dM.p, dpsi.p, dmag.p, darea.p, F.p, sq, out.tabs.p, out.tab_stride,
                        Mmax, st);
     stage_end(1, st);
-    sync(ctx);   // temporaries die at return
+    HIPCHECK(hipStreamSynchronize(st));   // the tables are complete for any stream
 }
 
 // d/dk0 of the tables `val` of the points `in` at fixed phasing and u:
@@ -470,7 +488,7 @@ void build_table_dk0(pssfss_ctx *ctx, const std::vector<TabInput> &in, const Tab
     launch_summands_dk0(d_tp.p, np, cap, 0, R, t1.p, t2.p, st);
     out.M = val.M;
     out.converged = val.converged;
-    finish_tables(ctx, in, cap, t1.p, t2.p, out);
+    finish_tables(ctx, in, cap, t1.p, t2.p, out, st);
 }
 
 // --------------------------------------------------------- J/K spatial sums
@@ -489,27 +507,39 @@ void compute_jk(pssfss_ctx *ctx, pssfss_sheet *sh, double ul, const std::vector<
     const int nk = (int)psis.size() / 2;
     double us1[2] = {ul * sh->s1[0], ul * sh->s1[1]};
     double us2[2] = {ul * sh->s2[0], ul * sh->s2[1]};
-    DBuf<double> dpsi(psis.size());
+    // the host does not wait: the phasings sit in a context buffer (the next upload
+    // is ordered after this launch on the same stream)
+    DBuf<double> &dpsi = ctx->wjkpsi;
     dpsi.upload(psis.data(), psis.size(), ctx->st);
     const int64_t stride = 6 * sh->nufp;
     const int per = 1024;      // phasings per launch (grid y covers them in tiles)
     // with stats on: lattice points of the reference's sums, 8 flops each (J and K,
-    // real term times complex phase)
-    DBuf<unsigned long long> cnt(g_stage_rec ? 2 : 0);    // 16 bytes: cleared as one cd
-    if (g_stage_rec) launch_zero((cd *)cnt.p, 1, ctx->st);
+    // real term times complex phase), accumulated until flush_jk_count reads them
+    unsigned long long *cnt = nullptr;
+    if (g_stage_rec) {
+        ctx->wjkcnt.ensure(2);                               // 16 bytes: cleared as one cd
+        if (!ctx->jk_cnt_pending) launch_zero((cd *)ctx->wjkcnt.p, 1, ctx->st);
+        ctx->jk_cnt_pending = true;
+        cnt = ctx->wjkcnt.p;
+    }
     for (int g = 0; g < nk; g += per) {
         int nf = std::min(per, nk - g);
         stage_begin(0, ctx->st);
         launch_jk(sh->dev(), sh->nufp, ul, us1, us2, dpsi.p + 2 * g, nf, dst + g * stride, stride,
-                  mixed, ctx->st, cnt.p);
+                  mixed, ctx->st, cnt);
         stage_end(0, ctx->st);
     }
-    sync(ctx);
-    if (g_stage_rec) {
-        unsigned long long npts = 0;
-        HIPCHECK(hipMemcpy(&npts, cnt.p, sizeof(npts), hipMemcpyDeviceToHost));
-        stage_flops(0, 8.0 * (double)npts);
-    }
+}
+
+// The lattice-point count of the spatial sums launched since the last call (stats
+// on), read back after the context's stream has drained.
+void flush_jk_count(pssfss_ctx *ctx) {
+    if (!ctx->jk_cnt_pending) return;
+    ctx->jk_cnt_pending = false;
+    unsigned long long npts = 0;
+    HIPCHECK(hipMemcpyAsync(&npts, ctx->wjkcnt.p, sizeof(npts), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHECK(hipStreamSynchronize(ctx->st));
+    stage_flops(0, 8.0 * (double)npts);
 }
 
 bool key_matches(const pssfss_sheet *sh, bool valid, double k1, double k2, double ku,
@@ -656,8 +686,9 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     ensure_singular(ctx, sh, key_u);
     std::vector<TabInput> tin(np);
     for (int p = 0; p < np; ++p) tin[p] = sheet_tab_input(sh, &pts[p]);
+    // the tables are built on the auxiliary stream while the spatial sums run
     TableSet tabs;
-    build_tables(ctx, tin, CONVTEST_FILL, tabs);
+    build_tables(ctx, tin, CONVTEST_FILL, tabs, ctx->st_aux);
     res.M = tabs.M;
     res.conv = tabs.converged;
     if (!zbase) {
@@ -693,6 +724,7 @@ void fill_chunk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts, int 
     }
     stage_end(2, st);
     sync(ctx);
+    flush_jk_count(ctx);
     if (keep_fp) *keep_fp = fp;
     if (keep_tabs) *keep_tabs = std::move(tabs);
 }
@@ -1494,8 +1526,15 @@ int pssfss_ctx_create(int device, pssfss_ctx **out) {
         delete c;
         return PSSFSS_EHIP;
     }
+    if (hipStreamCreate(&c->st_aux) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamDestroy(c->st);
+        delete c;
+        return PSSFSS_EHIP;
+    }
     c->scratch_dev = scratch_address(device);
     if (!c->scratch_dev) {
+        (void)hipStreamDestroy(c->st_aux);
         (void)hipStreamDestroy(c->st);
         delete c;
         return PSSFSS_EHIP;
@@ -1509,11 +1548,13 @@ int pssfss_ctx_destroy(pssfss_ctx *ctx) {
     if (!ctx) return PSSFSS_EINVAL;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->st);
+    (void)hipStreamSynchronize(ctx->st_aux);
     // sheets the caller did not free go with their context
     for (pssfss_sheet *sh : ctx->sheets) delete sh;
     ctx->sheets.clear();
     ctx->dft.clear();
     destroy_overlap(ctx->lu_ov);
+    (void)hipStreamDestroy(ctx->st_aux);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return PSSFSS_OK;
@@ -1782,6 +1823,7 @@ int pssfss_filljk(pssfss_ctx *ctx, pssfss_sheet *sheet, double u_local, double p
         std::vector<cd> h((size_t)6 * nu);
         d2h(ctx, h.data(), buf.p, h.size());
         sync(ctx);
+        flush_jk_count(ctx);
         // device order -> reference class order (ufp2fp: ifm + nface*ifs)
         const int64_t nf = sheet->nface;
         std::vector<double> rr(2 * nu), ri(nu);
@@ -1810,7 +1852,7 @@ int pssfss_modal_tables(pssfss_ctx *ctx, int magnetic, const pssfss_point *pt,
         check_point(nullptr, pt, false);
         std::vector<TabInput> in = {make_tab_input(pt, magnetic != 0, beta1, beta2, beta00)};
         TableSet ts;
-        build_tables(ctx, in, convtest, ts);
+        build_tables(ctx, in, convtest, ts, ctx->st);
         const int64_t E = (int64_t)ts.M[0] + 2;
         d2h(ctx, (cd *)tab1, ts.tab1(0), (size_t)(E * E));
         d2h(ctx, (cd *)tab2, ts.tab2(0), (size_t)(E * E));
