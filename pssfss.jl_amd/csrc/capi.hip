@@ -139,6 +139,8 @@ struct pssfss_ctx {
     DBuf<double> wnorm;
     DBuf<cd> wG;                 // fill class terms
     DBuf<double> wanorm;         // infinity norms of the systems (refinement)
+    DBuf<cd> wlu;                // LU panel scratch (FP64), kept between calls
+    DBuf<cf> wluf;               // the same for the FP32 factorisation
     // Spectral tables are built on st_aux while the spatial sums run on st; their
     // buffers are kept between calls, so that building them frees nothing (hipFree
     // waits for the whole device, which would end the overlap).
@@ -842,7 +844,8 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wperm.ensure((size_t)n * np);
     if (precision == PSSFSS_FP64) {
         ctx->wT.ensure((size_t)sB * np);
-        lu_factor_batched(n, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, st, &ctx->lu_ov);
+        ctx->wlu.ensure((size_t)np * lu_factor_ws_elems(n));
+        lu_factor_batched(n, np, ctx->wZ.p, nb2, ctx->wpiv.p, n, st, &ctx->lu_ov, ctx->wlu.p);
         stage_begin(5, st);
         lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
         lu_solve_batched(n, ncol, np, ctx->wZ.p, nb2, ctx->wperm.p, n, ctx->wB.p, sB, ctx->wT.p, st);
@@ -858,7 +861,8 @@ void solve_chunk(pssfss_ctx *ctx, int64_t n, int ncol, int np, int precision,
     ctx->wanorm.ensure((size_t)np + 2);
     convert_z_to_f32_inf_norms(n, np, ctx->wZ.p, nb2, ctx->wZf.p, ctx->wanorm.p, st);
     stage_end(3, st);
-    lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st, &ctx->lu_ov);
+    ctx->wluf.ensure((size_t)np * lu_factor_ws_elems(n));
+    lu_factor_batched_f32(n, np, ctx->wZf.p, nb2, ctx->wpiv.p, n, st, &ctx->lu_ov, ctx->wluf.p);
     stage_begin(5, st);
     lu_make_perm(n, np, ctx->wpiv.p, n, ctx->wperm.p, st);
     ctx->wDinv.ensure((size_t)np * 2 * ((n + 63) / 64) * 4096);

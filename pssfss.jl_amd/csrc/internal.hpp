@@ -182,11 +182,14 @@ struct LuOverlap {
     hipEvent_t e_start = nullptr, e_u12 = nullptr, e_panel = nullptr, e_done = nullptr;
     int max_batch = 0;
 };
+// ws (optional): batch * lu_factor_ws_elems(n) entries of caller-kept scratch (the
+// call then neither allocates nor waits); without it the scratch is allocated and
+// released inside, after a wait for the factorisation.
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                       hipStream_t st, const LuOverlap *ov = nullptr);
+                       hipStream_t st, const LuOverlap *ov = nullptr, cd *ws = nullptr);
 // Scratch entries per system of lu_factor_batched_ws, and the same factorisation
-// with that scratch passed in (ws holds batch * lu_factor_ws_elems(n) entries;
-// nothing is allocated).
+// (single stream) with that scratch passed in (ws holds batch * lu_factor_ws_elems(n)
+// entries; nothing is allocated).
 int64_t lu_factor_ws_elems(int64_t n);
 void lu_factor_batched_ws(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
                           cd *ws, hipStream_t st);
@@ -198,7 +201,8 @@ void lu_solve_batched(int64_t n, int nrhs, int batch, const cd *LU, int64_t sA,
 void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t *perm,
                   hipStream_t st);
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv,
-                           int64_t sP, hipStream_t st, const LuOverlap *ov = nullptr);
+                           int64_t sP, hipStream_t st, const LuOverlap *ov = nullptr,
+                           cf *ws = nullptr);
 // sel (optional): the batch lists system indices, one per entry.  tr: Z^T x = b.
 // dinv (optional, not with tr): the diagonal-block inverses of lu_diag_inverses_f32,
 // used in place of the sequential diagonal solves.

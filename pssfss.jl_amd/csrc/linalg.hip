@@ -1397,16 +1397,20 @@ void lu_factor_la_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64
     stage_flops(7, 8.0 / 3.0 * (double)n * n * n * batch);
 }
 
+// ws (optional): batch * lu_factor_ws_elems(n) entries of scratch kept by the caller;
+// without it the scratch is allocated here and released once the factorisation
+// has run (the host then waits for it).
 template <class C>
 void lu_factor_t(int64_t n, int batch, C *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                 hipStream_t st, const LuOverlap *ov = nullptr) {
-    DBuf<C> Pt((size_t)n * NB * batch), T((size_t)SB * n * batch), Linv((size_t)2 * 4096 * batch);
+                 hipStream_t st, const LuOverlap *ov = nullptr, C *ws = nullptr) {
+    DBuf<C> own(ws ? 0 : (size_t)batch * lu_factor_ws_elems(n));
+    C *Pt = ws ? ws : own.p;
+    C *T = Pt + n * NB * batch, *Linv = T + (int64_t)SB * n * batch;
     if (ov && ov->panel && batch <= ov->max_batch && n > NB)
-        lu_factor_la_t<C>(n, batch, A, sA, ipiv, sP, Pt.p, T.p, Linv.p, st, *ov);
+        lu_factor_la_t<C>(n, batch, A, sA, ipiv, sP, Pt, T, Linv, st, *ov);
     else
-        lu_factor_ws_t<C>(n, batch, A, sA, ipiv, sP, Pt.p, T.p, Linv.p, st);
-    // the scratch is released once the factorisation has run
-    HIPCHECK(hipStreamSynchronize(st));
+        lu_factor_ws_t<C>(n, batch, A, sA, ipiv, sP, Pt, T, Linv, st);
+    if (!ws) HIPCHECK(hipStreamSynchronize(st));
 }
 
 // Z x = b: P B, then L y = b and U x = y: 2 + 2 ceil(n / 64) launches of
@@ -1478,8 +1482,8 @@ void cgemm_batched(int m, int n, int k, cf alpha, const cf *A, int64_t lda, int6
 }
 
 void lu_factor_batched(int64_t n, int batch, cd *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                       hipStream_t st, const LuOverlap *ov) {
-    lu_factor_t<cd>(n, batch, A, sA, ipiv, sP, st, ov);
+                       hipStream_t st, const LuOverlap *ov, cd *ws) {
+    lu_factor_t<cd>(n, batch, A, sA, ipiv, sP, st, ov, ws);
 }
 
 int64_t lu_factor_ws_elems(int64_t n) { return n * NB + (int64_t)SB * n + 2 * 4096; }
@@ -1506,8 +1510,8 @@ void lu_make_perm(int64_t n, int batch, const int32_t *ipiv, int64_t sP, int32_t
 }
 
 void lu_factor_batched_f32(int64_t n, int batch, cf *A, int64_t sA, int32_t *ipiv, int64_t sP,
-                           hipStream_t st, const LuOverlap *ov) {
-    lu_factor_t<cf>(n, batch, A, sA, ipiv, sP, st, ov);
+                           hipStream_t st, const LuOverlap *ov, cf *ws) {
+    lu_factor_t<cf>(n, batch, A, sA, ipiv, sP, st, ov, ws);
 }
 
 void lu_solve_batched_f32(int64_t n, int nrhs, int batch, const cf *LU, int64_t sA,
