@@ -149,7 +149,18 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
                            const pssfss_point *pts, pssfss_gsm *gsms, int precision,
                            pssfss_point_info *infos);
 
-/* Points of several sheets (independent problems, or several sheets of one stack)
+/* Starts, without waiting, the spatial sums and singular integrals that a following
+ * sheet GSM call over points with these keys (psi1, psi2 in radians, u in 1/m, as in
+ * pssfss_point) will read, so that the host can prepare those points meanwhile.  The
+ * sums go to the sheet's J/K cache, whose key rule is the reference's
+ * (src/FillZY.jl:97-108: phasings equal, u within 1e-4 relative); they are computed
+ * only when every point reuses the first point's key under that rule (a sweep with
+ * one key, e.g. normal incidence), otherwise the call does nothing.  No counterpart
+ * in the reference, which sums them inside fillz at the first point of a key. */
+int pssfss_sheet_prefetch_jk(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts, const double *psi1,
+                             const double *psi2, const double *u, int precision);
+
+/* Points of several sheets (independent problems, or several sheets of one stack)/* Points of several sheets (independent problems, or several sheets of one stack)
  * in shared batches: sheet g contributes npts[g] consecutive entries of pts / gsms /
  * infos.  Every point must have the same end-region mode counts.  Each batch is
  * filled per sheet, then factored and solved as one set of systems padded to the

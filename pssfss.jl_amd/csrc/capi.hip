@@ -214,7 +214,7 @@ int guarded(pssfss_ctx *ctx, F &&f) {
     } scope(ctx);
     try {
         f();
-        if (ctx->stats_on) ctx->rec.flush();
+        if (ctx->stats_on) ctx->rec.flush(false);
         ctx->err.clear();
         return PSSFSS_OK;
     } catch (const ArgError &e) {
@@ -1622,6 +1622,39 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
             int np = std::min(c, npts - p0);
             gsm_chunk(ctx, sheet, pts + p0, gsms + p0, np, precision, infos ? infos + p0 : nullptr);
         }
+    });
+}
+
+int pssfss_sheet_prefetch_jk(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts, const double *psi1,
+                             const double *psi2, const double *u, int precision) {
+    return guarded(ctx, [&] {
+        if (!sheet || npts < 0 || (npts > 0 && (!psi1 || !psi2 || !u)))
+            throw ArgError("null argument");
+        if (precision != PSSFSS_FP64 && precision != PSSFSS_MIXED)
+            throw ArgError("precision must be PSSFSS_FP64 or PSSFSS_MIXED");
+        HIPCHECK(hipSetDevice(ctx->device));
+        if (npts == 0) return;
+        for (int p = 0; p < npts; ++p)
+            if (!(u[p] > 0)) throw ArgError("u must be positive");
+        const int mixed = precision == PSSFSS_MIXED ? 1 : 0;
+        const double ul = u[0] / sheet->upm;
+        // every point must reuse the first point's key under the cache rule
+        for (int p = 1; p < npts; ++p)
+            if (!key_matches(sheet, true, psi1[0], psi2[0], ul, psi1[p], psi2[p],
+                             u[p] / sheet->upm))
+                return;
+        const bool valid = sheet->jk_valid && (mixed || !sheet->jk_mixed);
+        if (!key_matches(sheet, valid, sheet->jk_psi1, sheet->jk_psi2, sheet->jk_u, psi1[0],
+                         psi2[0], ul)) {
+            sheet->jk.ensure((size_t)6 * sheet->nufp);
+            compute_jk(ctx, sheet, ul, {psi1[0], psi2[0]}, sheet->jk.p, mixed);
+            sheet->jk_valid = true;
+            sheet->jk_mixed = mixed != 0;
+            sheet->jk_psi1 = psi1[0];
+            sheet->jk_psi2 = psi2[0];
+            sheet->jk_u = ul;
+        }
+        ensure_singular(ctx, sheet, ul);
     });
 }
 

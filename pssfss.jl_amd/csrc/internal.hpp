@@ -293,8 +293,19 @@ struct StageRecorder {
             }
         }
     }
-    void flush() {
+    // wait = false: only the records whose end event has completed (the host does
+    // not wait for work it left running, e.g. spatial sums started ahead of a call)
+    void flush(bool wait = true) {
+        std::vector<Rec> keep;
         for (Rec &r : pending) {
+            if (!wait) {
+                const hipError_t q = hipEventQuery(r.b);
+                if (q == hipErrorNotReady) {
+                    keep.push_back(r);
+                    continue;
+                }
+                HIPCHECK(q);
+            }
             HIPCHECK(hipEventSynchronize(r.b));
             float t = 0;
             HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
@@ -304,7 +315,7 @@ struct StageRecorder {
             pool.push_back(r.a);
             pool.push_back(r.b);
         }
-        pending.clear();
+        pending.swap(keep);
     }
     void reset() {
         for (int i = 0; i < NSTAGE; ++i) { ms[i] = 0; flops[i] = 0; launches[i] = 0; }

@@ -94,6 +94,7 @@ def lib():
                                               C.POINTER(PointInfo)]
         L.pssfss_sheet_gsm_batch.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.POINTER(Gsm),
                                              C.c_int, C.POINTER(PointInfo)]
+        L.pssfss_sheet_prefetch_jk.argtypes = [vp, vp, C.c_int, _dp, _dp, _dp, C.c_int]
         L.pssfss_sheet_adjoint.argtypes = [vp, vp, C.c_int, C.POINTER(Point), C.c_int, _dp,
                                            C.c_int, _dp, _dp, C.c_int]
         L.pssfss_face_pair_classes.argtypes = [vp, C.c_int64, _dp, C.c_int64, _ip,
@@ -129,7 +130,7 @@ EXPORTS = ("pssfss_ctx_create", "pssfss_ctx_destroy", "pssfss_last_error", "pssf
            "pssfss_rwgbfft", "pssfss_lu_solve_batch", "pssfss_cascade_batch",
            "pssfss_stats_enable", "pssfss_stats_read", "pssfss_sheet_adjoint",
            "pssfss_face_pair_classes", "pssfss_fill_dk0", "pssfss_lu_solve_batch_noalloc",
-           "pssfss_ctx_set_chunk", "pssfss_sheets_gsm_batch")
+           "pssfss_ctx_set_chunk", "pssfss_sheets_gsm_batch", "pssfss_sheet_prefetch_jk")
 
 
 def _d(a):
@@ -464,6 +465,7 @@ class SheetSolver:
 
     def run(self, limit: int = 0):
         """The sweep (its first `limit` analysed points when limit > 0)."""
+        self.prefetch()
         points, live, requests = self.requests(limit)
         solved = {}
         infos_all = []
@@ -477,6 +479,26 @@ class SheetSolver:
                     solved[(ip, ig)] = g
                 infos_all += infos
         return self.finish(points, live, solved, infos_all)
+
+    def prefetch(self):
+        """Start every analysed sheet's spatial sums before the points are prepared on
+        the host (pssfss_sheet_prefetch_jk: computed when all the sweep's points share
+        one key, the phasings and u exactly as PointBuilder sets them)."""
+        su = self.setup
+        points = list(su.points())
+        if not points:
+            return
+        for ish in sorted({su.junc[g.j - 1] for g in su.gbls if g.j != 0}):
+            sheet = su.sheets[ish - 1]
+            if sheet.style == "NULL" or sheet.sheet_class not in ("J", "M"):
+                continue
+            ds = self._dsheet(ish - 1)
+            psi1 = np.array([float(p.beta00 @ sheet.s1) / sheet.upm for p in points])
+            psi2 = np.array([float(p.beta00 @ sheet.s2) / sheet.upm for p in points])
+            u = np.full(len(points), float(su.uvec[ish - 1]))
+            self.ctx.check(lib().pssfss_sheet_prefetch_jk(self.ctx.h, ds.h, len(points),
+                                                          _d(psi1), _d(psi2), _d(u),
+                                                          PRECISION[self.precision]))
 
     def sheet_of(self, ig):
         """Device sheet of sheet Gblock ig."""

@@ -90,3 +90,35 @@ def test_ctx_destroy_frees_sheets(pk):
     f1, _ = _free_bytes()
     print(f"  free before {f0 / 2**30:.3f} GiB, after {f1 / 2**30:.3f} GiB")
     assert abs(f1 - f0) <= 0.01 * f0
+
+
+@pytest.mark.parametrize("steer", [{"theta": [0.0], "phi": [0.0]},
+                                   {"theta": [20.0], "phi": [0.0]}])
+def test_prefetch_jk_is_transparent(ctx, pk, steer):
+    """pssfss_sheet_prefetch_jk starts a one-key sweep's spatial sums before the
+    points are prepared; the sweep's S must be bit-identical to a sweep that sums
+    them inside the GSM call (normal incidence: one key, prefetched; oblique: one
+    key per frequency, the call does nothing)."""
+    from pssfss_amd import hip
+    _say(f"prefetched spatial sums vs summed in the GSM call, {steer}")
+    strata, freqs, _ = pk.workloads.patch_free_space(n=8, nfreq=4)
+    su = pk.analyze.prepare(strata, freqs, steer)
+    for prec in ("fp64", "mixed"):
+        with hip.SheetSolver(ctx, su, prec) as s:
+            res_a, _ = s.run()
+        with hip.SheetSolver(ctx, su, prec) as s:
+            s.prefetch = lambda: None
+            res_b, _ = s.run()
+        d = _max_dS(res_a, res_b)
+        print(f"  {prec}: max |dS| prefetched vs not = {d:.3e}")
+        assert len(res_a) == len(res_b) == 4
+        assert d == 0.0
+    ds = hip.DeviceSheet(ctx, su.sheets[0], su.rwgdat[0])
+    try:
+        one = np.ones(1)
+        with pytest.raises(hip.HipError, match="u must be positive"):
+            ctx.check(hip.lib().pssfss_sheet_prefetch_jk(ctx.h, ds.h, 1, hip._d(one),
+                                                         hip._d(one), hip._d(-one),
+                                                         hip.PRECISION["fp64"]))
+    finally:
+        ds.free()
