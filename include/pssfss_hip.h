@@ -150,13 +150,16 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
                            pssfss_point_info *infos);
 
 /* Starts, without waiting, the spatial sums and singular integrals that a following
- * sheet GSM call over points with these keys (psi1, psi2 in radians, u in 1/m, as in
- * pssfss_point) will read, so that the host can prepare those points meanwhile.  The
- * sums go to the sheet's J/K cache, whose key rule is the reference's
- * (src/FillZY.jl:97-108: phasings equal, u within 1e-4 relative); they are computed
- * only when every point reuses the first point's key under that rule (a sweep with
- * one key, e.g. normal incidence), otherwise the call does nothing.  No counterpart
- * in the reference, which sums them inside fillz at the first point of a key. */
+ * pssfss_sheet_gsm_batch call over points with these keys (psi1, psi2 in radians, u
+ * in 1/m, as in pssfss_point) will read, so that the host can prepare those points
+ * meanwhile.  Keys follow the reference's cache rule (src/FillZY.jl:97-108: phasings
+ * equal, u within 1e-4 relative).  When every point reuses the first point's key
+ * (e.g. normal incidence) the sums go to the sheet's J/K cache.  Otherwise, when the
+ * points fit one chunk of the GSM call, the integrals of every new key go to the
+ * context's slots, and the GSM call over the same points (same sheet, precision and
+ * cached key) uses them instead of summing again; a later prefetch or GSM call
+ * replaces them.  Results are identical either way.  No counterpart in the
+ * reference, which sums them inside fillz at the first point of a key. */
 int pssfss_sheet_prefetch_jk(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts, const double *psi1,
                              const double *psi2, const double *u, int precision);
 

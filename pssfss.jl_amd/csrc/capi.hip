@@ -152,6 +152,17 @@ struct pssfss_ctx {
     DBuf<double> wjkpsi;                 // phasings of the spatial-sum launches
     DBuf<unsigned long long> wjkcnt;     // their lattice-point count (stats on)
     bool jk_cnt_pending = false;         // wjkcnt not read back yet
+    // spatial sums of a sweep's keys started ahead in wjk (pssfss_sheet_prefetch_jk):
+    // the sheet, precision, plan and the sheet's cached key they were planned from
+    struct Prefetched {
+        bool valid = false;
+        const pssfss_sheet *sheet = nullptr;
+        int mixed = 0;
+        std::vector<double> keys;
+        bool uses_old = false;
+        bool c_valid = false, c_mixed = false;
+        double c_psi1 = 0, c_psi2 = 0, c_u = 0;
+    } pf;
     std::set<pssfss_sheet *> sheets;   // sheets uploaded and not yet freed
     LuOverlap lu_ov;             // look-ahead LU streams (panel / update CU subsets)
     int chunk_max = 128;         // points per chunk of a batched sheet call
@@ -552,69 +563,106 @@ bool key_matches(const pssfss_sheet *sh, bool valid, double k1, double k2, doubl
     return !(fabs((ul - ku) / ku) > TOL_U);
 }
 
-// Resolve the J/K data every point of a chunk uses.  Keys follow the reference's
-// sequential cache: a point reuses the most recent integrals when its (psi, u)
-// pass the test of src/FillZY.jl:97-98, otherwise new integrals are computed.
-// Returns per-point device pointers; the sheet's cache ends holding the last key.
-std::vector<const cd *> resolve_jk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts,
-                                   int np, int mixed, DBuf<cd> &slots, double &key_u) {
-    std::vector<int> slot(np);
-    std::vector<double> keys;   // psi1, psi2, u per new slot
-    // an FP64 consumer treats FP32-summed integrals as a miss
+// The spatial-sum keys of a chunk's points under the sequential cache of
+// src/FillZY.jl:97-108: a point reuses the most recent integrals when its (psi, u)
+// pass the test of src/FillZY.jl:97-98, otherwise its key is new.  Starts from the
+// sheet's cached key (an FP64 consumer treats FP32-summed integrals as a miss).
+struct JkPlan {
+    std::vector<int> slot;       // per point: new-key index, -1 = the sheet's cached integrals
+    std::vector<double> keys;    // psi1, psi2, u per new key
+    bool uses_old = false;
+};
+
+JkPlan plan_jk(const pssfss_sheet *sh, int np, const double *psi1, const double *psi2,
+               const double *ul, int mixed) {
+    JkPlan pl;
+    pl.slot.resize(np);
     bool valid = sh->jk_valid && (mixed || !sh->jk_mixed);
     double k1 = sh->jk_psi1, k2 = sh->jk_psi2, ku = sh->jk_u;
-    int cur = -1;               // -1: the sheet's cached integrals
-    bool uses_old = false;
+    int cur = -1;
     for (int p = 0; p < np; ++p) {
-        double ul = pts[p].u / sh->upm;
-        if (key_matches(sh, valid, k1, k2, ku, pts[p].psi1, pts[p].psi2, ul)) {
-            slot[p] = cur;
-            if (cur < 0) uses_old = true;
+        if (key_matches(sh, valid, k1, k2, ku, psi1[p], psi2[p], ul[p])) {
+            pl.slot[p] = cur;
+            if (cur < 0) pl.uses_old = true;
         } else {
-            keys.push_back(pts[p].psi1);
-            keys.push_back(pts[p].psi2);
-            keys.push_back(ul);
-            cur = (int)keys.size() / 3 - 1;
+            pl.keys.push_back(psi1[p]);
+            pl.keys.push_back(psi2[p]);
+            pl.keys.push_back(ul[p]);
+            cur = (int)pl.keys.size() / 3 - 1;
             valid = true;
-            k1 = pts[p].psi1; k2 = pts[p].psi2; ku = ul;
-            slot[p] = cur;
+            k1 = psi1[p]; k2 = psi2[p]; ku = ul[p];
+            pl.slot[p] = cur;
         }
     }
-    key_u = slot[0] < 0 ? sh->jk_u : keys[3 * slot[0] + 2];
-    const int nnew = (int)keys.size() / 3;
+    return pl;
+}
+
+// The integrals of a plan's new keys into slots (key i at i * 6 nufp), the sheet's
+// cached integrals copied behind them when some points reuse those.  Nothing waits.
+void compute_plan(pssfss_ctx *ctx, pssfss_sheet *sh, const JkPlan &pl, int mixed,
+                  DBuf<cd> &slots) {
+    const int nnew = (int)pl.keys.size() / 3;
     const int64_t stride = 6 * sh->nufp;
-    std::vector<const cd *> out(np);
-    if (nnew == 0) {
-        for (int p = 0; p < np; ++p) out[p] = sh->jk.p;
-        return out;
-    }
-    const int total = nnew + (uses_old ? 1 : 0);
-    slots.ensure((size_t)stride * total);
-    if (uses_old)   // the cache is overwritten below; keep the old integrals for these points
+    slots.ensure((size_t)stride * (nnew + (pl.uses_old ? 1 : 0)));
+    if (pl.uses_old)   // the cache is overwritten afterwards; keep the old integrals
         HIPCHECK(hipMemcpyAsync(slots.p + nnew * stride, sh->jk.p, stride * sizeof(cd),
                                 hipMemcpyDeviceToDevice, ctx->st));
     int g0 = 0;     // consecutive keys with equal u share a launch
     while (g0 < nnew) {
         int g1 = g0;
         std::vector<double> ps;
-        while (g1 < nnew && keys[3 * g1 + 2] == keys[3 * g0 + 2]) {
-            ps.push_back(keys[3 * g1]);
-            ps.push_back(keys[3 * g1 + 1]);
+        while (g1 < nnew && pl.keys[3 * g1 + 2] == pl.keys[3 * g0 + 2]) {
+            ps.push_back(pl.keys[3 * g1]);
+            ps.push_back(pl.keys[3 * g1 + 1]);
             ++g1;
         }
-        compute_jk(ctx, sh, keys[3 * g0 + 2], ps, slots.p + g0 * stride, mixed);
+        compute_jk(ctx, sh, pl.keys[3 * g0 + 2], ps, slots.p + g0 * stride, mixed);
         g0 = g1;
     }
+}
+
+bool prefetched_matches(const pssfss_ctx *ctx, const pssfss_sheet *sh, const JkPlan &pl,
+                        int mixed) {
+    const auto &f = ctx->pf;
+    return f.valid && f.sheet == sh && f.mixed == mixed && f.keys == pl.keys &&
+           f.uses_old == pl.uses_old && f.c_valid == sh->jk_valid && f.c_mixed == sh->jk_mixed &&
+           f.c_psi1 == sh->jk_psi1 && f.c_psi2 == sh->jk_psi2 && f.c_u == sh->jk_u;
+}
+
+// Resolve the J/K data every point of a chunk uses (plan_jk), computing the new
+// keys' integrals into slots unless pssfss_sheet_prefetch_jk already started that
+// very plan there.  Returns per-point device pointers; the sheet's cache ends
+// holding the last key.
+std::vector<const cd *> resolve_jk(pssfss_ctx *ctx, pssfss_sheet *sh, const pssfss_point *pts,
+                                   int np, int mixed, DBuf<cd> &slots, double &key_u) {
+    std::vector<double> p1(np), p2(np), ul(np);
+    for (int p = 0; p < np; ++p) {
+        p1[p] = pts[p].psi1;
+        p2[p] = pts[p].psi2;
+        ul[p] = pts[p].u / sh->upm;
+    }
+    const JkPlan pl = plan_jk(sh, np, p1.data(), p2.data(), ul.data(), mixed);
+    key_u = pl.slot[0] < 0 ? sh->jk_u : pl.keys[3 * pl.slot[0] + 2];
+    const int nnew = (int)pl.keys.size() / 3;
+    const int64_t stride = 6 * sh->nufp;
+    std::vector<const cd *> out(np);
+    if (nnew == 0) {
+        for (int p = 0; p < np; ++p) out[p] = sh->jk.p;
+        return out;
+    }
+    const bool pre = &slots == &ctx->wjk && prefetched_matches(ctx, sh, pl, mixed);
+    ctx->pf.valid = false;   // slots are (re)written or consumed here
+    if (!pre) compute_plan(ctx, sh, pl, mixed, slots);
     for (int p = 0; p < np; ++p)
-        out[p] = slot[p] < 0 ? slots.p + nnew * stride : slots.p + slot[p] * stride;
+        out[p] = pl.slot[p] < 0 ? slots.p + nnew * stride : slots.p + pl.slot[p] * stride;
     sh->jk.ensure(stride);
     HIPCHECK(hipMemcpyAsync(sh->jk.p, slots.p + (nnew - 1) * stride, stride * sizeof(cd),
                             hipMemcpyDeviceToDevice, ctx->st));
     sh->jk_valid = true;
     sh->jk_mixed = mixed != 0;
-    sh->jk_psi1 = keys[3 * (nnew - 1)];
-    sh->jk_psi2 = keys[3 * (nnew - 1) + 1];
-    sh->jk_u = keys[3 * (nnew - 1) + 2];
+    sh->jk_psi1 = pl.keys[3 * (nnew - 1)];
+    sh->jk_psi2 = pl.keys[3 * (nnew - 1) + 1];
+    sh->jk_u = pl.keys[3 * (nnew - 1) + 2];
     return out;
 }
 
@@ -1603,6 +1651,7 @@ int pssfss_sheet_free(pssfss_ctx *ctx, pssfss_sheet *sheet) {
         if (!ctx->sheets.erase(sheet)) throw ArgError("sheet is not live in this context");
         HIPCHECK(hipSetDevice(ctx->device));
         sync(ctx);
+        if (ctx->pf.sheet == sheet) ctx->pf.valid = false;
         delete sheet;
     });
 }
@@ -1632,29 +1681,51 @@ int pssfss_sheet_prefetch_jk(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts, con
             throw ArgError("null argument");
         if (precision != PSSFSS_FP64 && precision != PSSFSS_MIXED)
             throw ArgError("precision must be PSSFSS_FP64 or PSSFSS_MIXED");
+        if (!ctx->sheets.count(sheet)) throw ArgError("sheet is not live in this context");
         HIPCHECK(hipSetDevice(ctx->device));
         if (npts == 0) return;
-        for (int p = 0; p < npts; ++p)
+        std::vector<double> ul(npts);
+        for (int p = 0; p < npts; ++p) {
             if (!(u[p] > 0)) throw ArgError("u must be positive");
-        const int mixed = precision == PSSFSS_MIXED ? 1 : 0;
-        const double ul = u[0] / sheet->upm;
-        // every point must reuse the first point's key under the cache rule
-        for (int p = 1; p < npts; ++p)
-            if (!key_matches(sheet, true, psi1[0], psi2[0], ul, psi1[p], psi2[p],
-                             u[p] / sheet->upm))
-                return;
-        const bool valid = sheet->jk_valid && (mixed || !sheet->jk_mixed);
-        if (!key_matches(sheet, valid, sheet->jk_psi1, sheet->jk_psi2, sheet->jk_u, psi1[0],
-                         psi2[0], ul)) {
-            sheet->jk.ensure((size_t)6 * sheet->nufp);
-            compute_jk(ctx, sheet, ul, {psi1[0], psi2[0]}, sheet->jk.p, mixed);
-            sheet->jk_valid = true;
-            sheet->jk_mixed = mixed != 0;
-            sheet->jk_psi1 = psi1[0];
-            sheet->jk_psi2 = psi2[0];
-            sheet->jk_u = ul;
+            ul[p] = u[p] / sheet->upm;
         }
-        ensure_singular(ctx, sheet, ul);
+        const int mixed = precision == PSSFSS_MIXED ? 1 : 0;
+        bool one = true;   // every point reuses the first point's key
+        for (int p = 1; p < npts && one; ++p)
+            one = key_matches(sheet, true, psi1[0], psi2[0], ul[0], psi1[p], psi2[p], ul[p]);
+        if (one) {         // into the sheet's cache, where the GSM call finds it
+            const bool valid = sheet->jk_valid && (mixed || !sheet->jk_mixed);
+            if (!key_matches(sheet, valid, sheet->jk_psi1, sheet->jk_psi2, sheet->jk_u, psi1[0],
+                             psi2[0], ul[0])) {
+                sheet->jk.ensure((size_t)6 * sheet->nufp);
+                compute_jk(ctx, sheet, ul[0], {psi1[0], psi2[0]}, sheet->jk.p, mixed);
+                sheet->jk_valid = true;
+                sheet->jk_mixed = mixed != 0;
+                sheet->jk_psi1 = psi1[0];
+                sheet->jk_psi2 = psi2[0];
+                sheet->jk_u = ul[0];
+            }
+            ensure_singular(ctx, sheet, ul[0]);
+            return;
+        }
+        // several keys: the plan of the GSM call's single chunk into the context's
+        // slots, used there when that call plans the same keys from the same cache
+        if (chunk_size(ctx, sheet, npts, precision) < npts) return;
+        const JkPlan pl = plan_jk(sheet, npts, psi1, psi2, ul.data(), mixed);
+        ctx->pf.valid = false;
+        compute_plan(ctx, sheet, pl, mixed, ctx->wjk);
+        auto &f = ctx->pf;
+        f.sheet = sheet;
+        f.mixed = mixed;
+        f.keys = pl.keys;
+        f.uses_old = pl.uses_old;
+        f.c_valid = sheet->jk_valid;
+        f.c_mixed = sheet->jk_mixed;
+        f.c_psi1 = sheet->jk_psi1;
+        f.c_psi2 = sheet->jk_psi2;
+        f.c_u = sheet->jk_u;
+        f.valid = true;
+        ensure_singular(ctx, sheet, pl.slot[0] < 0 ? sheet->jk_u : pl.keys[3 * pl.slot[0] + 2]);
     });
 }
 

@@ -465,7 +465,7 @@ class SheetSolver:
 
     def run(self, limit: int = 0):
         """The sweep (its first `limit` analysed points when limit > 0)."""
-        self.prefetch()
+        self.prefetch(limit)
         points, live, requests = self.requests(limit)
         solved = {}
         infos_all = []
@@ -480,23 +480,44 @@ class SheetSolver:
                 infos_all += infos
         return self.finish(points, live, solved, infos_all)
 
-    def prefetch(self):
+    def prefetch(self, limit: int = 0):
         """Start every analysed sheet's spatial sums before the points are prepared on
-        the host (pssfss_sheet_prefetch_jk: computed when all the sweep's points share
-        one key, the phasings and u exactly as PointBuilder sets them)."""
+        the host (pssfss_sheet_prefetch_jk), with the phasings and u of exactly the
+        points run() will pass (the cut-off skip applied, as PointBuilder sets them).
+        One key: into the sheet's cache; several: the GSM call's plan, used when all
+        of a sheet's points go to the library in one call."""
         su = self.setup
-        points = list(su.points())
-        if not points:
+        live = []
+        for pt in su.points():
+            if limit > 0 and len(live) >= limit:
+                break
+            for l in su.layers:
+                Md.setup_modes(l, pt.k0, pt.beta00)
+            if su.principal_ok():
+                live.append(pt)
+        if not live:
             return
-        for ish in sorted({su.junc[g.j - 1] for g in su.gbls if g.j != 0}):
+        if self.max_batch > 0 and self.max_batch < len(live):
+            return
+        # sheets in the order run() solves them; a plan of several keys occupies the
+        # context's one set of slots, so only the first sheet's is started
+        order = []
+        for ig, gbl in enumerate(su.gbls):
+            if gbl.j == 0 or su.gbldup[ig] > 0:
+                continue
+            ish = su.junc[gbl.j - 1]
             sheet = su.sheets[ish - 1]
-            if sheet.style == "NULL" or sheet.sheet_class not in ("J", "M"):
+            if sheet.style != "NULL" and sheet.sheet_class in ("J", "M") and ish not in order:
+                order.append(ish)
+        for k, ish in enumerate(order):
+            sheet = su.sheets[ish - 1]
+            psi1 = np.array([float(p.beta00 @ sheet.s1) / sheet.upm for p in live])
+            psi2 = np.array([float(p.beta00 @ sheet.s2) / sheet.upm for p in live])
+            if k > 0 and len(set(zip(psi1.tolist(), psi2.tolist()))) > 1:
                 continue
             ds = self._dsheet(ish - 1)
-            psi1 = np.array([float(p.beta00 @ sheet.s1) / sheet.upm for p in points])
-            psi2 = np.array([float(p.beta00 @ sheet.s2) / sheet.upm for p in points])
-            u = np.full(len(points), float(su.uvec[ish - 1]))
-            self.ctx.check(lib().pssfss_sheet_prefetch_jk(self.ctx.h, ds.h, len(points),
+            u = np.full(len(live), float(su.uvec[ish - 1]))
+            self.ctx.check(lib().pssfss_sheet_prefetch_jk(self.ctx.h, ds.h, len(live),
                                                           _d(psi1), _d(psi2), _d(u),
                                                           PRECISION[self.precision]))
 

@@ -95,24 +95,32 @@ def test_ctx_destroy_frees_sheets(pk):
 @pytest.mark.parametrize("steer", [{"theta": [0.0], "phi": [0.0]},
                                    {"theta": [20.0], "phi": [0.0]}])
 def test_prefetch_jk_is_transparent(ctx, pk, steer):
-    """pssfss_sheet_prefetch_jk starts a one-key sweep's spatial sums before the
-    points are prepared; the sweep's S must be bit-identical to a sweep that sums
-    them inside the GSM call (normal incidence: one key, prefetched; oblique: one
-    key per frequency, the call does nothing)."""
+    """pssfss_sheet_prefetch_jk starts a sweep's spatial sums before the points are
+    prepared; the sweep's S must be bit-identical to a sweep that sums them inside
+    the GSM call (normal incidence: one key, into the sheet's cache; oblique: one
+    key per frequency, the GSM call's plan in the context's slots)."""
     from pssfss_amd import hip
     _say(f"prefetched spatial sums vs summed in the GSM call, {steer}")
     strata, freqs, _ = pk.workloads.patch_free_space(n=8, nfreq=4)
     su = pk.analyze.prepare(strata, freqs, steer)
     for prec in ("fp64", "mixed"):
+        ctx.stats_read(reset=True)
+        ctx.stats_enable(True)
+        try:
+            with hip.SheetSolver(ctx, su, prec) as s:
+                res_a, _ = s.run()
+        finally:
+            ctx.stats_enable(False)
+        # one spatial-sum launch (the prefetch's): the GSM call used its integrals
+        launches = ctx.stats_read(reset=True)["spatial_sums"]["launches"]
         with hip.SheetSolver(ctx, su, prec) as s:
-            res_a, _ = s.run()
-        with hip.SheetSolver(ctx, su, prec) as s:
-            s.prefetch = lambda: None
+            s.prefetch = lambda *a, **k: None
             res_b, _ = s.run()
         d = _max_dS(res_a, res_b)
-        print(f"  {prec}: max |dS| prefetched vs not = {d:.3e}")
+        print(f"  {prec}: max |dS| prefetched vs not = {d:.3e}; spatial-sum launches {launches}")
         assert len(res_a) == len(res_b) == 4
         assert d == 0.0
+        assert launches == 1
     ds = hip.DeviceSheet(ctx, su.sheets[0], su.rwgdat[0])
     try:
         one = np.ones(1)
