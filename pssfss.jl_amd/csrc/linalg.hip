@@ -366,9 +366,17 @@ __device__ __forceinline__ void wave_argmax(Rl &v, int &i) {
 // pivots are kept in LDS and written once after the column loop: a global store
 // inside it makes the next LDS staging of its wavefront wait for the store.
 // ipiv receives absolute pivot rows (k is the panel's first row).
+// When the panel (kb columns) is wider than the leaf, the same workgroup then applies
+// the leaf's interchanges to the panel's other columns and forms the leaf's U rows
+// for the columns to its right up to ce, the end of its sub-panel (unit-lower solve
+// with the leaf's lw x lw block): one thread per panel column.  Thread 0 first
+// resolves the lw interchanges on row indices (the rows they touch, at most 2 lw, and
+// for each the row whose value ends up there), so that every column's thread issues
+// all its loads before it stores any value.
 template <class C, int R, int LW>
-__global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0, int64_t sPt,
-                                               int32_t *ipiv0, int64_t sP, int64_t k) {
+__global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, int kb, int ce,
+                                               C *P0, int64_t sPt, int32_t *ipiv0, int64_t sP,
+                                               int64_t k) {
     typedef typename CTraits<C>::R Rl;
     constexpr int NT = 1024;
     C *P = P0 + (int64_t)blockIdx.x * sPt;
@@ -378,6 +386,9 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
     __shared__ Rl rv[16];
     __shared__ int ri[16];
     __shared__ int pv[LW];
+    __shared__ C L11[LW][LW];
+    __shared__ int trow[2 * LW], tsrc[2 * LW];
+    __shared__ int ntouch;
     const int tid = threadIdx.x, lane = tid & 63;
     const C zero = CTraits<C>::zero();
     C x[R][LW];
@@ -466,33 +477,15 @@ __global__ __launch_bounds__(1024) void k_leaf(int64_t m, int c0, int lw, C *P0,
                 if (c < lw) P[(int64_t)(c0 + c) * m + c0 + i] = x[r][c];
     }
     if (tid < lw) ipiv[k + c0 + tid] = (int32_t)(k + c0 + pv[tid]);
-}
-
-// After a leaf: its interchanges on the panel's other columns (in pivot order),
-// and the leaf's U rows for the columns to its right up to ce, the end of its
-// sub-panel (unit-lower solve with the leaf's lw x lw block).  One thread per
-// panel column.  Thread 0 first resolves the lw interchanges on row indices: the
-// rows they touch (at most 2 lw) and, for each, the row whose value ends up there;
-// every column's thread then issues all its loads (a masked slot reads row c0, in
-// bounds) before it stores any value, instead of lw dependent read-modify-writes.
-template <class C, int LW>
-__global__ __launch_bounds__(NB) void k_leaf_apply(int64_t m, int kb, int ce, int c0, int lw,
-                                                   C *P0, int64_t sPt, const int32_t *ipiv0,
-                                                   int64_t sP, int64_t k) {
-    C *P = P0 + (int64_t)blockIdx.x * sPt;
-    const int32_t *ipiv = ipiv0 + (int64_t)blockIdx.x * sP;
-    __shared__ C L[LW][LW];
-    __shared__ int trow[2 * LW], tsrc[2 * LW];
-    __shared__ int ntouch;
-    const int c = threadIdx.x;
-    for (int t = c; t < lw * lw; t += blockDim.x) {
-        const int a = t / lw, b = t % lw;
-        L[a][b] = P[(int64_t)(c0 + b) * m + c0 + a];
-    }
-    if (c == 0) {
+    if (kb <= lw) return;
+    // rows 0..lw-1 of the leaf are register row 0 of threads 0..lw-1
+    if (tid < lw)
+#pragma unroll
+        for (int c = 0; c < LW; ++c) L11[tid][c] = x[0][c];
+    if (tid == 0) {
         int cnt = 0;
         for (int j = 0; j < lw; ++j) {
-            const int r = c0 + j, p = (int)(ipiv[k + c0 + j] - k);
+            const int r = c0 + j, p = c0 + pv[j];
             if (p == r) continue;
             int a = -1, b = -1;
             for (int t = 0; t < cnt; ++t) {
@@ -501,31 +494,31 @@ __global__ __launch_bounds__(NB) void k_leaf_apply(int64_t m, int kb, int ce, in
             }
             if (a < 0) { trow[cnt] = r; tsrc[cnt] = r; a = cnt++; }
             if (b < 0) { trow[cnt] = p; tsrc[cnt] = p; b = cnt++; }
-            const int x = tsrc[a];
+            const int t2 = tsrc[a];
             tsrc[a] = tsrc[b];
-            tsrc[b] = x;
+            tsrc[b] = t2;
         }
         ntouch = cnt;
     }
     __syncthreads();
+    const int c = tid;
     if (c >= kb || (c >= c0 && c < c0 + lw)) return;
     C *col = P + (int64_t)c * m;
     const int cnt = ntouch;
-    typedef typename CTraits<C>::R Rl;
     Rl vr[2 * LW], vi[2 * LW];      // parts apart: a complex array went to scratch in FP64
 #pragma unroll
     for (int t = 0; t < 2 * LW; ++t) {
-        const C x = col[t < cnt ? tsrc[t] : c0];
-        vr[t] = x.x;
-        vi[t] = x.y;
+        const C v = col[t < cnt ? tsrc[t] : c0];
+        vr[t] = v.x;
+        vi[t] = v.y;
     }
 #pragma unroll
     for (int t = 0; t < 2 * LW; ++t)
         if (t < cnt) {
-            C x;
-            x.x = vr[t];
-            x.y = vi[t];
-            col[trow[t]] = x;
+            C v;
+            v.x = vr[t];
+            v.y = vi[t];
+            col[trow[t]] = v;
         }
     if (c >= c0 + lw && c < ce) {
         C u[LW];
@@ -534,7 +527,7 @@ __global__ __launch_bounds__(NB) void k_leaf_apply(int64_t m, int kb, int ce, in
             if (j < lw) {
                 C w = col[c0 + j];
 #pragma unroll
-                for (int l = 0; l < j; ++l) cfma(w, -L[j][l], u[l]);
+                for (int l = 0; l < j; ++l) cfma(w, -L11[j][l], u[l]);
                 u[j] = w;
                 col[c0 + j] = w;
             }
@@ -1128,10 +1121,10 @@ int leaf_width(int64_t m) {
 }
 
 template <class C>
-void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, int64_t sP,
-                 int64_t k, int batch, hipStream_t st) {
+void launch_leaf(int64_t m, int c0, int lw, int kb, int ce, C *P, int64_t sPt, int32_t *ipiv,
+                 int64_t sP, int64_t k, int batch, hipStream_t st) {
     dim3 g((unsigned)batch), b(1024);
-#define LF(RR, LL) hipLaunchKernelGGL((k_leaf<C, RR, LL>), g, b, 0, st, m, c0, lw, P, sPt, ipiv, sP, k)
+#define LF(RR, LL) hipLaunchKernelGGL((k_leaf<C, RR, LL>), g, b, 0, st, m, c0, lw, kb, ce, P, sPt, ipiv, sP, k)
     constexpr bool F = sizeof(C) == sizeof(cf);
     if constexpr (F) {
         if (m <= 1024) LF(1, 16);
@@ -1149,21 +1142,6 @@ void launch_leaf(int64_t m, int c0, int lw, C *P, int64_t sPt, int32_t *ipiv, in
         else LF(12, 2);
     }
 #undef LF
-    HIPCHECK(hipGetLastError());
-}
-template <class C>
-void launch_leaf_apply(int64_t m, int kb, int ce, int c0, int lw, C *P, int64_t sPt,
-                       const int32_t *ipiv, int64_t sP, int64_t k, int batch, hipStream_t st) {
-    dim3 g((unsigned)batch), b(NB);
-    if (lw > 8)
-        hipLaunchKernelGGL((k_leaf_apply<C, 16>), g, b, 0, st, m, kb, ce, c0, lw, P, sPt, ipiv,
-                           sP, k);
-    else if (lw > 4)
-        hipLaunchKernelGGL((k_leaf_apply<C, 8>), g, b, 0, st, m, kb, ce, c0, lw, P, sPt, ipiv, sP,
-                           k);
-    else
-        hipLaunchKernelGGL((k_leaf_apply<C, 4>), g, b, 0, st, m, kb, ce, c0, lw, P, sPt, ipiv, sP,
-                           k);
     HIPCHECK(hipGetLastError());
 }
 
@@ -1280,9 +1258,7 @@ void panel_leaves(int64_t n, int64_t k, int kb, C *A, int64_t sA, int32_t *ipiv,
         }
         for (int c0 = sb0; c0 < ce; c0 += LW) {
             const int lw = std::min(LW, ce - c0);
-            launch_leaf<C>(m, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
-            if (kb > lw)
-                launch_leaf_apply<C>(m, kb, ce, c0, lw, Pt, sPt, ipiv, sP, k, batch, st);
+            launch_leaf<C>(m, c0, lw, kb, ce, Pt, sPt, ipiv, sP, k, batch, st);
             const int64_t rr = m - c0 - lw;
             const int rc = ce - c0 - lw;
             if (rr > 0 && rc > 0)
