@@ -41,6 +41,8 @@ def _cost(setup) -> float:
 class ChunkTuner:
     """Points per chunk by (precision, unknown-count class), measured once."""
 
+    TIE = 0.95      # a larger chunk within 5 % of the best rate is taken
+
     def __init__(self, candidates=(16, 32, 64), path: Optional[str] = None):
         self.candidates = tuple(int(c) for c in candidates)
         self.path = path
@@ -61,10 +63,12 @@ class ChunkTuner:
             if k in self.table:
                 return int(self.table[k]["chunk"])
         # the same first points of the sweep under every candidate chunking, each on
-        # freshly uploaded sheets (every spatial sum computed, none from a cache)
+        # freshly uploaded sheets (every spatial sum computed, none from a cache); the
+        # warm-up runs the largest candidate, so that no timed run pays for growing the
+        # context's workspaces (tens of GB of Z at the largest chunk)
         limit = max(self.candidates)
         rates = {}
-        for k_run, c in enumerate((self.candidates[0],) + self.candidates):
+        for k_run, c in enumerate((max(self.candidates),) + self.candidates):
             ctx.set_chunk(c, mem_fraction)
             with hip.SheetSolver(ctx, setup, precision) as s:
                 s.upload()
@@ -72,7 +76,10 @@ class ChunkTuner:
                 res, _ = s.run(limit=limit)
                 if k_run > 0:                       # the first run is the warm-up
                     rates[c] = len(res) / (time.perf_counter() - t0)
-        best = max(rates, key=rates.get)
+        # near-ties go to the larger chunk (fewer factorisation chains, whose panel
+        # latency a larger batch amortises); single timings differ by a few per cent
+        top = max(rates.values())
+        best = max(c for c, r in rates.items() if r >= self.TIE * top)
         with self.lock:
             self.table[k] = {"chunk": best, "points_per_s": {str(c): round(r, 3)
                                                              for c, r in rates.items()}}
