@@ -16,10 +16,15 @@ namespace {
 constexpr double TWOPI = 6.283185307179586;
 constexpr double PI_ = 3.141592653589793;
 
-// x / y for the FP32 lattice terms: the hardware reciprocal (1 ulp) and a product
-// (a correctly rounded FP32 division costs about ten instructions more).
-__device__ __forceinline__ float fdiv_rcp(float x, float y) {
-    return x * __builtin_amdgcn_rcpf(y);
+// The FP32 lattice term pair of squared distance d2 (in units of 1/u): J = e^-a / a
+// and K = e^-a with a = sqrt(d2).  One reciprocal square root gives both 1/a and
+// a = d2 / sqrt(d2), so a term takes two quarter-rate instructions (rsq, exp)
+// instead of three (sqrt, exp, rcp).
+__device__ __forceinline__ void term_pair_f32(float d2, float &j, float &k) {
+    const float ia = __builtin_amdgcn_rsqf(d2);
+    const float e = __expf(-(d2 * ia));
+    j = e * ia;
+    k = e;
 }
 
 // Summands of eqs. (5.19a,b) (electric) or (5.26a,b) (magnetic) for lattice index
@@ -406,9 +411,9 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                 double dy = ury - ((double)m * us1y + (double)n * us2y);
                 if (MIXED) {
                     float fx = (float)dx, fy = (float)dy;
-                    float a = __builtin_amdgcn_sqrtf(fx * fx + fy * fy);
-                    float e = __expf(-a);
-                    jr = fmaf(fdiv_rcp(e, a), 1.0f, jr);
+                    float ea, e;
+                    term_pair_f32(fx * fx + fy * fy, ea, e);
+                    jr = fmaf(ea, 1.0f, jr);
                     kr = fmaf(e, 1.0f, kr);
                 } else {
                     double a = sqrt(dx * dx + dy * dy);
@@ -442,9 +447,8 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                 double dy = ury - ((double)m * us1y + (double)n * us2y);
                 if (MIXED) {
                     float fx = (float)dx, fy = (float)dy;
-                    float a = __builtin_amdgcn_sqrtf(fx * fx + fy * fy);   // 1 ulp
-                    float e = __expf(-a);
-                    float ea = fdiv_rcp(e, a);
+                    float ea, e;
+                    term_pair_f32(fx * fx + fy * fy, ea, e);
                     if (r <= RT) {
                         const int off = (m + RT) * W + (n + RT);
 #pragma unroll
@@ -758,10 +762,10 @@ __device__ __forceinline__ void jk_mfma_body(SheetDev sh, int64_t nufp, double u
         if constexpr (F32) {
             const float dx = (urx - lt[h].x) + (urlx - lt[h].lx);
             const float dy = (ury - lt[h].y) + (urly - lt[h].ly);
-            const float a = __builtin_amdgcn_sqrtf(dx * dx + dy * dy);
-            const float ex = __expf(-a);
-            TL.Aj[b][kk][lane] = fdiv_rcp(ex, a);
-            TL.Ak[b][kk][lane] = ex;
+            float tj, tk;
+            term_pair_f32(dx * dx + dy * dy, tj, tk);
+            TL.Aj[b][kk][lane] = tj;
+            TL.Ak[b][kk][lane] = tk;
         } else {
             const double dx = urx - lt[h].x, dy = ury - lt[h].y;
             const double a = sqrt(dx * dx + dy * dy);
