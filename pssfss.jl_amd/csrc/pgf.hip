@@ -328,11 +328,22 @@ __global__ void k_phase_ghost(const int *Ms, const double *psis, const int *mags
 // LDS.  Each phasing keeps its own ring count (break test at 1e-8 after ring 5).
 // MIXED: ring terms (r >= 1) and ring partial sums in FP32, the extracted (0,0)
 // term and the running sums in FP64.
+template <class T>
+struct LatPtT {
+    T ox, oy;      // the lattice offset rounded to T
+    T lx, ly;      // what that rounding left out (FP32 only; zero in FP64)
+    int m, n;
+};
+typedef LatPtT<float> LatPt;
+
+// lat (MIXED): the lattice offsets of rings 1..JK_RINGMAX in ring order as FP32 hi + lo
+// pairs (k_lattice), so that a ring term's distance takes a few FP32 operations on
+// values that are uniform over the wavefront instead of FP64 products per lane.
 template <int F, bool MIXED>
 __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul, double us1x,
                                             double us1y, double us2x, double us2y,
                                             const double *psis, int nf, cd *jk,
-                                            int64_t jk_stride) {
+                                            int64_t jk_stride, const LatPt *lat) {
 #pragma clang fp contract(off)
     constexpr int RT = JK_RT, W = 2 * JK_RT + 1;
     typedef typename std::conditional<MIXED, cf, cd>::type PT;
@@ -375,6 +386,9 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
         urx = ul * (rmcx - rtx);
         ury = ul * (rmcy - rty);
     }
+    // the observation offset as an FP32 hi + lo pair (MIXED ring terms)
+    const float uxh = (float)urx, uyh = (float)ury;
+    const float uxl = (float)(urx - (double)uxh), uyl = (float)(ury - (double)uyh);
     cd js[F], ks[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
@@ -404,18 +418,21 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
         for (int r = 1; zp && r <= JK_RINGMAX && done != all; ++r) {
             typename CTraits<PT>::R jr = 0, kr = 0;
             const int len = 8 * r;
+            const int64_t rb = ring_base(r);
             for (int k = 0; k < len; ++k) {
-                int m, n;
-                ring_mn(r, k, m, n);
-                double dx = urx - ((double)m * us1x + (double)n * us2x);
-                double dy = ury - ((double)m * us1y + (double)n * us2y);
                 if (MIXED) {
-                    float fx = (float)dx, fy = (float)dy;
+                    const LatPt L = lat[rb + k];
+                    const float fx = (uxh - L.ox) + (uxl - L.lx);
+                    const float fy = (uyh - L.oy) + (uyl - L.ly);
                     float ea, e;
                     term_pair_f32(fx * fx + fy * fy, ea, e);
                     jr = fmaf(ea, 1.0f, jr);
                     kr = fmaf(e, 1.0f, kr);
                 } else {
+                    int m, n;
+                    ring_mn(r, k, m, n);
+                    double dx = urx - ((double)m * us1x + (double)n * us2x);
+                    double dy = ury - ((double)m * us1y + (double)n * us2y);
                     double a = sqrt(dx * dx + dy * dy);
                     double e = exp(-a);
                     double ia = 1.0 / a;
@@ -440,13 +457,23 @@ __global__ __launch_bounds__(256) void k_jk(SheetDev sh, int64_t nufp, double ul
                 kr[f].x = 0; kr[f].y = 0;
             }
             const int len = 8 * r;
+            const int64_t rb = ring_base(r);
             for (int k = 0; k < len; ++k) {
                 int m, n;
-                ring_mn(r, k, m, n);
-                double dx = urx - ((double)m * us1x + (double)n * us2x);
-                double dy = ury - ((double)m * us1y + (double)n * us2y);
+                double dx = 0, dy = 0;
+                float fx = 0, fy = 0;
                 if (MIXED) {
-                    float fx = (float)dx, fy = (float)dy;
+                    const LatPt L = lat[rb + k];
+                    m = L.m;
+                    n = L.n;
+                    fx = (uxh - L.ox) + (uxl - L.lx);
+                    fy = (uyh - L.oy) + (uyl - L.ly);
+                } else {
+                    ring_mn(r, k, m, n);
+                    dx = urx - ((double)m * us1x + (double)n * us2x);
+                    dy = ury - ((double)m * us1y + (double)n * us2y);
+                }
+                if (MIXED) {
                     float ea, e;
                     term_pair_f32(fx * fx + fy * fy, ea, e);
                     if (r <= RT) {
@@ -585,13 +612,6 @@ __global__ void k_sum_u64(const unsigned long long *a, int64_t n, unsigned long 
 // about 195 VGPRs, runs two waves per SIMD and measured slower on MI355X.
 constexpr int JK_MFMA_WH = 1;
 
-template <class T>
-struct LatPtT {
-    T ox, oy;      // the lattice offset rounded to T
-    T lx, ly;      // what that rounding left out (FP32 only; zero in FP64)
-    int m, n;
-};
-typedef LatPtT<float> LatPt;
 
 template <class T>
 __global__ void k_lattice(double us1x, double us1y, double us2x, double us2y, LatPtT<T> *lat) {
@@ -1070,13 +1090,26 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
     if (nf < JK_MFMA_MIN) {
         const int per = mixed ? 16 : 8;
         dim3 grid((unsigned)((nufp + 35) / 36));
+        // the mixed kernels read the lattice offsets from a table (stream-ordered
+        // allocation: nothing waits)
+        LatPt *lat = nullptr;
+        if (mixed) {
+            const int64_t E = (int64_t)(2 * JK_RINGMAX + 1) * (2 * JK_RINGMAX + 1);
+            if (hipMallocAsync((void **)&lat, (size_t)E * sizeof(LatPt), st) != hipSuccess) {
+                (void)hipGetLastError();
+                throw HipError("hipMallocAsync of the lattice offset table failed");
+            }
+            hipLaunchKernelGGL(k_lattice<float>, dim3((unsigned)((E + 255) / 256)), dim3(256), 0,
+                               st, us1[0], us1[1], us2[0], us2[1], lat);
+            HIPCHECK(hipGetLastError());
+        }
         for (int g = 0; g < nf; g += per) {
             const int n = nf - g < per ? nf - g : per;
             const double *ps = d_psi + 2 * g;
             cd *o = jk + (int64_t)g * jk_stride;
 #define JKL(FF, MX)                                                                        \
     hipLaunchKernelGGL((k_jk<FF, MX>), grid, dim3(256), 0, st, sh, nufp, ulocal, us1[0],  \
-                       us1[1], us2[0], us2[1], ps, n, o, jk_stride)
+                       us1[1], us2[0], us2[1], ps, n, o, jk_stride, lat)
             if (mixed) {
                 if (n <= 1) JKL(1, true); else if (n <= 2) JKL(2, true);
                 else if (n <= 4) JKL(4, true); else if (n <= 8) JKL(8, true);
@@ -1088,6 +1121,7 @@ void launch_jk(const SheetDev &sh, int64_t nufp, double ulocal, const double *us
 #undef JKL
             HIPCHECK(hipGetLastError());
         }
+        if (lat) HIPCHECK(hipFreeAsync(lat, st));
         return;
     }
     // phasings per workgroup: 32 JK_MFMA_WH (FP32) or 32 (FP64)
