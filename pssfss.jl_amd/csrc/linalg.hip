@@ -948,19 +948,24 @@ __global__ __launch_bounds__(256) void k_residual(int64_t n, int nrhs, const cd 
     for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int c = 0; c < NR; ++c) acc[r][c] = cd{0.0, 0.0};
+    // Every load of a step is issued before the first product: rows past n read
+    // row n - 1 and unused right-hand sides read column 0 (their sums are never
+    // written), so no load sits under a branch that would wait for it.
+    const int64_t ilast = n - 1;
     for (int64_t j = lane; j < n; j += 64) {
-        cd xv[NR];
+        cd xv[NR], av[RW];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) xv[c] = c < nrhs ? X[j + n * c] : cd{0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            if (i0 + r < n) {
-                const cd a = A[(i0 + r) * n + j];
-#pragma unroll
-                for (int c = 0; c < NR; ++c)
-                    if (c < nrhs) cfma(acc[r][c], a, xv[c]);
-            }
+        for (int c = 0; c < NR; ++c) {
+            const cd t = X[j + n * (c < nrhs ? c : 0)];
+            xv[c].x = c < nrhs ? t.x : 0.0;
+            xv[c].y = c < nrhs ? t.y : 0.0;
         }
+#pragma unroll
+        for (int r = 0; r < RW; ++r) av[r] = A[min(i0 + r, ilast) * n + j];
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) cfma(acc[r][c], av[r], xv[c]);
     }
 #pragma unroll
     for (int r = 0; r < RW; ++r)
