@@ -163,7 +163,7 @@ int pssfss_sheet_gsm_batch(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts,
 int pssfss_sheet_prefetch_jk(pssfss_ctx *ctx, pssfss_sheet *sheet, int npts, const double *psi1,
                              const double *psi2, const double *u, int precision);
 
-/* Points of several sheets (independent problems, or several sheets of one stack)/* Points of several sheets (independent problems, or several sheets of one stack)
+/* Points of several sheets (independent problems, or several sheets of one stack)
  * in shared batches: sheet g contributes npts[g] consecutive entries of pts / gsms /
  * infos.  Every point must have the same end-region mode counts.  Each batch is
  * filled per sheet, then factored and solved as one set of systems padded to the
