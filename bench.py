@@ -36,7 +36,7 @@ for _sub in ("pssfss.jl_amd", "oracle"):
 # with the 256 MiB copy of --calibrate copy: FETCH x2.00, WRITE x1.00).
 JK_MFMA_TRAFFIC_BYTES = ((2860.8 + 558.3) * 1024 * 2.0
                          + (40685465.2 + 2554272.0) * 1024 * 1.0)
-# HBM bytes per launch of the mixed LU trailing update k_gemm_mfma<cf,2,false,1> on the
+# HBM bytes per launch of the mixed LU trailing update k_gemm_mfma<cf,2,false,1,4> on the
 # two-layer patch (mean over its 35 launches per sweep), from the FETCH_SIZE / WRITE_SIZE
 # passes in profiles/two_layer_patch_traffic_pmc.md, calibrated the same way.
 TRAIL_GEMM_TRAFFIC_BYTES = 1.056e6 * 1024 * 2.0 + 8.171e5 * 1024 * 1.0
@@ -183,7 +183,7 @@ def main():
     st = stages["mixed"]
     g = st["lu_gemm"]
     g_ach = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
-    roof = {"bound": "mfma", "kernel": "k_gemm_mfma<cf,2,false,1> (LU trailing updates, FP32 "
+    roof = {"bound": "mfma", "kernel": "k_gemm_mfma<cf,2,false,1,4> (LU trailing updates, FP32 "
             "matrix cores, three real products per complex k)",
             "achieved": round(g_ach, 3), "peak": PEAK["mixed"], "unit": "TFLOP/s",
             "frac": round(g_ach / PEAK["mixed"], 4),

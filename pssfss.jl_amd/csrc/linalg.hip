@@ -22,11 +22,14 @@ constexpr int SB = 64;   // sub-panel width: triangular blocks held in LDS
 // and each 4-deep real step covers two complex k.  The matrix cores compute the
 // transposed tile (B^T A^T), so the 16 lanes that share an output row of the
 // instruction hold 16 consecutive rows of the column-major C: the epilogue's
-// loads and stores are coalesced.  Four wavefronts in 2 x 2, each owning
-// (16 WT) x (16 WT) of the (32 WT) x (32 WT) workgroup tile; BK complex k per
-// LDS stage, double-buffered: 8 in FP64, 16 in FP32.  WT = 2 (64 x 64) is what the library
+// loads and stores are coalesced.  2 x WN wavefronts, each owning (16 WT) x
+// (32 WT / WN) of the (32 WT) x (32 WT) workgroup tile; BK complex k per LDS
+// stage, double-buffered: 8 in FP64, 16 in FP32.  WT = 2 (64 x 64) is what the library
 // launches: the 128 x 128 tile (WT = 4) runs one wavefront per SIMD and measured
-// slower on MI355X.  TA: A is given transposed (C = A^T B, A stored k x m), which
+// slower on MI355X.  FP32 launches WN = 4 (eight wavefronts of 32 x 16, 60
+// registers: eight per SIMD instead of four, which hides more of the operand and
+// C loads; 0.850 -> 0.794 ms per two-layer trailing update); FP64 keeps WN = 2 (at
+// WN = 4 it was no faster).  TA: A is given transposed (C = A^T B, A stored k x m), which
 // is a row-major m x k operand such as Z.  sel (optional) lists the batch entries.
 // FP32 uses three real products per complex k instead (Gauss): T1 = Ar Br,
 // T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi), Re C = T1 - T2, Im C = T3 - T1 - T2, with
@@ -36,8 +39,8 @@ constexpr int SB = 64;   // sub-panel width: triangular blocks held in LDS
 // the FP32 factors it builds are only used under FP64 refinement.
 // TAG only names instantiations apart in kernel traces (1: the LU's trailing
 // updates); the code is the same.
-template <class C, int WT, bool TA, int TAG = 0>
-__global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
+template <class C, int WT, bool TA, int TAG = 0, int WN = 2>
+__global__ __launch_bounds__(128 * WN) void k_gemm_mfma(int m, int n, int k, C alpha,
                                                    const C *__restrict__ A, int64_t lda,
                                                    int64_t sA, const C *__restrict__ B,
                                                    int64_t ldb, int64_t sB, C beta, C *Cm,
@@ -48,8 +51,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     // FP64: 8 complex k per LDS stage (33 KB double-buffered, three workgroups per
     // CU by registers) instead of 16 (66 KB, two workgroups per CU by LDS)
     constexpr int BM = 32 * WT, BN = 32 * WT, BK = sizeof(C) == sizeof(cd) ? 8 : 16,
-                  WS = 16 * WT;
-    constexpr int NL = BM * BK / 256;   // elements of each operand per thread and stage
+                  WS = 16 * WT, NT = 128 * WN, WSN = BN / WN, TJ = WSN / 16;
+    constexpr int NL = BM * BK / NT;   // elements of each operand per thread and stage
     constexpr bool G3 = sizeof(C) == sizeof(cf);   // three-product complex form
     constexpr int NACC = G3 ? 3 : 2;
     __shared__ C As[2][BK][BM + (TA ? 1 : 0)];
@@ -62,17 +65,17 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
     const C zero = CTraits<C>::zero();
-    V acc[WT][WT][NACC];
+    V acc[WT][TJ][NACC];
 #pragma unroll
     for (int a = 0; a < WT; ++a)
 #pragma unroll
-        for (int b = 0; b < WT; ++b)
+        for (int b = 0; b < TJ; ++b)
 #pragma unroll
             for (int c = 0; c < NACC; ++c) acc[a][b][c] = V{0, 0, 0, 0};
     C ra[NL], rb[NL];
 #define GEMM_LOAD(K0)                                                                          \
     _Pragma("unroll") for (int q = 0; q < NL; ++q) {                                          \
-        const int t = tid + 256 * q;                                                           \
+        const int t = tid + NT * q;                                                            \
         const int i = TA ? t / BK : t % BM, kk = TA ? t % BK : t / BM;                         \
         ra[q] = zero;                                                                          \
         if (m0 + i < m && (K0) + kk < k)                                                       \
@@ -83,7 +86,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
     }
 #define GEMM_STORE(BUF)                                                                        \
     _Pragma("unroll") for (int q = 0; q < NL; ++q) {                                          \
-        const int t = tid + 256 * q;                                                           \
+        const int t = tid + NT * q;                                                            \
         if (TA) As[BUF][t % BK][t / BK] = ra[q];                                               \
         else As[BUF][t / BM][t % BM] = ra[q];                                                  \
         Bs[BUF][t % BK][t / BK] = rb[q];                                                       \
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
 #pragma unroll UNR
         for (int s4 = 0; s4 < BK / 4; ++s4) {
             const int kc = 4 * s4 + (lane >> 4);
-            R ar[WT], ai[WT], as[WT], br[WT], bi[WT], bs[WT];
+            R ar[WT], ai[WT], as[WT], br[TJ], bi[TJ], bs[TJ];
 #pragma unroll
             for (int ti = 0; ti < WT; ++ti) {
                 const C a = As[buf][kc][WS * wm + 16 * ti + (lane & 15)];
@@ -112,8 +115,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                 as[ti] = a.x + a.y;
             }
 #pragma unroll
-            for (int tj = 0; tj < WT; ++tj) {
-                const C b = Bs[buf][kc][WS * wn + 16 * tj + (lane & 15)];
+            for (int tj = 0; tj < TJ; ++tj) {
+                const C b = Bs[buf][kc][WSN * wn + 16 * tj + (lane & 15)];
                 br[tj] = b.x;
                 bi[tj] = b.y;
                 bs[tj] = b.x + b.y;
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
 #pragma unroll
             for (int ti = 0; ti < WT; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < WT; ++tj) {
+                for (int tj = 0; tj < TJ; ++tj) {
                     acc[ti][tj][0] = MT::mma(br[tj], ar[ti], acc[ti][tj][0]);
                     acc[ti][tj][1] = MT::mma(bi[tj], ai[ti], acc[ti][tj][1]);
                     acc[ti][tj][2] = MT::mma(bs[tj], as[ti], acc[ti][tj][2]);
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
 #pragma unroll UNR
         for (int s2 = 0; s2 < BK / 2; ++s2) {
             const int kc = 2 * s2 + kh;
-            R are[WT], aim[WT], bop[WT];
+            R are[WT], aim[WT], bop[TJ];
 #pragma unroll
             for (int ti = 0; ti < WT; ++ti) {
                 const C a = As[buf][kc][WS * wm + 16 * ti + (lane & 15)];
@@ -139,14 +142,14 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
                 aim[ti] = c == 0 ? a.y : a.x;
             }
 #pragma unroll
-            for (int tj = 0; tj < WT; ++tj) {
-                const C b = Bs[buf][kc][WS * wn + 16 * tj + (lane & 15)];
+            for (int tj = 0; tj < TJ; ++tj) {
+                const C b = Bs[buf][kc][WSN * wn + 16 * tj + (lane & 15)];
                 bop[tj] = c == 0 ? b.x : b.y;
             }
 #pragma unroll
             for (int ti = 0; ti < WT; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < WT; ++tj) {
+                for (int tj = 0; tj < TJ; ++tj) {
                     acc[ti][tj][0] = MT::mma(bop[tj], are[ti], acc[ti][tj][0]);
                     acc[ti][tj][1] = MT::mma(bop[tj], aim[ti], acc[ti][tj][1]);
                 }
@@ -161,11 +164,11 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int k, C alpha,
 #pragma unroll
     for (int ti = 0; ti < WT; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < WT; ++tj)
+        for (int tj = 0; tj < TJ; ++tj)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int64_t i = m0 + WS * wm + 16 * ti + (lane & 15);
-                const int64_t j = n0 + WS * wn + 16 * tj + MT::row(lane, e);
+                const int64_t j = n0 + WSN * wn + 16 * tj + MT::row(lane, e);
                 if (i < m && j < n) {
                     C v;
                     if constexpr (G3) {
@@ -1103,7 +1106,8 @@ void gemm_t(int m, int n, int k, C alpha, const C *A, int64_t lda, int64_t sA, c
             hipStream_t st, const int32_t *sel = nullptr) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
     dim3 grid((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
-    hipLaunchKernelGGL((k_gemm_mfma<C, 2, TA, TAG>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, sA,
+    constexpr int WN = sizeof(C) == sizeof(cf) ? 4 : 2;
+    hipLaunchKernelGGL((k_gemm_mfma<C, 2, TA, TAG, WN>), grid, dim3(128 * WN), 0, st, m, n, k, alpha, A, lda, sA,
                        B, ldb, sB, beta, Cm, ldc, sC, sel);
     HIPCHECK(hipGetLastError());
 }
