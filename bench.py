@@ -37,9 +37,9 @@ for _sub in ("pssfss.jl_amd", "oracle"):
 JK_MFMA_TRAFFIC_BYTES = ((2860.8 + 558.3) * 1024 * 2.0
                          + (40685465.2 + 2554272.0) * 1024 * 1.0)
 # HBM bytes per launch of the mixed LU trailing update k_gemm_mfma<cf,2,false,1,4> on the
-# two-layer patch (mean over its 35 launches per sweep), from the FETCH_SIZE / WRITE_SIZE
+# two-layer patch (mean over its 34 launches per sweep), from the FETCH_SIZE / WRITE_SIZE
 # passes in profiles/two_layer_patch_traffic_pmc.md, calibrated the same way.
-TRAIL_GEMM_TRAFFIC_BYTES = 1.056e6 * 1024 * 2.0 + 8.171e5 * 1024 * 1.0
+TRAIL_GEMM_TRAFFIC_BYTES = 1.024e6 * 1024 * 2.0 + 6.930e5 * 1024 * 1.0
 PEAK = {"fp64": 78.6, "mixed": 157.3}    # TFLOP/s, MI355X dense FP64 (vector = matrix) / FP32 matrix
 
 
